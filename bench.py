@@ -1,0 +1,200 @@
+"""Benchmark: explicit MPM step throughput on one MI355X.
+
+Workload (default): the large APIC slab -- 256x128x256 = 8,388,608 particles
+on a jittered lattice in [0.1,0.9]x[0.1,0.5]x[0.1,0.9], 256^3 grid,
+quadratic B-spline, APIC, v ~ N(0, 0.5^2), E = 1.4e5 * U[1,4], dt = 2.5e-5,
+FP32, seeded synthetic data.  A step is one simulateOneStep (sort, P2G,
+grid update, G2P + F update + advection) over all particles; particles and
+grid are resident in HBM before the timed region starts.
+
+Prints ONE JSON line.  ``value`` is particle-updates/s over the timed steps.
+``roofline`` is for the dominant kernel (largest total time), with
+algorithmic bytes per launch (reads/writes of particle state and active grid
+nodes, per the byte model below) divided by its mean HIP-event duration.
+``cpu_baseline`` is the serial C++ oracle (a port of the reference step) on a
+bounded sample of the same workload.
+
+Algorithmic bytes (FP32): APIC per particle-update 364 B = P2G 148 B
+(x, v, B, Fe, Fp, m, V0, mu0, lambda0) + G2P 216 B (reads x, Fe, Fp; writes
+x, v, B, Fe, Fp); PIC-FLIP 304 B = 112 + 192.  Grid: 28 B per active node
+written by P2G and 28 B read by G2P (active = mass > 0, counted per step).
+"""
+from __future__ import annotations
+
+import argparse
+import json
+import os
+import sys
+import time
+
+ROOT = os.path.dirname(os.path.abspath(__file__))
+sys.path.insert(0, ROOT)
+
+import numpy as np  # noqa: E402
+
+HBM_PEAK = 8.0e12
+METRIC = "particle-updates/sec & %HBM roofline; batched p50/p99 step latency vs plain"
+
+BYTES = {  # per particle-update, per kernel
+    1: {"p2g": 148, "g2p": 216},   # APIC
+    0: {"p2g": 112, "g2p": 192},   # PIC-FLIP
+}
+GRID_BYTES = {"p2g": 28, "g2p": 28}
+
+
+def log(msg):
+    print(f"[bench] {msg}", file=sys.stderr, flush=True)
+
+
+def cpu_baseline(scene, seconds_target=15.0):
+    """Serial oracle on a spatial sub-box of the same workload (full grid)."""
+    from mpm_amd.solver import ParticleInput
+    from oracle import pyoracle
+
+    p = scene.particles
+    x = p.position
+    # take x,z in the low corner of the slab: ~1/16 of the particles
+    sel = np.nonzero((x[:, 0] < 0.3) & (x[:, 2] < 0.3))[0]
+    sub = p.subset(sel)
+    o = pyoracle.OracleSolver(scene.desc, sub, 32)
+    o.time_steps(1)  # warm-up step
+    steps = 0
+    t = 0.0
+    while t < seconds_target and steps < 3:
+        t += o.time_steps(1)
+        steps += 1
+    o.close()
+    return {
+        "value": sub.count * steps / t,
+        "unit": "particle-updates/s",
+        "cores": 1,
+        "kind": "port",
+        "sample": f"serial C++ oracle (FP32 restatement of the reference step), {sub.count} particles "
+                  f"(slab sub-box x,z<0.3) on the full 256^3 grid, {steps} steps after 1 warm-up, {t:.1f} s",
+    }
+
+
+def main(argv=None):
+    ap = argparse.ArgumentParser()
+    ap.add_argument("--gpus", type=int, default=1)
+    ap.add_argument("--steps", type=int, default=200)
+    ap.add_argument("--warmup", type=int, default=10)
+    ap.add_argument("--workload", default="apic_slab", choices=["apic_slab", "two_balls", "snow_cube"])
+    ap.add_argument("--graph", action="store_true", help="replay the step as a captured hipGraph")
+    ap.add_argument("--no-cpu-baseline", action="store_true")
+    ap.add_argument("--seed", type=int, default=None)
+    a = ap.parse_args(argv)
+
+    if a.gpus != 1:
+        print(f"bench.py runs on one GPU only (asked for --gpus {a.gpus}); multi-GPU is not part of this build",
+              file=sys.stderr)
+        return 2
+
+    from mpm_amd import scenes
+    from mpm_amd.solver import Solver
+
+    kw = {} if a.seed is None else {"seed": a.seed}
+    t0 = time.time()
+    sc = scenes.workload(a.workload, **kw)
+    n = sc.particles.count
+    log(f"workload {a.workload}: {n} particles, grid {sc.desc.resolution}, generated in {time.time() - t0:.1f}s")
+    t0 = time.time()
+    solver = Solver(sc.desc, sc.particles)
+    solver.synchronize()
+    log(f"created + initialized in {time.time() - t0:.1f}s")
+
+    run = solver.step_graph if a.graph else solver.step
+    for _ in range(a.warmup):
+        run(1)
+    solver.synchronize()
+    log(f"warmup {a.warmup} steps done")
+
+    # timed region: K steps; per-kernel HIP events on the solver stream
+    if not a.graph:
+        solver.profile(True)
+        solver.profile_reset()
+    s0 = solver.stats()
+    solver.synchronize()
+    t0 = time.perf_counter()
+    for _ in range(a.steps):
+        run(1)
+    solver.synchronize()
+    t1 = time.perf_counter()
+    s1 = solver.stats()
+    prof = solver.profile_read() if not a.graph else {}
+    solver.profile(False)
+    elapsed = t1 - t0
+    ms_per_step = 1e3 * elapsed / a.steps
+    value = n * a.steps / elapsed
+    act = (s1["active_nodes_sum"] - s0["active_nodes_sum"]) / a.steps
+    log(f"timed {a.steps} steps: {ms_per_step:.3f} ms/step, {value:.3e} p-u/s, mean active nodes {act:.0f}")
+
+    apic = sc.desc.transfer == 1
+    bp = BYTES[1 if apic else 0]
+    step_bytes = n * (bp["p2g"] + bp["g2p"]) + act * (GRID_BYTES["p2g"] + GRID_BYTES["g2p"])
+    step_frac = step_bytes / (elapsed / a.steps) / HBM_PEAK
+
+    roof = None
+    kernels = {}
+    if prof:
+        for k, (launches, tot) in prof.items():
+            kernels[k] = {"launches": launches, "mean_ms": tot / max(launches, 1)}
+        dom = max(prof.items(), key=lambda kv: kv[1][1])[0]
+        mean_s = kernels[dom]["mean_ms"] * 1e-3
+        if dom in bp:
+            alg = n * bp[dom] + act * GRID_BYTES[dom]
+            roof = {"bound": "hbm", "kernel": dom, "achieved": alg / mean_s / 1e9, "peak": HBM_PEAK / 1e9,
+                    "unit": "GB/s", "frac": alg / mean_s / HBM_PEAK, "traffic": None,
+                    "algorithmic_bytes": alg, "mean_ms": kernels[dom]["mean_ms"]}
+        else:
+            roof = {"bound": "hbm", "kernel": dom, "achieved": None, "peak": HBM_PEAK / 1e9, "unit": "GB/s",
+                    "frac": None, "traffic": None, "mean_ms": kernels[dom]["mean_ms"]}
+        log("kernels: " + ", ".join(f"{k}={v['mean_ms']:.3f}ms x{v['launches']}" for k, v in kernels.items()))
+
+    # per-step latency (synchronised steps)
+    lat = []
+    for _ in range(min(50, a.steps)):
+        q0 = time.perf_counter()
+        run(1)
+        solver.synchronize()
+        lat.append(1e3 * (time.perf_counter() - q0))
+    lat = np.array(lat)
+
+    cpu = None
+    if not a.no_cpu_baseline:
+        log("cpu baseline (serial oracle on a bounded sample) ...")
+        cpu = cpu_baseline(sc)
+        log(f"cpu baseline {cpu['value']:.3e} p-u/s")
+
+    out = {
+        "metric": METRIC,
+        "value": value,
+        "unit": "particle-updates/s",
+        "n_gpus": 1,
+        "steps": a.steps,
+        "warmup": a.warmup,
+        "ms_per_step": ms_per_step,
+        "higher_is_better": True,
+        "scaling": "weak",
+        "vs_baseline": None,
+        "dtype": "f32",
+        "data": "synthetic (seeded SplitMix64 scene generator)",
+        "config": {"workload": sc.name, "particles": n, "grid": list(sc.desc.resolution),
+                   "interp": "quadratic" if sc.desc.interp == 1 else ("cubic" if sc.desc.interp == 2 else "linear"),
+                   "transfer": "APIC" if apic else "PIC-FLIP", "dt": sc.desc.dt, "seed": sc.seed,
+                   "graph": bool(a.graph), "parallelism": "single GPU"},
+        "hbm_roofline_frac_step": step_frac,
+        "algorithmic_bytes_per_step": step_bytes,
+        "active_nodes_mean": act,
+        "step_latency_ms": {"p50": float(np.percentile(lat, 50)), "p99": float(np.percentile(lat, 99))},
+        "kernels": kernels,
+        "roofline": roof,
+        "cpu_baseline": cpu,
+    }
+    print(json.dumps(out), flush=True)
+    solver.close()
+    return 0
+
+
+if __name__ == "__main__":
+    sys.exit(main())

@@ -1,0 +1,243 @@
+/*
+ * mpm_hip.h -- C-ABI of the MI355X (gfx950) explicit MPM snow solver.
+ *
+ * Plain C types only (no torch, no HIP types): every entry point takes an
+ * opaque handle, plain pointers and sizes, and returns an int status
+ * (MPM_OK == 0, negative on error; the message is in mpm_last_error()).
+ * Nothing in the library calls exit().
+ *
+ * The entry points replace the reference's C++ solver surface one for one:
+ *   reference include/mpm_solver3d.h:25-74      (class MPMSolver3D)
+ *   reference include/apic_mpm_solver3d.h:24-71 (class APICMPMSolver3D)
+ * Both reference classes map onto one handle type; mpm_desc.transfer picks
+ * PIC-FLIP or APIC.  The C++ drop-in facade (namespace chains, same class,
+ * method and builder names as the reference) forwards to these functions.
+ *
+ * Threading: a handle owns its device memory and one HIP stream; calls on
+ * one handle must not overlap.  Different handles may be used from
+ * different host threads.
+ */
+#ifndef MPM_HIP_H_
+#define MPM_HIP_H_
+
+#include <stdint.h>
+
+#ifdef __cplusplus
+extern "C" {
+#endif
+
+#define MPM_API __attribute__((visibility("default")))
+
+/* ---- status codes (the reference exit()s instead: include/cuda_utils.h:6-20) */
+enum {
+    MPM_OK = 0,
+    MPM_ERR_INVALID = -1,     /* bad argument / shape / enum value          */
+    MPM_ERR_HIP = -2,         /* HIP runtime error (message has details)    */
+    MPM_ERR_OOM = -3,         /* device allocation failed / budget exceeded */
+    MPM_ERR_UNSUPPORTED = -4, /* combination the solver rejects             */
+    MPM_ERR_STATE = -5,       /* call not valid in the current state        */
+    MPM_ERR_NOT_FOUND = -6    /* unknown scene id                           */
+};
+
+/* ---- enums: same values and order as reference include/enums.h:6-15 */
+typedef enum mpm_interp {
+    MPM_INTERP_LINEAR = 0,
+    MPM_INTERP_QUADRATIC_BSPLINE = 1,
+    MPM_INTERP_CUBIC_BSPLINE = 2
+} mpm_interp;
+
+typedef enum mpm_model {
+    MPM_MODEL_FIXED_COROTATED = 0,
+    MPM_MODEL_NEO_HOOKEAN = 1
+} mpm_model;
+
+/* MPMSolver3D (PIC-FLIP) vs APICMPMSolver3D: the reference has two classes. */
+typedef enum mpm_transfer {
+    MPM_TRANSFER_PIC_FLIP = 0,
+    MPM_TRANSFER_APIC = 1
+} mpm_transfer;
+
+/* Solver description: the reference constructor arguments
+ * (include/mpm_solver3d.h:25-34, include/apic_mpm_solver3d.h:24-32)
+ * plus the gravity magnitude the reference hard-codes to 9.80665 on -y
+ * (src/mpm_solver3d.cu:397). */
+typedef struct mpm_desc {
+    float origin[3];            /* gridOrigin                                  */
+    int32_t resolution[3];      /* gridResolution (nodes per axis)             */
+    float stride;               /* gridStride h                                */
+    float friction;             /* gridBoundaryFrictionCoefficient             */
+    float blend;                /* blendCoefficient (FLIP weight), PIC-FLIP    */
+    int32_t interp;             /* mpm_interp                                  */
+    int32_t transfer;           /* mpm_transfer                                */
+    float dt;                   /* deltaTime                                   */
+    float gravity;              /* magnitude along -y; 9.80665f = reference    */
+    int32_t particle_collision; /* initial _enable_particles_collision (0)     */
+    int32_t reserved[4];        /* must be zero                                */
+} mpm_desc;
+
+/* Constructor arguments of MaterialPoint3D / APICMaterialPoint3D, one entry
+ * per particle (include/material_point3d.h:27-37).  mu0 and lambda0 are
+ * derived as the reference constructor does (src/material_point3d.cu:25-26),
+ * Fe = Fp = I, B = 0. */
+typedef struct mpm_particle_input {
+    int64_t count;
+    const float* position;             /* [3*count] x,y,z interleaved         */
+    const float* velocity;             /* [3*count]                           */
+    const float* mass;                 /* [count]                             */
+    const float* hardening;            /* [count] xi                          */
+    const float* young;                /* [count] E                           */
+    const float* poisson;              /* [count] nu                          */
+    const float* critical_compression; /* [count] theta_c                     */
+    const float* critical_stretch;     /* [count] theta_s                     */
+    const int32_t* model;              /* [count] mpm_model                   */
+} mpm_particle_input;
+
+/* Full per-particle state: the fields of MaterialPoint3D
+ * (include/material_point3d.h:15-21) plus APIC B
+ * (include/apic_material_point3d.h:23).  Matrices are row-major 3x3.
+ * Arrays are in the caller's (original) particle order.  Any pointer may be
+ * NULL on read to skip that field.  On write (mpm_create_state /
+ * mpm_write_particles) every pointer except B must be set; B may be NULL
+ * (then B = 0). */
+typedef struct mpm_particle_state {
+    int64_t count;
+    float* volume0;              /* [count]   */
+    float* mass;                 /* [count]   */
+    float* position;             /* [3*count] */
+    float* velocity;             /* [3*count] */
+    float* Fe;                   /* [9*count] deformation_gradient_elastic */
+    float* Fp;                   /* [9*count] deformation_gradient_plastic */
+    float* mu0;                  /* [count]   */
+    float* lambda0;              /* [count]   */
+    float* hardening;            /* [count]   */
+    float* critical_compression; /* [count]   */
+    float* critical_stretch;     /* [count]   */
+    int32_t* model;              /* [count]   */
+    float* B;                    /* [9*count] APIC only                    */
+} mpm_particle_state;
+
+/* Dense grid in the reference's node order, linear index
+ * i = x + rx*(y + ry*z) (src/mpm_solver3d.cu:158); fields of
+ * CollocatedGridData3D (include/grid3d.h:28-31).  NULL pointers skipped. */
+typedef struct mpm_grid_host {
+    int64_t nodes;        /* must equal rx*ry*rz */
+    float* mass;          /* [nodes]   */
+    float* velocity;      /* [3*nodes] xyz interleaved per node */
+    float* velocity_star; /* [3*nodes] */
+    float* force;         /* [3*nodes] */
+} mpm_grid_host;
+
+typedef struct mpm_stats {
+    int64_t particles;          /* N                                         */
+    int64_t steps;              /* completed simulateOneStep calls           */
+    int64_t active_nodes;       /* nodes with mass > 0 in the current grid   */
+    int64_t active_nodes_sum;   /* sum of active_nodes over all steps        */
+    int64_t work_items;         /* block work items of the last sort         */
+    int64_t escaped;            /* particles whose stencil misses the grid   */
+    int64_t device_bytes;       /* device memory owned by the handle         */
+    double last_step_ms;        /* host wall time of the last mpm_step call  */
+} mpm_stats;
+
+/* Per-kernel timing (hipEvent pairs on the handle's stream), enabled by
+ * mpm_profile_enable.  Durations are sums over launches since reset. */
+typedef struct mpm_kernel_time {
+    char name[32];
+    int64_t launches;
+    double total_ms;
+} mpm_kernel_time;
+
+/* ---------------------------------------------------------------- lifetime */
+
+/* Build a solver from constructor-style particle input and run
+ * initialize() (volume0 from grid density), as the reference constructor
+ * does (src/mpm_solver3d.cu:36-91, src/apic_mpm_solver3d.cu:36-90). */
+MPM_API int mpm_create(const mpm_desc* desc, const mpm_particle_input* particles,
+                       void** out_handle);
+
+/* Build a solver from a full particle state (checkpoint restore).  When
+ * run_initialize != 0 volume0 is recomputed as in the reference
+ * initialize(); otherwise the given volume0 is kept. */
+MPM_API int mpm_create_state(const mpm_desc* desc, const mpm_particle_state* state,
+                             int run_initialize, void** out_handle);
+
+/* Reference destructor (src/mpm_solver3d.cu:93-98). */
+MPM_API void mpm_destroy(void* handle);
+
+/* Thread-local message for the last failing call on this host thread. */
+MPM_API const char* mpm_last_error(void);
+MPM_API const char* mpm_version(void);
+
+/* -------------------------------------------------------------- time steps */
+
+/* simulateOneStep() repeated n times (src/mpm_solver3d.cu:100-125,
+ * src/apic_mpm_solver3d.cu:99-124); fused device path.  Asynchronous with
+ * respect to the host; mpm_synchronize waits. */
+MPM_API int mpm_step(void* handle, int n);
+MPM_API int mpm_synchronize(void* handle);
+
+/* Capture one simulateOneStep into a hipGraph and replay it n times
+ * (identical results to mpm_step). */
+MPM_API int mpm_step_graph(void* handle, int n);
+
+/* Sub-steps, one per reference public method (include/mpm_solver3d.h:40-63).
+ * They run the unfused device kernels and leave the grid materialised so
+ * that mpm_read_grid shows exactly what the reference grid holds. */
+MPM_API int mpm_initialize(void* handle);                         /* :42 */
+MPM_API int mpm_reset_grid(void* handle);                         /* :44 */
+MPM_API int mpm_particles_to_grid(void* handle);                  /* :46 */
+MPM_API int mpm_compute_external_forces(void* handle);            /* :48 */
+MPM_API int mpm_compute_gravity_forces(void* handle);             /* :59 */
+MPM_API int mpm_update_grid_velocities(void* handle, float dt);   /* :51 */
+MPM_API int mpm_solve_linear_system(void* handle, float dt);      /* :53 (no-op) */
+MPM_API int mpm_grid_collision(void* handle, float dt);           /* :61 */
+MPM_API int mpm_grid_to_particles(void* handle, float dt, float blend); /* :55 */
+MPM_API int mpm_particles_collision(void* handle, float dt);      /* :63 */
+MPM_API int mpm_advect_particles(void* handle, float dt);         /* :57 */
+MPM_API int mpm_switch_particles_collision(void* handle);         /* :40 */
+MPM_API int mpm_set_particles_collision(void* handle, int enable); /* field :23 */
+
+/* Particle-by-cell key generation + stable LSD radix sort (no reference
+ * counterpart: the reference scatters in array order,
+ * src/mpm_solver3d.cu:292).  Runs at the start of every step; exposed for
+ * parity tests. */
+MPM_API int mpm_sort(void* handle);
+
+/* ------------------------------------------------------------- state I/O */
+
+/* Copy particle state to the host in the original particle order (the
+ * reference never reorders: src/mpm_solver3d_builder.cu:38-65). */
+MPM_API int mpm_read_particles(void* handle, mpm_particle_state* out);
+/* Overwrite the particle state (same count as at creation). */
+MPM_API int mpm_write_particles(void* handle, const mpm_particle_state* in);
+/* Copy the dense grid (reference node order) to the host. */
+MPM_API int mpm_read_grid(void* handle, mpm_grid_host* out);
+/* Device positions packed as float4 (x,y,z,1) in original order into a
+ * host buffer of 4*count floats (reference updateGLBufferByCPU,
+ * src/mpm_solver3d.cu:484-508, without the GL buffer). */
+MPM_API int mpm_read_positions4(void* handle, float* out_xyz1);
+/* Last sort: keys[i] and original particle id ids[i] of sorted position i;
+ * seq_ids[j] = original id of the particle at input position j of that sort
+ * (the sequence the stable sort was applied to).  Any pointer may be NULL. */
+MPM_API int mpm_read_sort(void* handle, uint32_t* keys, uint32_t* ids, uint32_t* seq_ids, int64_t count);
+/* The key function used by the sort, evaluated on the host for n positions
+ * (x,y,z interleaved) under this handle's grid and interpolation. */
+MPM_API int mpm_host_keys(void* handle, const float* positions, int64_t n, uint32_t* keys);
+
+MPM_API int mpm_get_stats(void* handle, mpm_stats* out);
+
+/* ------------------------------------------------------------- profiling */
+MPM_API int mpm_profile_enable(void* handle, int enable);
+MPM_API int mpm_profile_reset(void* handle);
+/* Fills up to max entries; *count gets the number of kernels tracked. */
+MPM_API int mpm_profile_read(void* handle, mpm_kernel_time* out, int max, int* count);
+
+/* ------------------------------------------------------------- test hooks */
+/* Device 3x3 SVD (the solver's own) on n row-major matrices:
+ * A = U diag(s) V^T, U,V rotations, s[0] >= s[1] >= |s[2]|. */
+MPM_API int mpm_debug_svd3(const float* A, int64_t n, float* U, float* s, float* V);
+
+#ifdef __cplusplus
+}
+#endif
+
+#endif /* MPM_HIP_H_ */

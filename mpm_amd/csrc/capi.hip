@@ -1,0 +1,1013 @@
+// capi.hip -- host side of the C-ABI (include/mpm_hip.h).
+//
+// A handle owns the SoA particle store, the block-major grid, the sort
+// buffers and one HIP stream.  mpm_step runs the fused device path:
+//   clear touched grid blocks -> cell keys -> stable radix sort -> work
+//   items -> tiled P2G -> tiled G2P with the grid update fused into its
+//   tile load (+ F update, particle collision, advection).
+// The reference's sub-step methods run unfused kernels that materialise the
+// grid exactly as the reference leaves it, so the two paths can be compared.
+#include <hip/hip_runtime.h>
+
+#include <chrono>
+#include <cmath>
+#include <cstdio>
+#include <cstring>
+#include <map>
+#include <new>
+#include <string>
+#include <tuple>
+#include <vector>
+
+#include "devmath.h"
+#include "launch.h"
+
+using namespace mpm;
+
+namespace {
+
+thread_local std::string g_err;
+
+int fail(int code, const std::string& msg) {
+    g_err = msg;
+    return code;
+}
+
+#define HIP_TRY(call)                                                                                 \
+    do {                                                                                              \
+        hipError_t e_ = (call);                                                                       \
+        if (e_ != hipSuccess) {                                                                       \
+            return fail(MPM_ERR_HIP, std::string(#call) + ": " + hipGetErrorString(e_) + " at " +     \
+                                         __FILE__ + ":" + std::to_string(__LINE__));                  \
+        }                                                                                             \
+    } while (0)
+
+enum GridPhase { PH_CLEAR = 0, PH_MASS = 1, PH_RAW = 2, PH_SUBSTEP = 3 };
+
+struct Pending {
+    std::string name;
+    hipEvent_t a, b;
+};
+
+struct Solver {
+    mpm_desc desc{};
+    int interp = 0, apic = 0;
+    int64_t N = 0;
+    int device = 0;
+    hipStream_t st = nullptr;
+    SceneDev S{};
+    ParticlesDev P{};
+    ItemsDev I{};
+    std::vector<MaterialDev> mats;
+    MaterialDev* mats_dev = nullptr;
+    int64_t nsb = 0;
+    int key_bits = 1;
+    uint32_t *k0 = nullptr, *v0 = nullptr, *k1 = nullptr, *v1 = nullptr;
+    uint32_t* hist = nullptr;
+    uint32_t* scan_temp = nullptr;
+    uint32_t* flags = nullptr;
+    uint32_t* items = nullptr;
+    uint32_t* counters = nullptr;                 // [0] item count
+    unsigned long long* acc = nullptr;            // [0] active sum, [1] scratch, [2] escaped
+    bool sort_valid = false, grid_dirty = false, vstar_dirty = false, count_on_clear = false;
+    int phase = PH_CLEAR;
+    int64_t steps = 0;
+    int item_wgs = 1024;
+    double last_step_ms = 0.0;
+    std::vector<void*> allocs;
+    int64_t device_bytes = 0;
+    // profiling
+    bool prof = false;
+    std::vector<Pending> pending;
+    std::vector<hipEvent_t> event_pool;
+    std::map<std::string, std::pair<int64_t, double>> prof_tot;
+    std::vector<std::string> prof_order;
+    // graph
+    hipGraph_t graph = nullptr;
+    hipGraphExec_t gexec = nullptr;
+};
+
+int dalloc(Solver& s, void** p, size_t bytes) {
+    if (bytes == 0) bytes = 256;
+    hipError_t e = hipMalloc(p, bytes);
+    if (e != hipSuccess) {
+        *p = nullptr;
+        (void)hipGetLastError();
+        return fail(MPM_ERR_OOM, "hipMalloc(" + std::to_string(bytes) + " B) failed: " + hipGetErrorString(e));
+    }
+    s.allocs.push_back(*p);
+    s.device_bytes += (int64_t)bytes;
+    return MPM_OK;
+}
+
+void dfree(Solver& s, void* p) {
+    if (!p) return;
+    for (size_t i = 0; i < s.allocs.size(); ++i)
+        if (s.allocs[i] == p) { s.allocs.erase(s.allocs.begin() + (long)i); break; }
+    (void)hipFree(p);
+}
+
+// ---- profiling hook (hipEvent pairs on the solver stream)
+hipEvent_t take_event(Solver& s) {
+    if (!s.event_pool.empty()) {
+        hipEvent_t e = s.event_pool.back();
+        s.event_pool.pop_back();
+        return e;
+    }
+    hipEvent_t e = nullptr;
+    if (hipEventCreate(&e) != hipSuccess) return nullptr;
+    return e;
+}
+void prof_begin(void* ctx, const char* name) {
+    Solver& s = *static_cast<Solver*>(ctx);
+    Pending p;
+    p.name = name;
+    p.a = take_event(s);
+    p.b = take_event(s);
+    if (!p.a || !p.b) return;
+    (void)hipEventRecord(p.a, s.st);
+    s.pending.push_back(p);
+}
+void prof_end(void* ctx, const char* name) {
+    Solver& s = *static_cast<Solver*>(ctx);
+    for (size_t i = s.pending.size(); i-- > 0;)
+        if (s.pending[i].name == name) {
+            (void)hipEventRecord(s.pending[i].b, s.st);
+            return;
+        }
+}
+void prof_collect(Solver& s) {
+    if (s.pending.empty()) return;
+    (void)hipStreamSynchronize(s.st);
+    for (auto& p : s.pending) {
+        float ms = 0.0f;
+        if (hipEventElapsedTime(&ms, p.a, p.b) == hipSuccess) {
+            auto it = s.prof_tot.find(p.name);
+            if (it == s.prof_tot.end()) {
+                s.prof_order.push_back(p.name);
+                s.prof_tot[p.name] = {1, (double)ms};
+            } else {
+                it->second.first += 1;
+                it->second.second += ms;
+            }
+        }
+        s.event_pool.push_back(p.a);
+        s.event_pool.push_back(p.b);
+    }
+    s.pending.clear();
+}
+ProfileHook hook(Solver& s) {
+    ProfileHook h;
+    if (s.prof) {
+        h.ctx = &s;
+        h.fn_begin = prof_begin;
+        h.fn_end = prof_end;
+    }
+    return h;
+}
+struct Scope {
+    const ProfileHook& h;
+    const char* n;
+    Scope(const ProfileHook& h_, const char* n_) : h(h_), n(n_) { h.begin(n); }
+    ~Scope() { h.end(n); }
+};
+
+// ---- validation
+int check_desc(const mpm_desc* d) {
+    if (!d) return fail(MPM_ERR_INVALID, "desc is NULL");
+    if (d->interp < 0 || d->interp > 2) return fail(MPM_ERR_INVALID, "unknown interpolation type");
+    if (d->transfer < 0 || d->transfer > 1) return fail(MPM_ERR_INVALID, "unknown transfer scheme");
+    if (d->transfer == MPM_TRANSFER_APIC && d->interp == MPM_INTERP_LINEAR)
+        return fail(MPM_ERR_UNSUPPORTED,
+                    "APIC with linear interpolation has a singular D (Dinv non-finite); rejected");
+    for (int i = 0; i < 3; ++i) {
+        if (d->resolution[i] <= 0 || d->resolution[i] > 4096) return fail(MPM_ERR_INVALID, "resolution out of range");
+        if (!std::isfinite(d->origin[i])) return fail(MPM_ERR_INVALID, "origin not finite");
+    }
+    if (!(d->stride > 0.0f) || !std::isfinite(d->stride)) return fail(MPM_ERR_INVALID, "stride must be > 0");
+    if (!std::isfinite(d->dt) || !std::isfinite(d->gravity) || !std::isfinite(d->friction) ||
+        !std::isfinite(d->blend))
+        return fail(MPM_ERR_INVALID, "non-finite dt/gravity/friction/blend");
+    uint64_t kb = 1;
+    for (int i = 0; i < 3; ++i) kb *= (uint64_t)(((d->resolution[i] + 3) >> 2) + 1);
+    if (kb * 64ull >= (1ull << 32)) return fail(MPM_ERR_INVALID, "grid too large for 32-bit cell keys");
+    for (int i = 0; i < 4; ++i)
+        if (d->reserved[i] != 0) return fail(MPM_ERR_INVALID, "reserved fields must be zero");
+    return MPM_OK;
+}
+
+// Eigen's 3x3 inverse of D = k h^2 I (cofactor / determinant), float.
+float apic_dinv(const mpm_desc& d) {
+    volatile float k = 0.0f;
+    if (d.interp == MPM_INTERP_CUBIC_BSPLINE) k = (float)(1 / 3.0);
+    if (d.interp == MPM_INTERP_QUADRATIC_BSPLINE) k = (float)(1 / 4.0);
+    volatile float kh = k * d.stride;
+    volatile float D = kh * d.stride;
+    volatile float c = D * D;
+    volatile float det = c * D;
+    volatile float id = 1.0f / det;
+    volatile float r = c * id;
+    return r;
+}
+
+void fill_scene(Solver& s) {
+    const mpm_desc& d = s.desc;
+    SceneDev& S = s.S;
+    S.ox = d.origin[0]; S.oy = d.origin[1]; S.oz = d.origin[2];
+    volatile float hx = d.stride * (float)d.resolution[0];
+    volatile float hy = d.stride * (float)d.resolution[1];
+    volatile float hz = d.stride * (float)d.resolution[2];
+    S.tx = d.origin[0] + hx; S.ty = d.origin[1] + hy; S.tz = d.origin[2] + hz;
+    S.h = d.stride;
+    S.inv_h = 1.0f / d.stride;
+    S.dt = d.dt;
+    S.gravity = d.gravity;
+    S.friction = d.friction;
+    S.blend = d.blend;
+    S.dinv = s.apic ? apic_dinv(d) : 0.0f;
+    volatile float h3 = d.stride * d.stride;
+    h3 = h3 * d.stride;
+    S.inv_cell_vol = (float)(1.0 / (double)h3);
+    S.rx = d.resolution[0]; S.ry = d.resolution[1]; S.rz = d.resolution[2];
+    S.nbx = (S.rx + 3) / 4; S.nby = (S.ry + 3) / 4; S.nbz = (S.rz + 3) / 4;
+    S.kbx = ((S.rx + 3) >> 2) + 1; S.kby = ((S.ry + 3) >> 2) + 1; S.kbz = ((S.rz + 3) >> 2) + 1;
+    S.escape_key = (uint32_t)S.kbx * (uint32_t)S.kby * (uint32_t)S.kbz * 64u;
+    S.particle_collision = d.particle_collision;
+    s.nsb = (int64_t)S.nbx * S.nby * S.nbz;
+    S.plane = s.nsb * 64;
+    int bits = 1;
+    while (bits < 32 && ((uint64_t)S.escape_key >> bits) != 0) ++bits;
+    s.key_bits = bits;
+}
+
+// Host arrays in the C-ABI layout (original order).
+struct HostState {
+    std::vector<float> volume0, mass, pos, vel, Fe, Fp, mu0, lam0, B;
+    std::vector<uint16_t> mat;
+};
+
+int dedupe_materials(Solver& s, int64_t n, const float* xi, const float* thc, const float* ths, const int32_t* model,
+                     std::vector<uint16_t>& mat_ids) {
+    std::map<std::tuple<uint32_t, uint32_t, uint32_t, int32_t>, uint16_t> table;
+    for (size_t i = 0; i < s.mats.size(); ++i) {
+        uint32_t a, b, c;
+        std::memcpy(&a, &s.mats[i].hardening, 4);
+        std::memcpy(&b, &s.mats[i].crit_c, 4);
+        std::memcpy(&c, &s.mats[i].crit_s, 4);
+        table[{a, b, c, s.mats[i].model}] = (uint16_t)i;
+    }
+    mat_ids.resize((size_t)n);
+    for (int64_t i = 0; i < n; ++i) {
+        if (model[i] != MPM_MODEL_FIXED_COROTATED && model[i] != MPM_MODEL_NEO_HOOKEAN)
+            return fail(MPM_ERR_INVALID, "particle " + std::to_string(i) + ": unknown constitutive model");
+        uint32_t a, b, c;
+        std::memcpy(&a, &xi[i], 4);
+        std::memcpy(&b, &thc[i], 4);
+        std::memcpy(&c, &ths[i], 4);
+        auto key = std::make_tuple(a, b, c, model[i]);
+        auto it = table.find(key);
+        if (it == table.end()) {
+            if (s.mats.size() >= MPM_MAX_MATERIALS)
+                return fail(MPM_ERR_UNSUPPORTED, "more than 65536 distinct (hardening, theta_c, theta_s, model)");
+            MaterialDev m{xi[i], thc[i], ths[i], model[i]};
+            s.mats.push_back(m);
+            uint16_t id = (uint16_t)(s.mats.size() - 1);
+            table[key] = id;
+            mat_ids[(size_t)i] = id;
+        } else {
+            mat_ids[(size_t)i] = it->second;
+        }
+    }
+    return MPM_OK;
+}
+
+int upload_materials(Solver& s) {
+    if (s.mats_dev) dfree(s, s.mats_dev);
+    s.mats_dev = nullptr;
+    size_t bytes = std::max<size_t>(1, s.mats.size()) * sizeof(MaterialDev);
+    int rc = dalloc(s, (void**)&s.mats_dev, bytes);
+    if (rc) return rc;
+    if (!s.mats.empty()) HIP_TRY(hipMemcpyAsync(s.mats_dev, s.mats.data(), s.mats.size() * sizeof(MaterialDev),
+                                                hipMemcpyHostToDevice, s.st));
+    s.P.mats = s.mats_dev;
+    return MPM_OK;
+}
+
+int alloc_particles(Solver& s, ParticlesDev& P, int64_t n) {
+    P.n = n;
+    size_t f = (size_t)std::max<int64_t>(n, 1) * sizeof(float);
+    int rc;
+    if ((rc = dalloc(s, (void**)&P.x, 3 * f))) return rc;
+    if ((rc = dalloc(s, (void**)&P.v, 3 * f))) return rc;
+    if (s.apic && (rc = dalloc(s, (void**)&P.B, 9 * f))) return rc;
+    if (!s.apic) P.B = nullptr;
+    if ((rc = dalloc(s, (void**)&P.Fe, 9 * f))) return rc;
+    if ((rc = dalloc(s, (void**)&P.Fp, 9 * f))) return rc;
+    if ((rc = dalloc(s, (void**)&P.m, f))) return rc;
+    if ((rc = dalloc(s, (void**)&P.V0, f))) return rc;
+    if ((rc = dalloc(s, (void**)&P.mu0, f))) return rc;
+    if ((rc = dalloc(s, (void**)&P.lam0, f))) return rc;
+    if ((rc = dalloc(s, (void**)&P.mat, (size_t)std::max<int64_t>(n, 1) * 2))) return rc;
+    if ((rc = dalloc(s, (void**)&P.pid, (size_t)std::max<int64_t>(n, 1) * 4))) return rc;
+    P.mats = s.mats_dev;
+    return MPM_OK;
+}
+void free_particles(Solver& s, ParticlesDev& P) {
+    dfree(s, P.x); dfree(s, P.v); dfree(s, P.B); dfree(s, P.Fe); dfree(s, P.Fp); dfree(s, P.m);
+    dfree(s, P.V0); dfree(s, P.mu0); dfree(s, P.lam0); dfree(s, P.mat); dfree(s, P.pid);
+}
+
+// Host layout staging on the device for pack/unpack.
+struct Staging {
+    HostLayoutDev H{};
+    std::vector<void*> ptrs;
+};
+int alloc_staging(Solver& s, Staging& g, int64_t n) {
+    size_t f = (size_t)std::max<int64_t>(n, 1) * sizeof(float);
+    float** fields[] = {&g.H.volume0, &g.H.mass, &g.H.mu0, &g.H.lam0};
+    for (auto p : fields) {
+        int rc = dalloc(s, (void**)p, f);
+        if (rc) return rc;
+        g.ptrs.push_back(*p);
+    }
+    int rc;
+    if ((rc = dalloc(s, (void**)&g.H.pos, 3 * f))) return rc;
+    g.ptrs.push_back(g.H.pos);
+    if ((rc = dalloc(s, (void**)&g.H.vel, 3 * f))) return rc;
+    g.ptrs.push_back(g.H.vel);
+    if ((rc = dalloc(s, (void**)&g.H.Fe, 9 * f))) return rc;
+    g.ptrs.push_back(g.H.Fe);
+    if ((rc = dalloc(s, (void**)&g.H.Fp, 9 * f))) return rc;
+    g.ptrs.push_back(g.H.Fp);
+    if ((rc = dalloc(s, (void**)&g.H.B, 9 * f))) return rc;
+    g.ptrs.push_back(g.H.B);
+    if ((rc = dalloc(s, (void**)&g.H.mat, (size_t)std::max<int64_t>(n, 1) * 2))) return rc;
+    g.ptrs.push_back(g.H.mat);
+    return MPM_OK;
+}
+void free_staging(Solver& s, Staging& g) {
+    for (void* p : g.ptrs) dfree(s, p);
+    g.ptrs.clear();
+}
+
+// ---- device phases
+void run_sort(Solver& s) {
+    ProfileHook h = hook(s);
+    if (s.N <= 0) { s.sort_valid = true; return; }
+    {
+        Scope sc(h, "keys");
+        launch_keys(s.S, s.P, s.interp, s.k0, s.v0, s.st);
+    }
+    int sel = radix_sort_pairs(s.k0, s.v0, s.k1, s.v1, s.N, s.key_bits, s.hist, s.scan_temp, s.st, h);
+    s.I.sorted_keys = sel ? s.k1 : s.k0;
+    s.I.perm = sel ? s.v1 : s.v0;
+    s.I.start = s.items;
+    s.I.count = s.counters;
+    s.I.n = s.N;
+    {
+        Scope sc(h, "items");
+        launch_items(s.I.sorted_keys, s.N, s.flags, s.scan_temp, s.items, s.counters, s.st);
+    }
+    s.sort_valid = true;
+}
+
+void run_clear(Solver& s) {
+    ProfileHook h = hook(s);
+    Scope sc(h, "grid_clear");
+    launch_clear(s.S, s.nsb, s.vstar_dirty ? GF_ALL : GF_PRIMARY, s.count_on_clear ? s.acc : nullptr, s.st);
+    s.grid_dirty = false;
+    s.vstar_dirty = false;
+    s.count_on_clear = false;
+    s.phase = PH_CLEAR;
+}
+
+void run_fused_step(Solver& s) {
+    ProfileHook h = hook(s);
+    if (s.grid_dirty) run_clear(s);
+    run_sort(s);
+    {
+        Scope sc(h, "p2g");
+        launch_p2g(s.S, s.P, s.I, s.interp, s.apic, 0, s.item_wgs, s.st);
+    }
+    {
+        Scope sc(h, "g2p");
+        launch_g2p(s.S, s.P, s.I, s.interp, s.apic, 0, 1, s.S.particle_collision, s.item_wgs, s.st);
+    }
+    s.grid_dirty = true;
+    s.count_on_clear = true;
+    s.phase = PH_RAW;
+    s.sort_valid = false;
+    s.steps += 1;
+}
+
+void run_initialize(Solver& s) {
+    if (s.grid_dirty) run_clear(s);
+    run_sort(s);
+    launch_p2g(s.S, s.P, s.I, s.interp, s.apic, 1, s.item_wgs, s.st);
+    launch_g2p(s.S, s.P, s.I, s.interp, s.apic, 2, 0, 0, s.item_wgs, s.st);
+    s.grid_dirty = true;
+    s.count_on_clear = false;
+    s.phase = PH_MASS;
+}
+
+// Materialise the grid as the reference holds it after a step (gravity
+// applied to force, velocity normalised, v* after collision).
+void materialize(Solver& s) {
+    if (s.phase != PH_RAW) return;
+    launch_grid_op(s.S, s.nsb, 0, s.S.dt, s.st);
+    launch_grid_op(s.S, s.nsb, 1, s.S.dt, s.st);
+    launch_grid_op(s.S, s.nsb, 2, s.S.dt, s.st);
+    s.vstar_dirty = true;
+    s.phase = PH_SUBSTEP;
+}
+
+int build(Solver& s, HostState& H, int run_init) {
+    int rc;
+    HIP_TRY(hipGetDevice(&s.device));
+    HIP_TRY(hipStreamCreateWithFlags(&s.st, hipStreamNonBlocking));
+    hipDeviceProp_t prop;
+    HIP_TRY(hipGetDeviceProperties(&prop, s.device));
+    s.item_wgs = std::max(64, prop.multiProcessorCount * 8);
+    fill_scene(s);
+    if ((rc = upload_materials(s))) return rc;
+    // grid
+    size_t gbytes = (size_t)s.S.plane * GF_ALL * sizeof(float);
+    if ((rc = dalloc(s, (void**)&s.S.grid, gbytes))) return rc;
+    HIP_TRY(hipMemsetAsync(s.S.grid, 0, gbytes, s.st));
+    if ((rc = dalloc(s, (void**)&s.S.touched, (size_t)s.nsb * 4))) return rc;
+    HIP_TRY(hipMemsetAsync(s.S.touched, 0, (size_t)s.nsb * 4, s.st));
+    // sort buffers
+    const int64_t n = std::max<int64_t>(s.N, 1);
+    if ((rc = dalloc(s, (void**)&s.k0, (size_t)n * 4))) return rc;
+    if ((rc = dalloc(s, (void**)&s.v0, (size_t)n * 4))) return rc;
+    if ((rc = dalloc(s, (void**)&s.k1, (size_t)n * 4))) return rc;
+    if ((rc = dalloc(s, (void**)&s.v1, (size_t)n * 4))) return rc;
+    if ((rc = dalloc(s, (void**)&s.flags, (size_t)n * 4))) return rc;
+    if ((rc = dalloc(s, (void**)&s.items, (size_t)n * 4))) return rc;
+    int64_t hw = radix_hist_words(n);
+    if ((rc = dalloc(s, (void**)&s.hist, (size_t)hw * 4))) return rc;
+    int64_t tw = std::max(scan_temp_words(hw), scan_temp_words(n));
+    if ((rc = dalloc(s, (void**)&s.scan_temp, (size_t)tw * 4))) return rc;
+    if ((rc = dalloc(s, (void**)&s.counters, 64))) return rc;
+    HIP_TRY(hipMemsetAsync(s.counters, 0, 64, s.st));
+    if ((rc = dalloc(s, (void**)&s.acc, 64))) return rc;
+    HIP_TRY(hipMemsetAsync(s.acc, 0, 64, s.st));
+    // particles
+    if ((rc = alloc_particles(s, s.P, s.N))) return rc;
+    if (s.N > 0) {
+        std::vector<uint32_t> iota((size_t)s.N);
+        for (int64_t i = 0; i < s.N; ++i) iota[(size_t)i] = (uint32_t)i;
+        HIP_TRY(hipMemcpyAsync(s.P.pid, iota.data(), (size_t)s.N * 4, hipMemcpyHostToDevice, s.st));
+        Staging g;
+        if ((rc = alloc_staging(s, g, s.N))) return rc;
+        const size_t f = (size_t)s.N * 4;
+        HIP_TRY(hipMemcpyAsync(g.H.volume0, H.volume0.data(), f, hipMemcpyHostToDevice, s.st));
+        HIP_TRY(hipMemcpyAsync(g.H.mass, H.mass.data(), f, hipMemcpyHostToDevice, s.st));
+        HIP_TRY(hipMemcpyAsync(g.H.mu0, H.mu0.data(), f, hipMemcpyHostToDevice, s.st));
+        HIP_TRY(hipMemcpyAsync(g.H.lam0, H.lam0.data(), f, hipMemcpyHostToDevice, s.st));
+        HIP_TRY(hipMemcpyAsync(g.H.pos, H.pos.data(), 3 * f, hipMemcpyHostToDevice, s.st));
+        HIP_TRY(hipMemcpyAsync(g.H.vel, H.vel.data(), 3 * f, hipMemcpyHostToDevice, s.st));
+        HIP_TRY(hipMemcpyAsync(g.H.Fe, H.Fe.data(), 9 * f, hipMemcpyHostToDevice, s.st));
+        HIP_TRY(hipMemcpyAsync(g.H.Fp, H.Fp.data(), 9 * f, hipMemcpyHostToDevice, s.st));
+        HIP_TRY(hipMemcpyAsync(g.H.B, H.B.data(), 9 * f, hipMemcpyHostToDevice, s.st));
+        HIP_TRY(hipMemcpyAsync(g.H.mat, H.mat.data(), (size_t)s.N * 2, hipMemcpyHostToDevice, s.st));
+        launch_pack(g.H, s.P, s.st);
+        HIP_TRY(hipStreamSynchronize(s.st));
+        free_staging(s, g);
+        // Physical reorder by cell so that per-step accesses stream.
+        run_sort(s);
+        ParticlesDev Q{};
+        if ((rc = alloc_particles(s, Q, s.N))) return rc;
+        launch_permute(s.P, Q, s.I.perm, s.st);
+        HIP_TRY(hipStreamSynchronize(s.st));
+        free_particles(s, s.P);
+        s.P = Q;
+        s.sort_valid = false;
+    }
+    HIP_TRY(hipGetLastError());
+    if (run_init) run_initialize(s);
+    HIP_TRY(hipStreamSynchronize(s.st));
+    HIP_TRY(hipGetLastError());
+    return MPM_OK;
+}
+
+void destroy(Solver* s) {
+    if (!s) return;
+    if (s->st) (void)hipStreamSynchronize(s->st);
+    if (s->gexec) (void)hipGraphExecDestroy(s->gexec);
+    if (s->graph) (void)hipGraphDestroy(s->graph);
+    for (auto& p : s->pending) s->event_pool.push_back(p.a), s->event_pool.push_back(p.b);
+    for (auto e : s->event_pool) (void)hipEventDestroy(e);
+    for (void* p : s->allocs) (void)hipFree(p);
+    if (s->st) (void)hipStreamDestroy(s->st);
+    delete s;
+}
+
+Solver* as_solver(void* h) { return static_cast<Solver*>(h); }
+
+#define GET_SOLVER(var, h)                                                         \
+    Solver* var = as_solver(h);                                                    \
+    if (!var) return fail(MPM_ERR_INVALID, "handle is NULL")
+
+int finish(Solver& s) {
+    hipError_t e = hipGetLastError();
+    if (e != hipSuccess) return fail(MPM_ERR_HIP, std::string("kernel launch: ") + hipGetErrorString(e));
+    return MPM_OK;
+}
+
+int create_common(const mpm_desc* desc, HostState& H, int64_t n, const std::vector<MaterialDev>& mats, int run_init,
+                  void** out) {
+    Solver* s = new (std::nothrow) Solver();
+    if (!s) return fail(MPM_ERR_OOM, "host allocation failed");
+    s->desc = *desc;
+    s->mats = mats;
+    s->interp = desc->interp;
+    s->apic = desc->transfer == MPM_TRANSFER_APIC;
+    s->N = n;
+    int rc = build(*s, H, run_init);
+    if (rc) {
+        std::string keep = g_err;
+        destroy(s);
+        g_err = keep;
+        return rc;
+    }
+    *out = s;
+    return MPM_OK;
+}
+
+}  // namespace
+
+extern "C" {
+
+MPM_API const char* mpm_last_error(void) { return g_err.c_str(); }
+MPM_API const char* mpm_version(void) { return "mpm_hip 1.0 (gfx950)"; }
+
+MPM_API int mpm_create(const mpm_desc* desc, const mpm_particle_input* in, void** out_handle) {
+    if (!out_handle) return fail(MPM_ERR_INVALID, "out_handle is NULL");
+    *out_handle = nullptr;
+    int rc = check_desc(desc);
+    if (rc) return rc;
+    if (!in || in->count < 0) return fail(MPM_ERR_INVALID, "particle input is NULL or count < 0");
+    const int64_t n = in->count;
+    if (n > 0 && (!in->position || !in->velocity || !in->mass || !in->hardening || !in->young || !in->poisson ||
+                  !in->critical_compression || !in->critical_stretch || !in->model))
+        return fail(MPM_ERR_INVALID, "particle input has a NULL array");
+    if (n >= (int64_t)0xFFFFFFFFll) return fail(MPM_ERR_INVALID, "too many particles");
+    HostState H;
+    H.volume0.assign((size_t)n, 0.0f);
+    H.mass.assign(in->mass, in->mass + n);
+    H.pos.assign(in->position, in->position + 3 * n);
+    H.vel.assign(in->velocity, in->velocity + 3 * n);
+    H.Fe.assign((size_t)(9 * n), 0.0f);
+    H.Fp.assign((size_t)(9 * n), 0.0f);
+    H.B.assign((size_t)(9 * n), 0.0f);
+    H.mu0.resize((size_t)n);
+    H.lam0.resize((size_t)n);
+    for (int64_t i = 0; i < n; ++i) {
+        for (int k = 0; k < 3; ++k) {
+            H.Fe[(size_t)(9 * i + 4 * k)] = 1.0f;
+            H.Fp[(size_t)(9 * i + 4 * k)] = 1.0f;
+        }
+        // MaterialPoint3D constructor, src/material_point3d.cu:25-26
+        const float E = in->young[i], nu = in->poisson[i];
+        H.mu0[(size_t)i] = (float)(E / (2.0 * (1.0 + nu)));
+        H.lam0[(size_t)i] = (float)((E * nu) / ((1.0 + nu) * (1.0 - 2.0 * nu)));
+    }
+    Solver probe;
+    rc = dedupe_materials(probe, n, in->hardening, in->critical_compression, in->critical_stretch, in->model, H.mat);
+    if (rc) return rc;
+    return create_common(desc, H, n, probe.mats, 1, out_handle);
+}
+
+MPM_API int mpm_create_state(const mpm_desc* desc, const mpm_particle_state* st, int run_initialize,
+                             void** out_handle) {
+    if (!out_handle) return fail(MPM_ERR_INVALID, "out_handle is NULL");
+    *out_handle = nullptr;
+    int rc = check_desc(desc);
+    if (rc) return rc;
+    if (!st || st->count < 0) return fail(MPM_ERR_INVALID, "state is NULL or count < 0");
+    const int64_t n = st->count;
+    if (n > 0 && (!st->volume0 || !st->mass || !st->position || !st->velocity || !st->Fe || !st->Fp || !st->mu0 ||
+                  !st->lambda0 || !st->hardening || !st->critical_compression || !st->critical_stretch || !st->model))
+        return fail(MPM_ERR_INVALID, "state has a NULL array");
+    HostState H;
+    H.volume0.assign(st->volume0, st->volume0 + n);
+    H.mass.assign(st->mass, st->mass + n);
+    H.pos.assign(st->position, st->position + 3 * n);
+    H.vel.assign(st->velocity, st->velocity + 3 * n);
+    H.Fe.assign(st->Fe, st->Fe + 9 * n);
+    H.Fp.assign(st->Fp, st->Fp + 9 * n);
+    H.mu0.assign(st->mu0, st->mu0 + n);
+    H.lam0.assign(st->lambda0, st->lambda0 + n);
+    if (st->B) H.B.assign(st->B, st->B + 9 * n);
+    else H.B.assign((size_t)(9 * n), 0.0f);
+    Solver probe;
+    rc = dedupe_materials(probe, n, st->hardening, st->critical_compression, st->critical_stretch, st->model, H.mat);
+    if (rc) return rc;
+    return create_common(desc, H, n, probe.mats, run_initialize, out_handle);
+}
+
+MPM_API void mpm_destroy(void* handle) { destroy(as_solver(handle)); }
+
+MPM_API int mpm_step(void* handle, int n) {
+    GET_SOLVER(s, handle);
+    if (n < 0) return fail(MPM_ERR_INVALID, "n < 0");
+    auto t0 = std::chrono::steady_clock::now();
+    for (int i = 0; i < n; ++i) run_fused_step(*s);
+    int rc = finish(*s);
+    auto t1 = std::chrono::steady_clock::now();
+    s->last_step_ms = std::chrono::duration<double, std::milli>(t1 - t0).count();
+    return rc;
+}
+
+MPM_API int mpm_synchronize(void* handle) {
+    GET_SOLVER(s, handle);
+    HIP_TRY(hipStreamSynchronize(s->st));
+    return finish(*s);
+}
+
+MPM_API int mpm_step_graph(void* handle, int n) {
+    GET_SOLVER(s, handle);
+    if (n < 0) return fail(MPM_ERR_INVALID, "n < 0");
+    if (n == 0) return MPM_OK;
+    // A grid left by initialize / sub-steps is cleared outside the graph
+    // (without counting); the captured step always starts with a clear.
+    if (s->grid_dirty && !s->count_on_clear) run_clear(*s);
+    if (!s->gexec) {
+        bool prof = s->prof;
+        s->prof = false;
+        HIP_TRY(hipStreamBeginCapture(s->st, hipStreamCaptureModeThreadLocal));
+        bool dirty = s->grid_dirty;
+        s->grid_dirty = true;  // force the clear node into the graph
+        s->vstar_dirty = true;
+        run_fused_step(*s);
+        s->steps -= 1;
+        hipError_t e = hipStreamEndCapture(s->st, &s->graph);
+        s->prof = prof;
+        (void)dirty;
+        if (e != hipSuccess) return fail(MPM_ERR_HIP, std::string("graph capture: ") + hipGetErrorString(e));
+        HIP_TRY(hipGraphInstantiate(&s->gexec, s->graph, nullptr, nullptr, 0));
+    }
+    auto t0 = std::chrono::steady_clock::now();
+    for (int i = 0; i < n; ++i) HIP_TRY(hipGraphLaunch(s->gexec, s->st));
+    s->steps += n;
+    s->grid_dirty = true;
+    s->count_on_clear = true;
+    s->vstar_dirty = true;
+    s->phase = PH_RAW;
+    s->sort_valid = false;
+    auto t1 = std::chrono::steady_clock::now();
+    s->last_step_ms = std::chrono::duration<double, std::milli>(t1 - t0).count();
+    return finish(*s);
+}
+
+MPM_API int mpm_initialize(void* handle) {
+    GET_SOLVER(s, handle);
+    run_initialize(*s);
+    return finish(*s);
+}
+
+MPM_API int mpm_reset_grid(void* handle) {
+    GET_SOLVER(s, handle);
+    s->count_on_clear = false;
+    run_clear(*s);
+    return finish(*s);
+}
+
+MPM_API int mpm_particles_to_grid(void* handle) {
+    GET_SOLVER(s, handle);
+    if (!s->sort_valid) run_sort(*s);
+    launch_p2g(s->S, s->P, s->I, s->interp, s->apic, 0, s->item_wgs, s->st);
+    s->grid_dirty = true;
+    s->count_on_clear = false;
+    s->phase = PH_SUBSTEP;
+    return finish(*s);
+}
+
+MPM_API int mpm_compute_gravity_forces(void* handle) {
+    GET_SOLVER(s, handle);
+    launch_grid_op(s->S, s->nsb, 0, s->S.dt, s->st);
+    s->phase = PH_SUBSTEP;
+    return finish(*s);
+}
+
+MPM_API int mpm_compute_external_forces(void* handle) { return mpm_compute_gravity_forces(handle); }
+
+MPM_API int mpm_update_grid_velocities(void* handle, float dt) {
+    GET_SOLVER(s, handle);
+    launch_grid_op(s->S, s->nsb, 1, dt, s->st);
+    s->vstar_dirty = true;
+    s->phase = PH_SUBSTEP;
+    return finish(*s);
+}
+
+MPM_API int mpm_solve_linear_system(void* handle, float dt) {
+    GET_SOLVER(s, handle);
+    (void)dt;  // empty in the reference (src/mpm_solver3d.cu:312-314)
+    (void)s;
+    return MPM_OK;
+}
+
+MPM_API int mpm_grid_collision(void* handle, float dt) {
+    GET_SOLVER(s, handle);
+    launch_grid_op(s->S, s->nsb, 2, dt, s->st);
+    s->vstar_dirty = true;
+    s->phase = PH_SUBSTEP;
+    return finish(*s);
+}
+
+MPM_API int mpm_grid_to_particles(void* handle, float dt, float blend) {
+    GET_SOLVER(s, handle);
+    if (s->phase == PH_RAW) materialize(*s);
+    if (!s->sort_valid) run_sort(*s);
+    SceneDev S = s->S;
+    S.dt = dt;
+    S.blend = blend;
+    launch_g2p(S, s->P, s->I, s->interp, s->apic, 1, 0, 0, s->item_wgs, s->st);
+    return finish(*s);
+}
+
+MPM_API int mpm_particles_collision(void* handle, float dt) {
+    GET_SOLVER(s, handle);
+    launch_particles_collision(s->S, s->P, dt, s->st);
+    return finish(*s);
+}
+
+MPM_API int mpm_advect_particles(void* handle, float dt) {
+    GET_SOLVER(s, handle);
+    launch_advect(s->P, dt, s->st);
+    s->sort_valid = false;
+    return finish(*s);
+}
+
+MPM_API int mpm_switch_particles_collision(void* handle) {
+    GET_SOLVER(s, handle);
+    s->S.particle_collision = s->S.particle_collision ? 0 : 1;
+    s->desc.particle_collision = s->S.particle_collision;
+    if (s->gexec) { (void)hipGraphExecDestroy(s->gexec); s->gexec = nullptr; }
+    if (s->graph) { (void)hipGraphDestroy(s->graph); s->graph = nullptr; }
+    return MPM_OK;
+}
+
+MPM_API int mpm_set_particles_collision(void* handle, int enable) {
+    GET_SOLVER(s, handle);
+    if ((s->S.particle_collision != 0) != (enable != 0)) return mpm_switch_particles_collision(handle);
+    return MPM_OK;
+}
+
+MPM_API int mpm_sort(void* handle) {
+    GET_SOLVER(s, handle);
+    run_sort(*s);
+    return finish(*s);
+}
+
+MPM_API int mpm_read_particles(void* handle, mpm_particle_state* out) {
+    GET_SOLVER(s, handle);
+    if (!out || out->count != s->N) return fail(MPM_ERR_INVALID, "output count must equal the particle count");
+    const int64_t n = s->N;
+    if (n == 0) return MPM_OK;
+    Staging g;
+    int rc = alloc_staging(*s, g, n);
+    if (rc) { free_staging(*s, g); return rc; }
+    launch_unpack(s->P, g.H, s->st);
+    const size_t f = (size_t)n * 4;
+    std::vector<uint16_t> mat((size_t)n);
+    auto cp = [&](void* dst, const void* src, size_t bytes) -> hipError_t {
+        if (!dst) return hipSuccess;
+        return hipMemcpyAsync(dst, src, bytes, hipMemcpyDeviceToHost, s->st);
+    };
+    hipError_t e = hipSuccess;
+    if (e == hipSuccess) e = cp(out->volume0, g.H.volume0, f);
+    if (e == hipSuccess) e = cp(out->mass, g.H.mass, f);
+    if (e == hipSuccess) e = cp(out->position, g.H.pos, 3 * f);
+    if (e == hipSuccess) e = cp(out->velocity, g.H.vel, 3 * f);
+    if (e == hipSuccess) e = cp(out->Fe, g.H.Fe, 9 * f);
+    if (e == hipSuccess) e = cp(out->Fp, g.H.Fp, 9 * f);
+    if (e == hipSuccess) e = cp(out->mu0, g.H.mu0, f);
+    if (e == hipSuccess) e = cp(out->lambda0, g.H.lam0, f);
+    if (e == hipSuccess && out->B) {
+        if (s->apic) e = cp(out->B, g.H.B, 9 * f);
+        else std::memset(out->B, 0, 9 * f);
+    }
+    if (e == hipSuccess) e = cp(mat.data(), g.H.mat, (size_t)n * 2);
+    if (e == hipSuccess) e = hipStreamSynchronize(s->st);
+    free_staging(*s, g);
+    if (e != hipSuccess) return fail(MPM_ERR_HIP, std::string("read particles: ") + hipGetErrorString(e));
+    for (int64_t i = 0; i < n; ++i) {
+        const MaterialDev& m = s->mats[mat[(size_t)i]];
+        if (out->hardening) out->hardening[i] = m.hardening;
+        if (out->critical_compression) out->critical_compression[i] = m.crit_c;
+        if (out->critical_stretch) out->critical_stretch[i] = m.crit_s;
+        if (out->model) out->model[i] = m.model;
+    }
+    return finish(*s);
+}
+
+MPM_API int mpm_write_particles(void* handle, const mpm_particle_state* in) {
+    GET_SOLVER(s, handle);
+    if (!in || in->count != s->N) return fail(MPM_ERR_INVALID, "state count must equal the particle count");
+    const int64_t n = s->N;
+    if (n == 0) return MPM_OK;
+    if (!in->volume0 || !in->mass || !in->position || !in->velocity || !in->Fe || !in->Fp || !in->mu0 ||
+        !in->lambda0 || !in->hardening || !in->critical_compression || !in->critical_stretch || !in->model)
+        return fail(MPM_ERR_INVALID, "state has a NULL array");
+    std::vector<uint16_t> mat;
+    size_t before = s->mats.size();
+    int rc = dedupe_materials(*s, n, in->hardening, in->critical_compression, in->critical_stretch, in->model, mat);
+    if (rc) return rc;
+    if (s->mats.size() != before) {
+        HIP_TRY(hipStreamSynchronize(s->st));
+        if ((rc = upload_materials(*s))) return rc;
+    }
+    Staging g;
+    rc = alloc_staging(*s, g, n);
+    if (rc) { free_staging(*s, g); return rc; }
+    const size_t f = (size_t)n * 4;
+    std::vector<float> zeros;
+    const float* B = in->B;
+    if (!B) { zeros.assign((size_t)(9 * n), 0.0f); B = zeros.data(); }
+    hipError_t e = hipSuccess;
+    auto cp = [&](void* dst, const void* src, size_t bytes) {
+        if (e == hipSuccess) e = hipMemcpyAsync(dst, src, bytes, hipMemcpyHostToDevice, s->st);
+    };
+    cp(g.H.volume0, in->volume0, f);
+    cp(g.H.mass, in->mass, f);
+    cp(g.H.pos, in->position, 3 * f);
+    cp(g.H.vel, in->velocity, 3 * f);
+    cp(g.H.Fe, in->Fe, 9 * f);
+    cp(g.H.Fp, in->Fp, 9 * f);
+    cp(g.H.mu0, in->mu0, f);
+    cp(g.H.lam0, in->lambda0, f);
+    cp(g.H.B, B, 9 * f);
+    cp(g.H.mat, mat.data(), (size_t)n * 2);
+    if (e == hipSuccess) launch_pack(g.H, s->P, s->st);
+    if (e == hipSuccess) e = hipStreamSynchronize(s->st);
+    free_staging(*s, g);
+    if (e != hipSuccess) return fail(MPM_ERR_HIP, std::string("write particles: ") + hipGetErrorString(e));
+    s->sort_valid = false;
+    return finish(*s);
+}
+
+MPM_API int mpm_read_grid(void* handle, mpm_grid_host* out) {
+    GET_SOLVER(s, handle);
+    const int64_t total = (int64_t)s->S.rx * s->S.ry * s->S.rz;
+    if (!out || out->nodes != total) return fail(MPM_ERR_INVALID, "grid node count mismatch");
+    if (s->phase == PH_RAW) materialize(*s);
+    float *m = nullptr, *v = nullptr, *w = nullptr, *f = nullptr;
+    int rc;
+    if ((rc = dalloc(*s, (void**)&m, (size_t)total * 4))) return rc;
+    if ((rc = dalloc(*s, (void**)&v, (size_t)total * 12))) { dfree(*s, m); return rc; }
+    if ((rc = dalloc(*s, (void**)&w, (size_t)total * 12))) { dfree(*s, m); dfree(*s, v); return rc; }
+    if ((rc = dalloc(*s, (void**)&f, (size_t)total * 12))) { dfree(*s, m); dfree(*s, v); dfree(*s, w); return rc; }
+    launch_grid_dense(s->S, m, v, w, f, s->vstar_dirty ? 1 : 0, s->st);
+    hipError_t e = hipSuccess;
+    if (out->mass) e = hipMemcpyAsync(out->mass, m, (size_t)total * 4, hipMemcpyDeviceToHost, s->st);
+    if (e == hipSuccess && out->velocity)
+        e = hipMemcpyAsync(out->velocity, v, (size_t)total * 12, hipMemcpyDeviceToHost, s->st);
+    if (e == hipSuccess && out->velocity_star)
+        e = hipMemcpyAsync(out->velocity_star, w, (size_t)total * 12, hipMemcpyDeviceToHost, s->st);
+    if (e == hipSuccess && out->force)
+        e = hipMemcpyAsync(out->force, f, (size_t)total * 12, hipMemcpyDeviceToHost, s->st);
+    if (e == hipSuccess) e = hipStreamSynchronize(s->st);
+    dfree(*s, m); dfree(*s, v); dfree(*s, w); dfree(*s, f);
+    if (e != hipSuccess) return fail(MPM_ERR_HIP, std::string("read grid: ") + hipGetErrorString(e));
+    return finish(*s);
+}
+
+MPM_API int mpm_read_positions4(void* handle, float* out) {
+    GET_SOLVER(s, handle);
+    if (!out) return fail(MPM_ERR_INVALID, "out is NULL");
+    if (s->N == 0) return MPM_OK;
+    float* d = nullptr;
+    int rc = dalloc(*s, (void**)&d, (size_t)s->N * 16);
+    if (rc) return rc;
+    launch_positions4(s->P, d, s->st);
+    hipError_t e = hipMemcpyAsync(out, d, (size_t)s->N * 16, hipMemcpyDeviceToHost, s->st);
+    if (e == hipSuccess) e = hipStreamSynchronize(s->st);
+    dfree(*s, d);
+    if (e != hipSuccess) return fail(MPM_ERR_HIP, std::string("read positions: ") + hipGetErrorString(e));
+    return finish(*s);
+}
+
+MPM_API int mpm_read_sort(void* handle, uint32_t* keys, uint32_t* ids, uint32_t* seq, int64_t count) {
+    GET_SOLVER(s, handle);
+    if (count != s->N) return fail(MPM_ERR_INVALID, "count must equal the particle count");
+    if (s->N == 0) return MPM_OK;
+    if (!s->I.sorted_keys) return fail(MPM_ERR_STATE, "no sort has run");
+    uint32_t* tmp = nullptr;
+    int rc = dalloc(*s, (void**)&tmp, (size_t)s->N * 4);
+    if (rc) return rc;
+    // The particle storage order (pid) is fixed after creation, so it is
+    // the input sequence of every sort.
+    launch_gather_u32(s->P.pid, s->I.perm, tmp, s->N, s->st);
+    hipError_t e = hipSuccess;
+    if (keys) e = hipMemcpyAsync(keys, s->I.sorted_keys, (size_t)s->N * 4, hipMemcpyDeviceToHost, s->st);
+    if (e == hipSuccess && ids) e = hipMemcpyAsync(ids, tmp, (size_t)s->N * 4, hipMemcpyDeviceToHost, s->st);
+    if (e == hipSuccess && seq) e = hipMemcpyAsync(seq, s->P.pid, (size_t)s->N * 4, hipMemcpyDeviceToHost, s->st);
+    if (e == hipSuccess) e = hipStreamSynchronize(s->st);
+    dfree(*s, tmp);
+    if (e != hipSuccess) return fail(MPM_ERR_HIP, std::string("read sort: ") + hipGetErrorString(e));
+    return finish(*s);
+}
+
+MPM_API int mpm_host_keys(void* handle, const float* pos, int64_t n, uint32_t* keys) {
+    GET_SOLVER(s, handle);
+    if (n < 0 || (n > 0 && (!pos || !keys))) return fail(MPM_ERR_INVALID, "bad arguments");
+    const SceneDev& S = s->S;
+    const float shift = s->interp == MPM_INTERP_LINEAR ? 0.0f : s->interp == MPM_INTERP_QUADRATIC_BSPLINE ? 0.5f : 1.0f;
+    const int width = s->interp == MPM_INTERP_LINEAR ? 2 : s->interp == MPM_INTERP_QUADRATIC_BSPLINE ? 3 : 4;
+    const int res[3] = {S.rx, S.ry, S.rz};
+    for (int64_t i = 0; i < n; ++i) {
+        int b[3];
+        bool esc = false;
+        for (int a = 0; a < 3; ++a) {
+            volatile float u = pos[3 * i + a] / S.h;
+            volatile float t = u - shift;
+            float fb = std::floor(t);
+            if (!(fb >= (float)(-(width - 1)) && fb <= (float)(res[a] - 1))) { esc = true; break; }
+            b[a] = (int)fb;
+        }
+        if (esc) { keys[i] = S.escape_key; continue; }
+        uint32_t blk = (uint32_t)((b[0] + 4) >> 2) +
+                       (uint32_t)S.kbx * ((uint32_t)((b[1] + 4) >> 2) + (uint32_t)S.kby * (uint32_t)((b[2] + 4) >> 2));
+        keys[i] = blk * 64u + ((uint32_t)((b[0] + 4) & 3) | ((uint32_t)((b[1] + 4) & 3) << 2) |
+                               ((uint32_t)((b[2] + 4) & 3) << 4));
+    }
+    return MPM_OK;
+}
+
+MPM_API int mpm_get_stats(void* handle, mpm_stats* out) {
+    GET_SOLVER(s, handle);
+    if (!out) return fail(MPM_ERR_INVALID, "out is NULL");
+    std::memset(out, 0, sizeof(*out));
+    unsigned long long zero = 0;
+    HIP_TRY(hipMemcpyAsync(&s->acc[1], &zero, 8, hipMemcpyHostToDevice, s->st));
+    HIP_TRY(hipMemcpyAsync(&s->acc[2], &zero, 8, hipMemcpyHostToDevice, s->st));
+    if (s->grid_dirty) launch_count_active(s->S, s->nsb, &s->acc[1], s->st);
+    if (s->I.sorted_keys && s->N > 0) launch_count_escaped(s->I.sorted_keys, s->N, s->S.escape_key, &s->acc[2], s->st);
+    unsigned long long a[3] = {0, 0, 0};
+    uint32_t items = 0;
+    HIP_TRY(hipMemcpyAsync(a, s->acc, 24, hipMemcpyDeviceToHost, s->st));
+    HIP_TRY(hipMemcpyAsync(&items, s->counters, 4, hipMemcpyDeviceToHost, s->st));
+    HIP_TRY(hipStreamSynchronize(s->st));
+    out->particles = s->N;
+    out->steps = s->steps;
+    out->active_nodes = (int64_t)a[1];
+    out->active_nodes_sum = (int64_t)a[0] + (s->count_on_clear ? (int64_t)a[1] : 0);
+    out->work_items = items;
+    out->escaped = (int64_t)a[2];
+    out->device_bytes = s->device_bytes;
+    out->last_step_ms = s->last_step_ms;
+    return finish(*s);
+}
+
+MPM_API int mpm_profile_enable(void* handle, int enable) {
+    GET_SOLVER(s, handle);
+    s->prof = enable != 0;
+    return MPM_OK;
+}
+
+MPM_API int mpm_profile_reset(void* handle) {
+    GET_SOLVER(s, handle);
+    prof_collect(*s);
+    s->prof_tot.clear();
+    s->prof_order.clear();
+    return MPM_OK;
+}
+
+MPM_API int mpm_profile_read(void* handle, mpm_kernel_time* out, int max, int* count) {
+    GET_SOLVER(s, handle);
+    prof_collect(*s);
+    int k = 0;
+    for (auto& name : s->prof_order) {
+        if (out && k < max) {
+            std::memset(&out[k], 0, sizeof(mpm_kernel_time));
+            std::strncpy(out[k].name, name.c_str(), sizeof(out[k].name) - 1);
+            out[k].launches = s->prof_tot[name].first;
+            out[k].total_ms = s->prof_tot[name].second;
+        }
+        ++k;
+    }
+    if (count) *count = k;
+    return MPM_OK;
+}
+
+MPM_API int mpm_debug_svd3(const float* A, int64_t n, float* U, float* sv, float* V) {
+    if (n < 0 || (n > 0 && (!A || !U || !sv || !V))) return fail(MPM_ERR_INVALID, "bad arguments");
+    if (n == 0) return MPM_OK;
+    float *dA = nullptr, *dU = nullptr, *dS = nullptr, *dV = nullptr;
+    HIP_TRY(hipMalloc((void**)&dA, (size_t)n * 36));
+    HIP_TRY(hipMalloc((void**)&dU, (size_t)n * 36));
+    HIP_TRY(hipMalloc((void**)&dS, (size_t)n * 12));
+    HIP_TRY(hipMalloc((void**)&dV, (size_t)n * 36));
+    HIP_TRY(hipMemcpy(dA, A, (size_t)n * 36, hipMemcpyHostToDevice));
+    launch_debug_svd(dA, n, dU, dS, dV, nullptr);
+    HIP_TRY(hipGetLastError());
+    HIP_TRY(hipMemcpy(U, dU, (size_t)n * 36, hipMemcpyDeviceToHost));
+    HIP_TRY(hipMemcpy(sv, dS, (size_t)n * 12, hipMemcpyDeviceToHost));
+    HIP_TRY(hipMemcpy(V, dV, (size_t)n * 36, hipMemcpyDeviceToHost));
+    (void)hipFree(dA); (void)hipFree(dU); (void)hipFree(dS); (void)hipFree(dV);
+    return MPM_OK;
+}
+
+}  // extern "C"

@@ -1,0 +1,83 @@
+// common.h -- host/device shared definitions of the MI355X MPM solver.
+#pragma once
+
+#include <stdint.h>
+
+#include "mpm_hip.h"
+
+#define MPM_BLOCK 4          // grid block edge (nodes); blocks are 4x4x4
+#define MPM_BLOCK_NODES 64
+#define MPM_MAX_MATERIALS 65536
+
+// Grid field planes (each plane holds one float per padded node, in
+// block-major order).  Fields 0..6 are what P2G scatters (mass, momentum
+// stored in the reference's "velocity" slot, force); 7..9 hold velocity_star
+// for the unfused sub-step path.
+enum GridField {
+    GF_MASS = 0,
+    GF_VEL = 1,   // 1,2,3: momentum after P2G, velocity after updateVelocity
+    GF_FORCE = 4, // 4,5,6
+    GF_VSTAR = 7, // 7,8,9
+    GF_PRIMARY = 7,
+    GF_ALL = 10
+};
+
+// Material constants shared by groups of particles (hardening xi,
+// critical compression/stretch, constitutive model): the per-particle
+// fields of MaterialPoint3D that do not change (include/material_point3d.h:19-21).
+struct MaterialDev {
+    float hardening;
+    float crit_c;
+    float crit_s;
+    int32_t model;
+};
+
+// Everything a kernel needs to know about one scene (grid + integrator
+// constants).  Passed by value.
+struct SceneDev {
+    float ox, oy, oz;        // grid origin
+    float tx, ty, tz;        // target = origin + h * res (Grid3DSettings)
+    float h;                 // stride
+    float inv_h;             // 1/h (weight gradients)
+    float dt;
+    float gravity;           // magnitude along -y
+    float friction;          // boundary friction coefficient
+    float blend;             // FLIP weight (PIC-FLIP)
+    float dinv;              // APIC Dinv diagonal (k h^2)^-1, Eigen cofactor inverse
+    float inv_cell_vol;      // 1/(h^3) as the reference computes it (initialize)
+    int32_t rx, ry, rz;      // resolution
+    int32_t nbx, nby, nbz;   // storage blocks per axis = ceil(res/4)
+    int32_t kbx, kby, kbz;   // key blocks per axis = ((res+3)>>2)+1
+    uint32_t escape_key;     // kbx*kby*kbz*64
+    int32_t particle_collision;
+    int64_t plane;           // floats per grid field plane (= nbx*nby*nbz*64)
+    float* grid;             // GF_ALL (or GF_PRIMARY) planes
+    uint32_t* touched;       // one flag per storage block
+};
+
+// Particle SoA (planar: component c of a vector field lives at base + c*N).
+struct ParticlesDev {
+    int64_t n;
+    float* x;      // 3N
+    float* v;      // 3N
+    float* B;      // 9N (APIC) or null
+    float* Fe;     // 9N row-major
+    float* Fp;     // 9N
+    float* m;      // N
+    float* V0;     // N
+    float* mu0;    // N
+    float* lam0;   // N
+    uint16_t* mat; // N
+    uint32_t* pid; // N original particle id
+    const MaterialDev* mats;
+};
+
+// Work items of the block-tiled kernels: a contiguous run of sorted
+// particles that share one key block.
+struct ItemsDev {
+    const uint32_t* sorted_keys;  // N
+    const uint32_t* perm;         // N, sorted position -> storage slot
+    const uint32_t* start;        // item start positions
+    const uint32_t* count;        // device counter: number of items
+    int64_t n;
+};

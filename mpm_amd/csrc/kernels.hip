@@ -1,0 +1,839 @@
+// kernels.hip -- particle/grid kernels of the MI355X explicit MPM step.
+//
+// Work decomposition: particles are sorted by cell (key = 4x4x4 block,
+// then cell in block); a work item is a run of <= kItemMax sorted particles
+// of one block.  P2G and G2P are persistent kernels: each workgroup walks
+// the item list and stages the block's node tile (the block plus the
+// stencil halo, T = 4 + W - 1 nodes per axis) in LDS:
+//   P2G: particles scatter mass, momentum and force into the LDS tile with
+//        ds_add_f32; the tile is flushed to HBM with one global float atomic
+//        per non-zero node value.
+//   G2P: the tile is loaded once per item; in the fused step the grid
+//        velocity update (gravity, p/m, v* = v + dt f/m, boundary collision)
+//        is computed while loading, so there is no separate grid pass.
+#include <hip/hip_runtime.h>
+
+#include "devmath.h"
+#include "launch.h"
+
+namespace mpm {
+
+constexpr int kP2GWG = 128;
+constexpr int kG2PWG = 128;
+
+// ------------------------------------------------------------------ keys
+template <int INTERP>
+__global__ void k_keys(SceneDev S, ParticlesDev P, uint32_t* __restrict__ keys, uint32_t* __restrict__ vals) {
+    int64_t i = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+    if (i >= P.n) return;
+    const int64_t N = P.n;
+    keys[i] = cell_key(S, P.x[i], P.x[N + i], P.x[2 * N + i], Stencil<INTERP>::SHIFT, Stencil<INTERP>::W);
+    vals[i] = (uint32_t)i;
+}
+
+void launch_keys(const SceneDev& S, const ParticlesDev& P, int interp, uint32_t* keys, uint32_t* vals,
+                 hipStream_t st) {
+    if (P.n <= 0) return;
+    dim3 g((unsigned)((P.n + 255) / 256)), b(256);
+    switch (interp) {
+        case MPM_INTERP_LINEAR: hipLaunchKernelGGL(k_keys<MPM_INTERP_LINEAR>, g, b, 0, st, S, P, keys, vals); break;
+        case MPM_INTERP_QUADRATIC_BSPLINE:
+            hipLaunchKernelGGL(k_keys<MPM_INTERP_QUADRATIC_BSPLINE>, g, b, 0, st, S, P, keys, vals);
+            break;
+        default: hipLaunchKernelGGL(k_keys<MPM_INTERP_CUBIC_BSPLINE>, g, b, 0, st, S, P, keys, vals); break;
+    }
+}
+
+// ----------------------------------------------------------------- items
+__device__ __forceinline__ bool item_head(const uint32_t* sk, int64_t i) {
+    return i == 0 || (sk[i] >> 6) != (sk[i - 1] >> 6) || (i % kItemMax) == 0;
+}
+__global__ void k_item_flags(const uint32_t* __restrict__ sk, int64_t n, uint32_t* __restrict__ flags) {
+    int64_t i = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+    if (i < n) flags[i] = item_head(sk, i) ? 1u : 0u;
+}
+__global__ void k_item_compact(const uint32_t* __restrict__ sk, int64_t n, const uint32_t* __restrict__ pos,
+                               uint32_t* __restrict__ items) {
+    int64_t i = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+    if (i < n && item_head(sk, i)) items[pos[i]] = (uint32_t)i;
+}
+
+void launch_items(const uint32_t* sorted_keys, int64_t n, uint32_t* flags, uint32_t* scan_temp, uint32_t* items,
+                  uint32_t* item_count, hipStream_t st) {
+    if (n <= 0) return;
+    dim3 g((unsigned)((n + 255) / 256)), b(256);
+    hipLaunchKernelGGL(k_item_flags, g, b, 0, st, sorted_keys, n, flags);
+    exclusive_scan(flags, n, scan_temp, item_count, st);
+    hipLaunchKernelGGL(k_item_compact, g, b, 0, st, sorted_keys, n, flags, items);
+}
+
+// ------------------------------------------------------- item decoding
+struct ItemGeom {
+    uint32_t s, e;
+    bool escape;
+    int ox, oy, oz;  // tile origin node
+};
+__device__ __forceinline__ ItemGeom item_geom(const SceneDev& S, const ItemsDev& I, uint32_t it, uint32_t nitems) {
+    ItemGeom g;
+    g.s = I.start[it];
+    g.e = (it + 1 < nitems) ? I.start[it + 1] : (uint32_t)I.n;
+    uint32_t blk = I.sorted_keys[g.s] >> 6;
+    g.escape = blk == (S.escape_key >> 6);
+    uint32_t bx = blk % (uint32_t)S.kbx;
+    uint32_t rest = blk / (uint32_t)S.kbx;
+    uint32_t by = rest % (uint32_t)S.kby;
+    uint32_t bz = rest / (uint32_t)S.kby;
+    g.ox = 4 * (int)bx - 4;
+    g.oy = 4 * (int)by - 4;
+    g.oz = 4 * (int)bz - 4;
+    return g;
+}
+
+__device__ __forceinline__ bool in_grid(const SceneDev& S, int i, int j, int k) {
+    return i >= 0 && i < S.rx && j >= 0 && j < S.ry && k >= 0 && k < S.rz;
+}
+
+// ------------------------------------------------------------------ stress
+// -V0 * P(Fe) Fe^T of one particle: plasticLame (src/material_point3d.cu:158-162)
+// with fixed-corotated (:110-128) or Neo-Hookean (:131-155) stress.
+// With Fe = U diag(s) V^T (U, V rotations) the reference's expressions are
+// evaluated in the principal frame, which avoids the cancellation in
+// Fe - R (|Fe - R| ~ 1e-2 for snow):
+//   fixed corotated: 2 mu (Fe - R) Fe^T + la (J-1) J I
+//                  = U diag(2 mu (s_i - 1) s_i) U^T + la (J-1) J I
+//   Neo-Hookean:     mu (Fe Fe^T - I) + la log(J) I
+//                  = U diag(mu (s_i^2 - 1)) U^T + la log(J) I
+// with J = s0 s1 s2 = det Fe.  For Neo-Hookean the singular values are first
+// raised to FLT_EPSILON and the projected Fe is returned in Fe (the
+// reference writes it back into the particle, :142); fe_changed tells the
+// caller to store it.
+__device__ __forceinline__ void particle_stress(float Fe[3][3], const float Fp[3][3], float mu0, float lam0, float V0,
+                                                const MaterialDev& mat, float vs[3][3], bool& fe_changed) {
+    const float Jp = m3_det(Fp);
+    const float factor = expf(mat.hardening * (1.0f - Jp));
+    const float mu = mu0 * factor, la = lam0 * factor;
+    float U[3][3], V[3][3], s[3];
+    svd3(Fe, U, s, V);
+    float d[3], c;
+    if (mat.model == MPM_MODEL_NEO_HOOKEAN) {
+        fe_changed = true;
+#pragma unroll
+        for (int i = 0; i < 3; ++i) s[i] = fmaxf(s[i], MPM_FLT_EPS);
+        float US[3][3];
+#pragma unroll
+        for (int i = 0; i < 3; ++i)
+#pragma unroll
+            for (int j = 0; j < 3; ++j) US[i][j] = U[i][j] * s[j];
+        m3_mul_bt(US, V, Fe);
+        const float J = s[0] * s[1] * s[2];
+#pragma unroll
+        for (int i = 0; i < 3; ++i) d[i] = mu * (s[i] * s[i] - 1.0f);
+        c = la * logf(J);
+    } else {
+        fe_changed = false;
+        const float J = s[0] * s[1] * s[2];
+#pragma unroll
+        for (int i = 0; i < 3; ++i) d[i] = (2.0f * mu) * ((s[i] - 1.0f) * s[i]);
+        c = la * (J - 1.0f) * J;
+    }
+    float P[3][3];
+#pragma unroll
+    for (int i = 0; i < 3; ++i)
+#pragma unroll
+        for (int j = 0; j < 3; ++j)
+            P[i][j] = (U[i][0] * d[0] * U[j][0] + U[i][1] * d[1] * U[j][1]) + U[i][2] * d[2] * U[j][2] +
+                      (i == j ? c : 0.0f);
+#pragma unroll
+    for (int i = 0; i < 3; ++i)
+#pragma unroll
+        for (int j = 0; j < 3; ++j) vs[i][j] = -(V0 * P[i][j]);
+}
+
+// updateDeformationGradient, src/material_point3d.cu:53-70: trial Fe, SVD,
+// clamp singular values to [1-theta_c, 1+theta_s], split into Fe and Fp.
+__device__ __forceinline__ void update_deformation(float Fe[3][3], float Fp[3][3], const float grad[3][3], float dt,
+                                                   const MaterialDev& mat) {
+    float L[3][3], Fn[3][3];
+#pragma unroll
+    for (int i = 0; i < 3; ++i)
+#pragma unroll
+        for (int j = 0; j < 3; ++j) L[i][j] = (i == j ? 1.0f : 0.0f) + dt * grad[i][j];
+    m3_mul(L, Fe, Fn);
+    float F[3][3];
+    m3_mul(Fn, Fp, F);
+    float U[3][3], V[3][3], s[3];
+    svd3(Fn, U, s, V);
+    float hi = 1.0f + mat.crit_s, lo = 1.0f - mat.crit_c;
+    float is[3];
+#pragma unroll
+    for (int i = 0; i < 3; ++i) {
+        s[i] = fmaxf(fminf(s[i], hi), lo);
+        is[i] = 1.0f / s[i];
+    }
+    float US[3][3], VSi[3][3];
+#pragma unroll
+    for (int i = 0; i < 3; ++i)
+#pragma unroll
+        for (int j = 0; j < 3; ++j) {
+            US[i][j] = U[i][j] * s[j];
+            VSi[i][j] = V[i][j] * is[j];
+        }
+    m3_mul_bt(US, V, Fe);
+    float VSiUt[3][3];
+    m3_mul_bt(VSi, U, VSiUt);
+    m3_mul(VSiUt, F, Fp);
+}
+
+__device__ __forceinline__ void load_m3(const float* base, int64_t N, uint32_t p, float M[3][3]) {
+#pragma unroll
+    for (int i = 0; i < 3; ++i)
+#pragma unroll
+        for (int j = 0; j < 3; ++j) M[i][j] = base[(int64_t)(3 * i + j) * N + p];
+}
+__device__ __forceinline__ void store_m3(float* base, int64_t N, uint32_t p, const float M[3][3]) {
+#pragma unroll
+    for (int i = 0; i < 3; ++i)
+#pragma unroll
+        for (int j = 0; j < 3; ++j) base[(int64_t)(3 * i + j) * N + p] = M[i][j];
+}
+
+// -------------------------------------------------------------------- P2G
+// particlesToGrid: src/mpm_solver3d.cu:237-294 (PIC-FLIP),
+// src/apic_mpm_solver3d.cu:245-304 (APIC); MODE 1 is initialize()'s
+// P2GMass (src/mpm_solver3d.cu:136-171).
+template <int INTERP, int APIC, int MODE>
+__global__ void __launch_bounds__(kP2GWG) k_p2g(SceneDev S, ParticlesDev P, ItemsDev I) {
+    constexpr int W = Stencil<INTERP>::W;
+    constexpr int T = 4 + W - 1;
+    constexpr int TN = T * T * T;
+    constexpr int NF = MODE == 0 ? 7 : 1;
+    __shared__ float tile[NF * TN];
+    const uint32_t nitems = *I.count;
+    const int64_t N = P.n;
+    for (uint32_t it = blockIdx.x; it < nitems; it += gridDim.x) {
+        ItemGeom g = item_geom(S, I, it, nitems);
+        if (g.escape) continue;  // no stencil node inside the grid
+        for (int q = threadIdx.x; q < NF * TN; q += kP2GWG) tile[q] = 0.0f;
+        __syncthreads();
+        for (uint32_t i = g.s + threadIdx.x; i < g.e; i += kP2GWG) {
+            const uint32_t p = I.perm[i];
+            const float x0 = P.x[p], x1 = P.x[N + p], x2 = P.x[2 * N + p];
+            int bx, by, bz;
+            float Nx[W], Ny[W], Nz[W], dNx[W], dNy[W], dNz[W];
+            axis_weights<INTERP>(x0, S.ox, S.h, S.inv_h, bx, Nx, dNx);
+            axis_weights<INTERP>(x1, S.oy, S.h, S.inv_h, by, Ny, dNy);
+            axis_weights<INTERP>(x2, S.oz, S.h, S.inv_h, bz, Nz, dNz);
+            const int lx = bx - g.ox, ly = by - g.oy, lz = bz - g.oz;
+            const float m = P.m[p];
+            if (MODE == 1) {
+#pragma unroll
+                for (int a = 0; a < W; ++a)
+#pragma unroll
+                    for (int b = 0; b < W; ++b)
+#pragma unroll
+                        for (int c = 0; c < W; ++c) {
+                            float w = (Nx[a] * Ny[b]) * Nz[c];
+                            atomicAdd(&tile[((lz + c) * T + (ly + b)) * T + (lx + a)], m * w);
+                        }
+                continue;
+            }
+            float Fe[3][3], Fp[3][3], vs[3][3];
+            load_m3(P.Fe, N, p, Fe);
+            load_m3(P.Fp, N, p, Fp);
+            const MaterialDev mat = P.mats[P.mat[p]];
+            bool fe_changed;
+            particle_stress(Fe, Fp, P.mu0[p], P.lam0[p], P.V0[p], mat, vs, fe_changed);
+            if (fe_changed) store_m3(P.Fe, N, p, Fe);
+            const float v0 = P.v[p], v1 = P.v[N + p], v2 = P.v[2 * N + p];
+            float C[3][3];
+            float dxn[3][W];
+            if (APIC) {
+                float Bm[3][3];
+                load_m3(P.B, N, p, Bm);
+#pragma unroll
+                for (int i2 = 0; i2 < 3; ++i2)
+#pragma unroll
+                    for (int j2 = 0; j2 < 3; ++j2) C[i2][j2] = Bm[i2][j2] * S.dinv;
+#pragma unroll
+                for (int j = 0; j < W; ++j) {
+                    dxn[0][j] = (S.ox + S.h * (float)(bx + j)) - x0;
+                    dxn[1][j] = (S.oy + S.h * (float)(by + j)) - x1;
+                    dxn[2][j] = (S.oz + S.h * (float)(bz + j)) - x2;
+                }
+            }
+#pragma unroll
+            for (int a = 0; a < W; ++a)
+#pragma unroll
+                for (int b = 0; b < W; ++b)
+#pragma unroll
+                    for (int c = 0; c < W; ++c) {
+                        const float w = (Nx[a] * Ny[b]) * Nz[c];
+                        const float gx = (dNx[a] * Ny[b]) * Nz[c];
+                        const float gy = (Nx[a] * dNy[b]) * Nz[c];
+                        const float gz = (Nx[a] * Ny[b]) * dNz[c];
+                        const float mip = m * w;
+                        const float fx = (vs[0][0] * gx + vs[0][1] * gy) + vs[0][2] * gz;
+                        const float fy = (vs[1][0] * gx + vs[1][1] * gy) + vs[1][2] * gz;
+                        const float fz = (vs[2][0] * gx + vs[2][1] * gy) + vs[2][2] * gz;
+                        float px, py, pz;
+                        if (APIC) {
+                            const float d0 = dxn[0][a], d1 = dxn[1][b], d2 = dxn[2][c];
+                            px = mip * (v0 + ((C[0][0] * d0 + C[0][1] * d1) + C[0][2] * d2));
+                            py = mip * (v1 + ((C[1][0] * d0 + C[1][1] * d1) + C[1][2] * d2));
+                            pz = mip * (v2 + ((C[2][0] * d0 + C[2][1] * d1) + C[2][2] * d2));
+                        } else {
+                            px = v0 * mip;
+                            py = v1 * mip;
+                            pz = v2 * mip;
+                        }
+                        const int q = ((lz + c) * T + (ly + b)) * T + (lx + a);
+                        atomicAdd(&tile[0 * TN + q], mip);
+                        atomicAdd(&tile[1 * TN + q], px);
+                        atomicAdd(&tile[2 * TN + q], py);
+                        atomicAdd(&tile[3 * TN + q], pz);
+                        atomicAdd(&tile[4 * TN + q], fx);
+                        atomicAdd(&tile[5 * TN + q], fy);
+                        atomicAdd(&tile[6 * TN + q], fz);
+                    }
+        }
+        __syncthreads();
+        for (int q = threadIdx.x; q < TN; q += kP2GWG) {
+            const int lx = q % T, ly = (q / T) % T, lz = q / (T * T);
+            const int gi = g.ox + lx, gj = g.oy + ly, gk = g.oz + lz;
+            if (!in_grid(S, gi, gj, gk)) continue;
+            const int64_t ni = node_index(S, gi, gj, gk);
+            bool any = false;
+#pragma unroll
+            for (int f = 0; f < NF; ++f) {
+                const float val = tile[f * TN + q];
+                if (val != 0.0f) {
+                    unsafeAtomicAdd(&S.grid[(int64_t)f * S.plane + ni], val);
+                    any = true;
+                }
+            }
+            if (any) S.touched[storage_block(S, gi, gj, gk)] = 1u;
+        }
+        __syncthreads();
+    }
+}
+
+template <int INTERP, int APIC>
+static void p2g_dispatch_mode(const SceneDev& S, const ParticlesDev& P, const ItemsDev& I, int mode, int wgs,
+                              hipStream_t st) {
+    if (mode == 0) hipLaunchKernelGGL((k_p2g<INTERP, APIC, 0>), dim3(wgs), dim3(kP2GWG), 0, st, S, P, I);
+    else hipLaunchKernelGGL((k_p2g<INTERP, APIC, 1>), dim3(wgs), dim3(kP2GWG), 0, st, S, P, I);
+}
+template <int INTERP>
+static void p2g_dispatch_apic(const SceneDev& S, const ParticlesDev& P, const ItemsDev& I, int apic, int mode, int wgs,
+                              hipStream_t st) {
+    if (apic) p2g_dispatch_mode<INTERP, 1>(S, P, I, mode, wgs, st);
+    else p2g_dispatch_mode<INTERP, 0>(S, P, I, mode, wgs, st);
+}
+void launch_p2g(const SceneDev& S, const ParticlesDev& P, const ItemsDev& I, int interp, int apic, int mode,
+                int grid_wgs, hipStream_t st) {
+    if (P.n <= 0) return;
+    switch (interp) {
+        case MPM_INTERP_LINEAR: p2g_dispatch_apic<MPM_INTERP_LINEAR>(S, P, I, apic, mode, grid_wgs, st); break;
+        case MPM_INTERP_QUADRATIC_BSPLINE:
+            p2g_dispatch_apic<MPM_INTERP_QUADRATIC_BSPLINE>(S, P, I, apic, mode, grid_wgs, st);
+            break;
+        default: p2g_dispatch_apic<MPM_INTERP_CUBIC_BSPLINE>(S, P, I, apic, mode, grid_wgs, st); break;
+    }
+}
+
+// -------------------------------------------------------------------- G2P
+// gridToParticles: src/mpm_solver3d.cu:316-379 (PIC-FLIP),
+// src/apic_mpm_solver3d.cu:326-388 (APIC), with the optional
+// particlesCollision (:428-445) and advectParticles (:381-391) fused in.
+// SRC 2 is initialize()'s G2PVolume (src/mpm_solver3d.cu:173-214).
+template <int INTERP, int APIC, int SRC>
+__global__ void __launch_bounds__(kG2PWG) k_g2p(SceneDev S, ParticlesDev P, ItemsDev I, int advect, int pcollide) {
+    constexpr int W = Stencil<INTERP>::W;
+    constexpr int T = 4 + W - 1;
+    constexpr int TN = T * T * T;
+    constexpr int NF = SRC == 2 ? 1 : (APIC ? 3 : 6);
+    __shared__ float tile[NF * TN];
+    const uint32_t nitems = *I.count;
+    const int64_t N = P.n;
+    for (uint32_t it = blockIdx.x; it < nitems; it += gridDim.x) {
+        ItemGeom g = item_geom(S, I, it, nitems);
+        if (!g.escape) {
+            for (int q = threadIdx.x; q < TN; q += kG2PWG) {
+                const int lx = q % T, ly = (q / T) % T, lz = q / (T * T);
+                const int gi = g.ox + lx, gj = g.oy + ly, gk = g.oz + lz;
+                float vals[NF];
+#pragma unroll
+                for (int f = 0; f < NF; ++f) vals[f] = 0.0f;
+                if (in_grid(S, gi, gj, gk)) {
+                    const int64_t ni = node_index(S, gi, gj, gk);
+                    const float* G = S.grid;
+                    const int64_t pl = S.plane;
+                    if (SRC == 2) {
+                        vals[0] = G[ni];
+                    } else if (SRC == 1) {
+                        vals[0] = G[(GF_VSTAR + 0) * pl + ni];
+                        vals[1] = G[(GF_VSTAR + 1) * pl + ni];
+                        vals[2] = G[(GF_VSTAR + 2) * pl + ni];
+                        if (!APIC) {
+                            vals[3] = G[(GF_VEL + 0) * pl + ni];
+                            vals[4] = G[(GF_VEL + 1) * pl + ni];
+                            vals[5] = G[(GF_VEL + 2) * pl + ni];
+                        }
+                    } else {
+                        float vx, vy, vz, wx, wy, wz;
+                        grid_node_update(S, gi, gj, gk, G[ni], G[(GF_VEL + 0) * pl + ni], G[(GF_VEL + 1) * pl + ni],
+                                         G[(GF_VEL + 2) * pl + ni], G[(GF_FORCE + 0) * pl + ni],
+                                         G[(GF_FORCE + 1) * pl + ni], G[(GF_FORCE + 2) * pl + ni], vx, vy, vz, wx, wy,
+                                         wz);
+                        vals[0] = wx; vals[1] = wy; vals[2] = wz;
+                        if (!APIC) { vals[3] = vx; vals[4] = vy; vals[5] = vz; }
+                    }
+                }
+#pragma unroll
+                for (int f = 0; f < NF; ++f) tile[f * TN + q] = vals[f];
+            }
+        }
+        __syncthreads();
+        for (uint32_t i = g.s + threadIdx.x; i < g.e; i += kG2PWG) {
+            const uint32_t p = I.perm[i];
+            const float x0 = P.x[p], x1 = P.x[N + p], x2 = P.x[2 * N + p];
+            int bx = 0, by = 0, bz = 0;
+            float Nx[W], Ny[W], Nz[W], dNx[W], dNy[W], dNz[W];
+            if (!g.escape) {
+                axis_weights<INTERP>(x0, S.ox, S.h, S.inv_h, bx, Nx, dNx);
+                axis_weights<INTERP>(x1, S.oy, S.h, S.inv_h, by, Ny, dNy);
+                axis_weights<INTERP>(x2, S.oz, S.h, S.inv_h, bz, Nz, dNz);
+            }
+            const int lx = bx - g.ox, ly = by - g.oy, lz = bz - g.oz;
+            if (SRC == 2) {
+                float density = 0.0f;
+                if (!g.escape) {
+#pragma unroll
+                    for (int a = 0; a < W; ++a)
+#pragma unroll
+                        for (int b = 0; b < W; ++b)
+#pragma unroll
+                            for (int c = 0; c < W; ++c) {
+                                const float w = (Nx[a] * Ny[b]) * Nz[c];
+                                const float mi = tile[((lz + c) * T + (ly + b)) * T + (lx + a)];
+                                density += (mi * w) * S.inv_cell_vol;
+                            }
+                }
+                P.V0[p] = P.m[p] / density;
+                continue;
+            }
+            const float vp0 = APIC ? 0.0f : P.v[p];
+            const float vp1 = APIC ? 0.0f : P.v[N + p];
+            const float vp2 = APIC ? 0.0f : P.v[2 * N + p];
+            float vpic0 = 0.0f, vpic1 = 0.0f, vpic2 = 0.0f;
+            float vf0 = vp0, vf1 = vp1, vf2 = vp2;
+            float grad[3][3], Bm[3][3];
+#pragma unroll
+            for (int i2 = 0; i2 < 3; ++i2)
+#pragma unroll
+                for (int j2 = 0; j2 < 3; ++j2) { grad[i2][j2] = 0.0f; Bm[i2][j2] = 0.0f; }
+            if (!g.escape) {
+                float dxn[3][W];
+                if (APIC) {
+#pragma unroll
+                    for (int j = 0; j < W; ++j) {
+                        dxn[0][j] = (S.ox + S.h * (float)(bx + j)) - x0;
+                        dxn[1][j] = (S.oy + S.h * (float)(by + j)) - x1;
+                        dxn[2][j] = (S.oz + S.h * (float)(bz + j)) - x2;
+                    }
+                }
+#pragma unroll
+                for (int a = 0; a < W; ++a)
+#pragma unroll
+                    for (int b = 0; b < W; ++b)
+#pragma unroll
+                        for (int c = 0; c < W; ++c) {
+                            const float w = (Nx[a] * Ny[b]) * Nz[c];
+                            const float gx = (dNx[a] * Ny[b]) * Nz[c];
+                            const float gy = (Nx[a] * dNy[b]) * Nz[c];
+                            const float gz = (Nx[a] * Ny[b]) * dNz[c];
+                            const int q = ((lz + c) * T + (ly + b)) * T + (lx + a);
+                            const float w0 = tile[q], w1 = tile[TN + q], w2 = tile[2 * TN + q];
+                            const float vi0 = w0 * w, vi1 = w1 * w, vi2 = w2 * w;
+                            vpic0 += vi0; vpic1 += vi1; vpic2 += vi2;
+                            if (!APIC) {
+                                vf0 += (w0 - tile[3 * TN + q]) * w;
+                                vf1 += (w1 - tile[4 * TN + q]) * w;
+                                vf2 += (w2 - tile[5 * TN + q]) * w;
+                            }
+                            grad[0][0] += w0 * gx; grad[0][1] += w0 * gy; grad[0][2] += w0 * gz;
+                            grad[1][0] += w1 * gx; grad[1][1] += w1 * gy; grad[1][2] += w1 * gz;
+                            grad[2][0] += w2 * gx; grad[2][1] += w2 * gy; grad[2][2] += w2 * gz;
+                            if (APIC) {
+                                const float d0 = dxn[0][a], d1 = dxn[1][b], d2 = dxn[2][c];
+                                Bm[0][0] += vi0 * d0; Bm[0][1] += vi0 * d1; Bm[0][2] += vi0 * d2;
+                                Bm[1][0] += vi1 * d0; Bm[1][1] += vi1 * d1; Bm[1][2] += vi1 * d2;
+                                Bm[2][0] += vi2 * d0; Bm[2][1] += vi2 * d1; Bm[2][2] += vi2 * d2;
+                            }
+                        }
+            }
+            float n0, n1, n2;
+            if (APIC) {
+                n0 = vpic0; n1 = vpic1; n2 = vpic2;
+            } else {
+                const float ob = 1.0f - S.blend;
+                n0 = S.blend * vf0 + ob * vpic0;
+                n1 = S.blend * vf1 + ob * vpic1;
+                n2 = S.blend * vf2 + ob * vpic2;
+            }
+            float Fe[3][3], Fp[3][3];
+            load_m3(P.Fe, N, p, Fe);
+            load_m3(P.Fp, N, p, Fp);
+            const MaterialDev mat = P.mats[P.mat[p]];
+            update_deformation(Fe, Fp, grad, S.dt, mat);
+            store_m3(P.Fe, N, p, Fe);
+            store_m3(P.Fp, N, p, Fp);
+            if (APIC) store_m3(P.B, N, p, Bm);
+            if (pcollide) boundary_collision(S, x0 + S.dt * n0, x1 + S.dt * n1, x2 + S.dt * n2, n0, n1, n2);
+            P.v[p] = n0;
+            P.v[N + p] = n1;
+            P.v[2 * N + p] = n2;
+            if (advect) {
+                P.x[p] = x0 + S.dt * n0;
+                P.x[N + p] = x1 + S.dt * n1;
+                P.x[2 * N + p] = x2 + S.dt * n2;
+            }
+        }
+        __syncthreads();
+    }
+}
+
+template <int INTERP, int APIC>
+static void g2p_dispatch_src(const SceneDev& S, const ParticlesDev& P, const ItemsDev& I, int src, int advect,
+                             int pcollide, int wgs, hipStream_t st) {
+    if (src == 0)
+        hipLaunchKernelGGL((k_g2p<INTERP, APIC, 0>), dim3(wgs), dim3(kG2PWG), 0, st, S, P, I, advect, pcollide);
+    else if (src == 1)
+        hipLaunchKernelGGL((k_g2p<INTERP, APIC, 1>), dim3(wgs), dim3(kG2PWG), 0, st, S, P, I, advect, pcollide);
+    else
+        hipLaunchKernelGGL((k_g2p<INTERP, APIC, 2>), dim3(wgs), dim3(kG2PWG), 0, st, S, P, I, advect, pcollide);
+}
+template <int INTERP>
+static void g2p_dispatch_apic(const SceneDev& S, const ParticlesDev& P, const ItemsDev& I, int apic, int src,
+                              int advect, int pcollide, int wgs, hipStream_t st) {
+    if (apic) g2p_dispatch_src<INTERP, 1>(S, P, I, src, advect, pcollide, wgs, st);
+    else g2p_dispatch_src<INTERP, 0>(S, P, I, src, advect, pcollide, wgs, st);
+}
+void launch_g2p(const SceneDev& S, const ParticlesDev& P, const ItemsDev& I, int interp, int apic, int src, int advect,
+                int pcollide, int grid_wgs, hipStream_t st) {
+    if (P.n <= 0) return;
+    switch (interp) {
+        case MPM_INTERP_LINEAR:
+            g2p_dispatch_apic<MPM_INTERP_LINEAR>(S, P, I, apic, src, advect, pcollide, grid_wgs, st);
+            break;
+        case MPM_INTERP_QUADRATIC_BSPLINE:
+            g2p_dispatch_apic<MPM_INTERP_QUADRATIC_BSPLINE>(S, P, I, apic, src, advect, pcollide, grid_wgs, st);
+            break;
+        default:
+            g2p_dispatch_apic<MPM_INTERP_CUBIC_BSPLINE>(S, P, I, apic, src, advect, pcollide, grid_wgs, st);
+            break;
+    }
+}
+
+// ------------------------------------------------------------ grid passes
+// One wave per touched storage block (64 nodes, one per lane); the count of
+// nodes with mass > 0 is reduced per workgroup and added with one atomic.
+__device__ __forceinline__ void wg_count_add(unsigned long long* dst, unsigned long long mine) {
+    __shared__ unsigned long long part[4];
+    const int lane = threadIdx.x & 63, wave = threadIdx.x >> 6;
+    if (lane == 0) part[wave] = mine;
+    __syncthreads();
+    if (threadIdx.x == 0) {
+        unsigned long long t = 0;
+        for (int w = 0; w < (int)(blockDim.x >> 6); ++w) t += part[w];
+        if (t) atomicAdd(dst, t);
+    }
+}
+
+__global__ void __launch_bounds__(256) k_clear(SceneDev S, int64_t nsb, int nfields, unsigned long long* active) {
+    const int lane = threadIdx.x & 63;
+    const int64_t wave = ((int64_t)blockIdx.x * blockDim.x + threadIdx.x) >> 6;
+    const int64_t nwaves = ((int64_t)gridDim.x * blockDim.x) >> 6;
+    unsigned long long cnt = 0;
+    for (int64_t b = wave; b < nsb; b += nwaves) {
+        if (S.touched[b] == 0u) continue;
+        const int64_t ni = b * 64 + lane;
+        const float m = S.grid[ni];
+        cnt += (unsigned long long)__popcll(__ballot(m > 0.0f));
+        for (int f = 0; f < nfields; ++f) S.grid[(int64_t)f * S.plane + ni] = 0.0f;
+        if (lane == 0) S.touched[b] = 0u;
+    }
+    if (active) wg_count_add(active, cnt);
+}
+__global__ void __launch_bounds__(256) k_count_active(SceneDev S, int64_t nsb, unsigned long long* active) {
+    const int lane = threadIdx.x & 63;
+    const int64_t wave = ((int64_t)blockIdx.x * blockDim.x + threadIdx.x) >> 6;
+    const int64_t nwaves = ((int64_t)gridDim.x * blockDim.x) >> 6;
+    unsigned long long cnt = 0;
+    for (int64_t b = wave; b < nsb; b += nwaves) {
+        if (S.touched[b] == 0u) continue;
+        cnt += (unsigned long long)__popcll(__ballot(S.grid[b * 64 + lane] > 0.0f));
+    }
+    wg_count_add(active, cnt);
+}
+
+static unsigned grid_pass_wgs(int64_t nsb) {
+    int64_t w = (nsb + 3) / 4;  // 4 waves per 256-thread workgroup
+    if (w > 2048) w = 2048;
+    if (w < 1) w = 1;
+    return (unsigned)w;
+}
+
+void launch_clear(const SceneDev& S, int64_t nsb, int nfields, unsigned long long* active, hipStream_t st) {
+    hipLaunchKernelGGL(k_clear, dim3(grid_pass_wgs(nsb)), dim3(256), 0, st, S, nsb, nfields, active);
+}
+void launch_count_active(const SceneDev& S, int64_t nsb, unsigned long long* active, hipStream_t st) {
+    hipLaunchKernelGGL(k_count_active, dim3(grid_pass_wgs(nsb)), dim3(256), 0, st, S, nsb, active);
+}
+
+// Unfused grid sub-steps over touched blocks (untouched nodes hold zero
+// mass and zero v*, for which all three operations are no-ops).
+__global__ void k_grid_op(SceneDev S, int64_t nsb, int op, float dt) {
+    const int lane = threadIdx.x & 63;
+    const int64_t wave = ((int64_t)blockIdx.x * blockDim.x + threadIdx.x) >> 6;
+    const int64_t nwaves = ((int64_t)gridDim.x * blockDim.x) >> 6;
+    for (int64_t b = wave; b < nsb; b += nwaves) {
+        if (S.touched[b] == 0u) continue;
+        const int64_t bx = b % S.nbx, by = (b / S.nbx) % S.nby, bz = b / ((int64_t)S.nbx * S.nby);
+        const int i = (int)(4 * bx) + (lane & 3), j = (int)(4 * by) + ((lane >> 2) & 3), k = (int)(4 * bz) + (lane >> 4);
+        if (i >= S.rx || j >= S.ry || k >= S.rz) continue;
+        const int64_t ni = b * 64 + lane;
+        float* G = S.grid;
+        const int64_t pl = S.plane;
+        const float m = G[ni];
+        if (op == 0) {  // computeGravityForces, src/mpm_solver3d.cu:394-403
+            if (m > MPM_FLT_EPS) G[(GF_FORCE + 1) * pl + ni] = fmaf(-S.gravity, m, G[(GF_FORCE + 1) * pl + ni]);
+        } else if (op == 1) {  // updateVelocity, src/grid3d.cu:36-41
+            if (m > MPM_FLT_EPS) {
+#pragma unroll
+                for (int a = 0; a < 3; ++a) {
+                    const float v = G[(GF_VEL + a) * pl + ni] / m;
+                    G[(GF_VEL + a) * pl + ni] = v;
+                    G[(GF_VSTAR + a) * pl + ni] = v + (dt * G[(GF_FORCE + a) * pl + ni]) / m;
+                }
+            }
+        } else {  // gridCollision, src/mpm_solver3d.cu:405-426
+            float wx = G[(GF_VSTAR + 0) * pl + ni], wy = G[(GF_VSTAR + 1) * pl + ni], wz = G[(GF_VSTAR + 2) * pl + ni];
+            const float gx = (S.ox + S.h * (float)i) + dt * wx;
+            const float gy = (S.oy + S.h * (float)j) + dt * wy;
+            const float gz = (S.oz + S.h * (float)k) + dt * wz;
+            boundary_collision(S, gx, gy, gz, wx, wy, wz);
+            G[(GF_VSTAR + 0) * pl + ni] = wx;
+            G[(GF_VSTAR + 1) * pl + ni] = wy;
+            G[(GF_VSTAR + 2) * pl + ni] = wz;
+        }
+    }
+}
+void launch_grid_op(const SceneDev& S, int64_t nsb, int op, float dt, hipStream_t st) {
+    hipLaunchKernelGGL(k_grid_op, dim3(grid_pass_wgs(nsb)), dim3(256), 0, st, S, nsb, op, dt);
+}
+
+// ------------------------------------------------------- particle passes
+__global__ void k_particles_collision(SceneDev S, ParticlesDev P, float dt) {
+    int64_t p = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+    if (p >= P.n) return;
+    const int64_t N = P.n;
+    float v0 = P.v[p], v1 = P.v[N + p], v2 = P.v[2 * N + p];
+    boundary_collision(S, P.x[p] + dt * v0, P.x[N + p] + dt * v1, P.x[2 * N + p] + dt * v2, v0, v1, v2);
+    P.v[p] = v0; P.v[N + p] = v1; P.v[2 * N + p] = v2;
+}
+void launch_particles_collision(const SceneDev& S, const ParticlesDev& P, float dt, hipStream_t st) {
+    if (P.n <= 0) return;
+    hipLaunchKernelGGL(k_particles_collision, dim3((unsigned)((P.n + 255) / 256)), dim3(256), 0, st, S, P, dt);
+}
+__global__ void k_advect(ParticlesDev P, float dt) {
+    int64_t p = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+    if (p >= P.n) return;
+    const int64_t N = P.n;
+#pragma unroll
+    for (int a = 0; a < 3; ++a) P.x[a * N + p] = P.x[a * N + p] + dt * P.v[a * N + p];
+}
+void launch_advect(const ParticlesDev& P, float dt, hipStream_t st) {
+    if (P.n <= 0) return;
+    hipLaunchKernelGGL(k_advect, dim3((unsigned)((P.n + 255) / 256)), dim3(256), 0, st, P, dt);
+}
+
+// ----------------------------------------------------- layout conversion
+__global__ void k_pack(HostLayoutDev H, ParticlesDev P) {
+    int64_t s = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+    if (s >= P.n) return;
+    const int64_t N = P.n;
+    const int64_t id = P.pid[s];
+    P.V0[s] = H.volume0[id];
+    P.m[s] = H.mass[id];
+    P.mu0[s] = H.mu0[id];
+    P.lam0[s] = H.lam0[id];
+    P.mat[s] = H.mat[id];
+#pragma unroll
+    for (int a = 0; a < 3; ++a) {
+        P.x[a * N + s] = H.pos[3 * id + a];
+        P.v[a * N + s] = H.vel[3 * id + a];
+    }
+#pragma unroll
+    for (int k = 0; k < 9; ++k) {
+        P.Fe[k * N + s] = H.Fe[9 * id + k];
+        P.Fp[k * N + s] = H.Fp[9 * id + k];
+        if (P.B) P.B[k * N + s] = H.B ? H.B[9 * id + k] : 0.0f;
+    }
+}
+void launch_pack(const HostLayoutDev& H, const ParticlesDev& P, hipStream_t st) {
+    if (P.n <= 0) return;
+    hipLaunchKernelGGL(k_pack, dim3((unsigned)((P.n + 255) / 256)), dim3(256), 0, st, H, P);
+}
+__global__ void k_unpack(ParticlesDev P, HostLayoutDev H) {
+    int64_t s = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+    if (s >= P.n) return;
+    const int64_t N = P.n;
+    const int64_t id = P.pid[s];
+    H.volume0[id] = P.V0[s];
+    H.mass[id] = P.m[s];
+    H.mu0[id] = P.mu0[s];
+    H.lam0[id] = P.lam0[s];
+    H.mat[id] = P.mat[s];
+#pragma unroll
+    for (int a = 0; a < 3; ++a) {
+        H.pos[3 * id + a] = P.x[a * N + s];
+        H.vel[3 * id + a] = P.v[a * N + s];
+    }
+#pragma unroll
+    for (int k = 0; k < 9; ++k) {
+        H.Fe[9 * id + k] = P.Fe[k * N + s];
+        H.Fp[9 * id + k] = P.Fp[k * N + s];
+        if (H.B) H.B[9 * id + k] = P.B ? P.B[k * N + s] : 0.0f;
+    }
+}
+void launch_unpack(const ParticlesDev& P, const HostLayoutDev& H, hipStream_t st) {
+    if (P.n <= 0) return;
+    hipLaunchKernelGGL(k_unpack, dim3((unsigned)((P.n + 255) / 256)), dim3(256), 0, st, P, H);
+}
+__global__ void k_positions4(ParticlesDev P, float* out) {
+    int64_t s = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+    if (s >= P.n) return;
+    const int64_t N = P.n;
+    const int64_t id = P.pid[s];
+    out[4 * id + 0] = P.x[s];
+    out[4 * id + 1] = P.x[N + s];
+    out[4 * id + 2] = P.x[2 * N + s];
+    out[4 * id + 3] = 1.0f;
+}
+void launch_positions4(const ParticlesDev& P, float* out, hipStream_t st) {
+    if (P.n <= 0) return;
+    hipLaunchKernelGGL(k_positions4, dim3((unsigned)((P.n + 255) / 256)), dim3(256), 0, st, P, out);
+}
+// Physical reorder: dst slot i <- src slot perm[i].
+__global__ void k_permute(ParticlesDev S, ParticlesDev D, const uint32_t* __restrict__ perm) {
+    int64_t i = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+    if (i >= S.n) return;
+    const int64_t N = S.n;
+    const int64_t s = perm[i];
+    D.V0[i] = S.V0[s];
+    D.m[i] = S.m[s];
+    D.mu0[i] = S.mu0[s];
+    D.lam0[i] = S.lam0[s];
+    D.mat[i] = S.mat[s];
+    D.pid[i] = S.pid[s];
+#pragma unroll
+    for (int a = 0; a < 3; ++a) {
+        D.x[a * N + i] = S.x[a * N + s];
+        D.v[a * N + i] = S.v[a * N + s];
+    }
+#pragma unroll
+    for (int k = 0; k < 9; ++k) {
+        D.Fe[k * N + i] = S.Fe[k * N + s];
+        D.Fp[k * N + i] = S.Fp[k * N + s];
+        if (S.B) D.B[k * N + i] = S.B[k * N + s];
+    }
+}
+void launch_permute(const ParticlesDev& src, const ParticlesDev& dst, const uint32_t* perm, hipStream_t st) {
+    if (src.n <= 0) return;
+    hipLaunchKernelGGL(k_permute, dim3((unsigned)((src.n + 255) / 256)), dim3(256), 0, st, src, dst, perm);
+}
+__global__ void k_gather_u32(const uint32_t* __restrict__ src, const uint32_t* __restrict__ idx, uint32_t* dst,
+                             int64_t n) {
+    int64_t i = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+    if (i < n) dst[i] = src[idx[i]];
+}
+void launch_gather_u32(const uint32_t* src, const uint32_t* idx, uint32_t* dst, int64_t n, hipStream_t st) {
+    if (n <= 0) return;
+    hipLaunchKernelGGL(k_gather_u32, dim3((unsigned)((n + 255) / 256)), dim3(256), 0, st, src, idx, dst, n);
+}
+
+// Dense reference-order grid copy (linear index x + rx*(y + ry*z)).
+__global__ void k_grid_dense(SceneDev S, float* mass, float* vel, float* vstar, float* force, int have_vstar) {
+    int64_t i = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+    const int64_t total = (int64_t)S.rx * S.ry * S.rz;
+    if (i >= total) return;
+    const int x = (int)(i % S.rx), y = (int)((i / S.rx) % S.ry), z = (int)(i / ((int64_t)S.rx * S.ry));
+    const int64_t ni = node_index(S, x, y, z);
+    const float* G = S.grid;
+    const int64_t pl = S.plane;
+    if (mass) mass[i] = G[ni];
+#pragma unroll
+    for (int a = 0; a < 3; ++a) {
+        if (vel) vel[3 * i + a] = G[(GF_VEL + a) * pl + ni];
+        if (force) force[3 * i + a] = G[(GF_FORCE + a) * pl + ni];
+        if (vstar) vstar[3 * i + a] = have_vstar ? G[(GF_VSTAR + a) * pl + ni] : 0.0f;
+    }
+}
+void launch_grid_dense(const SceneDev& S, float* mass, float* vel, float* vstar, float* force, int have_vstar,
+                       hipStream_t st) {
+    const int64_t total = (int64_t)S.rx * S.ry * S.rz;
+    hipLaunchKernelGGL(k_grid_dense, dim3((unsigned)((total + 255) / 256)), dim3(256), 0, st, S, mass, vel, vstar,
+                       force, have_vstar);
+}
+
+__global__ void __launch_bounds__(256) k_count_escaped(const uint32_t* __restrict__ sk, int64_t n, uint32_t esc,
+                                                       unsigned long long* out) {
+    unsigned long long cnt = 0;
+    for (int64_t i = (int64_t)blockIdx.x * blockDim.x + threadIdx.x; i - threadIdx.x < n;
+         i += (int64_t)gridDim.x * blockDim.x)
+        cnt += (unsigned long long)__popcll(__ballot(i < n && sk[i] == esc));
+    wg_count_add(out, (threadIdx.x & 63) == 0 ? cnt : 0ull);
+}
+void launch_count_escaped(const uint32_t* sorted_keys, int64_t n, uint32_t escape_key, unsigned long long* out,
+                          hipStream_t st) {
+    if (n <= 0) return;
+    int64_t w = (n + 255) / 256;
+    if (w > 1024) w = 1024;
+    hipLaunchKernelGGL(k_count_escaped, dim3((unsigned)w), dim3(256), 0, st, sorted_keys, n, escape_key, out);
+}
+
+__global__ void k_debug_svd(const float* A, int64_t n, float* U, float* s, float* V) {
+    int64_t i = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+    if (i >= n) return;
+    float a[3][3], u[3][3], v[3][3], sv[3];
+#pragma unroll
+    for (int r = 0; r < 3; ++r)
+#pragma unroll
+        for (int c = 0; c < 3; ++c) a[r][c] = A[9 * i + 3 * r + c];
+    svd3(a, u, sv, v);
+#pragma unroll
+    for (int r = 0; r < 3; ++r) {
+        s[3 * i + r] = sv[r];
+#pragma unroll
+        for (int c = 0; c < 3; ++c) {
+            U[9 * i + 3 * r + c] = u[r][c];
+            V[9 * i + 3 * r + c] = v[r][c];
+        }
+    }
+}
+void launch_debug_svd(const float* A, int64_t n, float* U, float* s, float* V, hipStream_t st) {
+    if (n <= 0) return;
+    hipLaunchKernelGGL(k_debug_svd, dim3((unsigned)((n + 127) / 128)), dim3(128), 0, st, A, n, U, s, V);
+}
+
+__global__ void k_fill_u32(uint32_t* p, uint32_t v, int64_t n) {
+    int64_t i = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+    if (i < n) p[i] = v;
+}
+void launch_fill_u32(uint32_t* p, uint32_t v, int64_t n, hipStream_t st) {
+    if (n <= 0) return;
+    hipLaunchKernelGGL(k_fill_u32, dim3((unsigned)((n + 255) / 256)), dim3(256), 0, st, p, v, n);
+}
+
+}  // namespace mpm

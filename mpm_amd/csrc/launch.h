@@ -1,0 +1,74 @@
+// launch.h -- host-side launchers of the device kernels (one TU per kernel
+// family; the C-ABI in capi.hip calls these).
+#pragma once
+
+#include <hip/hip_runtime.h>
+#include <stdint.h>
+
+#include "common.h"
+
+namespace mpm {
+
+// Optional per-kernel timing: begin/end bracket the launches of one named
+// kernel on the solver's stream.
+struct ProfileHook {
+    void* ctx = nullptr;
+    void (*fn_begin)(void*, const char*) = nullptr;
+    void (*fn_end)(void*, const char*) = nullptr;
+    void begin(const char* n) const { if (fn_begin) fn_begin(ctx, n); }
+    void end(const char* n) const { if (fn_end) fn_end(ctx, n); }
+};
+
+// ---- sort.hip
+int64_t scan_temp_words(int64_t n);
+void exclusive_scan(uint32_t* data, int64_t n, uint32_t* temp, uint32_t* total, hipStream_t st);
+int64_t radix_tiles(int64_t n);
+int64_t radix_hist_words(int64_t n);
+int radix_passes(int key_bits);
+int radix_sort_pairs(uint32_t* k0, uint32_t* v0, uint32_t* k1, uint32_t* v1, int64_t n, int key_bits, uint32_t* hist,
+                     uint32_t* scan_temp, hipStream_t st, const ProfileHook& prof);
+
+// ---- kernels.hip
+// Persistent launch width of the item kernels.
+constexpr int kItemMax = 512;   // particles per work item (max)
+
+void launch_keys(const SceneDev& S, const ParticlesDev& P, int interp, uint32_t* keys, uint32_t* vals,
+                 hipStream_t st);
+void launch_items(const uint32_t* sorted_keys, int64_t n, uint32_t* flags, uint32_t* scan_temp, uint32_t* items,
+                  uint32_t* item_count, hipStream_t st);
+
+// mode: 0 = full P2G (mass, momentum, force), 1 = mass only (initialize)
+void launch_p2g(const SceneDev& S, const ParticlesDev& P, const ItemsDev& I, int interp, int apic, int mode,
+                int grid_wgs, hipStream_t st);
+// src: 0 = fused grid update from P2G fields, 1 = materialised v / v*,
+//      2 = initialize() volume gather
+void launch_g2p(const SceneDev& S, const ParticlesDev& P, const ItemsDev& I, int interp, int apic, int src,
+                int advect, int pcollide, int grid_wgs, hipStream_t st);
+
+void launch_clear(const SceneDev& S, int64_t storage_blocks, int nfields, unsigned long long* active_nodes,
+                  hipStream_t st);
+void launch_count_active(const SceneDev& S, int64_t storage_blocks, unsigned long long* active_nodes,
+                         hipStream_t st);
+// op: 0 gravity, 1 update velocities, 2 grid collision (touched blocks only)
+void launch_grid_op(const SceneDev& S, int64_t storage_blocks, int op, float dt, hipStream_t st);
+void launch_particles_collision(const SceneDev& S, const ParticlesDev& P, float dt, hipStream_t st);
+void launch_advect(const ParticlesDev& P, float dt, hipStream_t st);
+
+// Host-layout (interleaved, original order) <-> device SoA.
+struct HostLayoutDev {
+    float *volume0, *mass, *pos, *vel, *Fe, *Fp, *mu0, *lam0, *B;
+    uint16_t* mat;
+};
+void launch_pack(const HostLayoutDev& H, const ParticlesDev& P, hipStream_t st);
+void launch_unpack(const ParticlesDev& P, const HostLayoutDev& H, hipStream_t st);
+void launch_positions4(const ParticlesDev& P, float* out, hipStream_t st);
+void launch_permute(const ParticlesDev& src, const ParticlesDev& dst, const uint32_t* perm, hipStream_t st);
+void launch_gather_u32(const uint32_t* src, const uint32_t* idx, uint32_t* dst, int64_t n, hipStream_t st);
+void launch_grid_dense(const SceneDev& S, float* mass, float* vel, float* vstar, float* force, int have_vstar,
+                       hipStream_t st);
+void launch_count_escaped(const uint32_t* sorted_keys, int64_t n, uint32_t escape_key, unsigned long long* out,
+                          hipStream_t st);
+void launch_debug_svd(const float* A, int64_t n, float* U, float* s, float* V, hipStream_t st);
+void launch_fill_u32(uint32_t* p, uint32_t v, int64_t n, hipStream_t st);
+
+}  // namespace mpm

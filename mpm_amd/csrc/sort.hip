@@ -1,0 +1,280 @@
+// sort.hip -- stable LSD radix sort of (key, value) uint32 pairs and a
+// device-wide exclusive scan, written for wave64 (gfx950).
+//
+// One pass over an 8-bit digit is three launches:
+//   hist    : per-tile digit histogram, stored digit-major (digit * tiles + tile)
+//   scan    : exclusive scan of that table -> global base of (digit, tile)
+//   scatter : per-tile stable ranking (64-key chunks ranked with 8 ballots,
+//             chunk prefix sums in LDS), staged in LDS in digit order, then
+//             written out so that runs of one digit are contiguous stores.
+// Stability follows from ranking by (tile, chunk, lane) = input order.
+// No look-back spinning: every wave runs to completion independently.
+#include <hip/hip_runtime.h>
+#include <stdint.h>
+
+#include "launch.h"
+
+namespace mpm {
+
+constexpr int kRadixBits = 8;
+constexpr int kRadix = 1 << kRadixBits;
+constexpr int kSortWG = 256;
+constexpr int kSortIPT = 8;
+constexpr int kSortTile = kSortWG * kSortIPT;  // 2048 keys per tile
+constexpr int kSortChunks = kSortTile / 64;    // 32 chunks of 64 keys
+constexpr int kChunksPerWave = kSortChunks / (kSortWG / 64);
+
+__device__ __forceinline__ uint64_t digit_match(uint32_t digit, uint64_t active) {
+    uint64_t m = active;
+#pragma unroll
+    for (int b = 0; b < kRadixBits; ++b) {
+        bool bit = (digit >> b) & 1u;
+        uint64_t bb = __ballot(bit);
+        m &= bit ? bb : ~bb;
+    }
+    return m;
+}
+
+__global__ void __launch_bounds__(kSortWG) k_radix_hist(const uint32_t* __restrict__ keys, int64_t n, int shift,
+                                                        uint32_t mask, uint32_t* __restrict__ hist, int64_t tiles) {
+    __shared__ uint32_t h[kRadix];
+    const int tid = threadIdx.x;
+    const int lane = tid & 63;
+    for (int i = tid; i < kRadix; i += kSortWG) h[i] = 0;
+    __syncthreads();
+    const int64_t tile = blockIdx.x;
+    const int64_t base = tile * kSortTile;
+    const int wave = tid >> 6;
+#pragma unroll
+    for (int c = 0; c < kChunksPerWave; ++c) {
+        int64_t idx = base + (int64_t)(wave * kChunksPerWave + c) * 64 + lane;
+        bool valid = idx < n;
+        uint32_t digit = valid ? ((keys[idx] >> shift) & mask) : 0u;
+        uint64_t active = __ballot(valid);
+        if (active == 0) break;
+        uint64_t m = digit_match(digit, active);
+        int leader = __ffsll((unsigned long long)m) - 1;
+        if (valid && lane == leader) atomicAdd(&h[digit], (uint32_t)__popcll(m));
+    }
+    __syncthreads();
+    for (int d = tid; d < kRadix; d += kSortWG) hist[(int64_t)d * tiles + tile] = h[d];
+}
+
+__global__ void __launch_bounds__(kSortWG) k_radix_scatter(const uint32_t* __restrict__ keys_in,
+                                                           const uint32_t* __restrict__ vals_in,
+                                                           uint32_t* __restrict__ keys_out,
+                                                           uint32_t* __restrict__ vals_out, int64_t n, int shift,
+                                                           uint32_t mask, const uint32_t* __restrict__ hist_scan,
+                                                           int64_t tiles) {
+    __shared__ uint16_t chunk_off[kSortChunks][kRadix];  // 16 KB
+    __shared__ uint32_t digit_base[kRadix];
+    __shared__ uint32_t global_base[kRadix];
+    __shared__ uint32_t stage_k[kSortTile];
+    __shared__ uint32_t stage_v[kSortTile];
+    __shared__ uint32_t wave_sums[kSortWG / 64];
+
+    const int tid = threadIdx.x;
+    const int lane = tid & 63;
+    const int wave = tid >> 6;
+    const int64_t tile = blockIdx.x;
+    const int64_t base = tile * kSortTile;
+    const int tile_count = (int)((n - base) < kSortTile ? (n - base) : kSortTile);
+
+    uint32_t* co32 = reinterpret_cast<uint32_t*>(&chunk_off[0][0]);
+    for (int i = tid; i < kSortChunks * kRadix / 2; i += kSortWG) co32[i] = 0u;
+    __syncthreads();
+
+    uint32_t my_key[kChunksPerWave], my_val[kChunksPerWave], my_digit[kChunksPerWave], my_rank[kChunksPerWave];
+#pragma unroll
+    for (int c = 0; c < kChunksPerWave; ++c) {
+        int chunk = wave * kChunksPerWave + c;
+        int64_t idx = base + (int64_t)chunk * 64 + lane;
+        bool valid = idx < n;
+        uint32_t k = valid ? keys_in[idx] : 0u;
+        uint32_t v = valid ? vals_in[idx] : 0u;
+        uint32_t digit = (k >> shift) & mask;
+        uint64_t active = __ballot(valid);
+        uint64_t m = digit_match(digit, active);
+        uint64_t lt = (lane == 0) ? 0ull : ((~0ull) >> (64 - lane));
+        my_rank[c] = (uint32_t)__popcll(m & lt);
+        int leader = __ffsll((unsigned long long)m) - 1;
+        if (valid && lane == leader) chunk_off[chunk][digit] = (uint16_t)__popcll(m);
+        my_key[c] = k;
+        my_val[c] = v;
+        my_digit[c] = valid ? digit : 0xFFFFFFFFu;
+    }
+    __syncthreads();
+
+    // Per digit: exclusive prefix over chunks (thread d owns digit d).
+    uint32_t total = 0;
+    if (tid < kRadix) {
+        for (int c = 0; c < kSortChunks; ++c) {
+            uint32_t cnt = chunk_off[c][tid];
+            chunk_off[c][tid] = (uint16_t)total;
+            total += cnt;
+        }
+        global_base[tid] = hist_scan[(int64_t)tid * tiles + tile];
+    }
+    // Block exclusive scan of the per-digit totals (kRadix == kSortWG).
+    uint32_t incl = total;
+#pragma unroll
+    for (int off = 1; off < 64; off <<= 1) {
+        uint32_t up = __shfl_up(incl, off, 64);
+        if (lane >= off) incl += up;
+    }
+    if (lane == 63) wave_sums[wave] = incl;
+    __syncthreads();
+    uint32_t wave_prefix = 0;
+    for (int w = 0; w < wave; ++w) wave_prefix += wave_sums[w];
+    if (tid < kRadix) digit_base[tid] = wave_prefix + incl - total;
+    __syncthreads();
+
+#pragma unroll
+    for (int c = 0; c < kChunksPerWave; ++c) {
+        if (my_digit[c] == 0xFFFFFFFFu) continue;
+        int chunk = wave * kChunksPerWave + c;
+        uint32_t d = my_digit[c];
+        uint32_t pos = digit_base[d] + chunk_off[chunk][d] + my_rank[c];
+        stage_k[pos] = my_key[c];
+        stage_v[pos] = my_val[c];
+    }
+    __syncthreads();
+
+    for (int i = tid; i < tile_count; i += kSortWG) {
+        uint32_t k = stage_k[i];
+        uint32_t d = (k >> shift) & mask;
+        int64_t out = (int64_t)global_base[d] + (i - (int64_t)digit_base[d]);
+        keys_out[out] = k;
+        vals_out[out] = stage_v[i];
+    }
+}
+
+// ------------------------------------------------------------------ scan
+constexpr int kScanWG = 256;
+constexpr int kScanIPT = 4;
+constexpr int kScanTile = kScanWG * kScanIPT;
+
+__device__ __forceinline__ uint32_t block_exclusive_scan(uint32_t v, uint32_t* wave_sums, uint32_t& block_total) {
+    const int lane = threadIdx.x & 63, wave = threadIdx.x >> 6;
+    uint32_t incl = v;
+#pragma unroll
+    for (int off = 1; off < 64; off <<= 1) {
+        uint32_t up = __shfl_up(incl, off, 64);
+        if (lane >= off) incl += up;
+    }
+    if (lane == 63) wave_sums[wave] = incl;
+    __syncthreads();
+    uint32_t pre = 0, tot = 0;
+    for (int w = 0; w < kScanWG / 64; ++w) {
+        uint32_t s = wave_sums[w];
+        if (w < wave) pre += s;
+        tot += s;
+    }
+    __syncthreads();
+    block_total = tot;
+    return pre + incl - v;
+}
+
+__global__ void __launch_bounds__(kScanWG) k_scan_reduce(const uint32_t* __restrict__ in, int64_t n,
+                                                         uint32_t* __restrict__ partial) {
+    __shared__ uint32_t ws[kScanWG / 64];
+    int64_t base = (int64_t)blockIdx.x * kScanTile;
+    uint32_t s = 0;
+#pragma unroll
+    for (int i = 0; i < kScanIPT; ++i) {
+        int64_t idx = base + (int64_t)threadIdx.x * kScanIPT + i;
+        s += idx < n ? in[idx] : 0u;
+    }
+    uint32_t tot;
+    block_exclusive_scan(s, ws, tot);
+    if (threadIdx.x == 0) partial[blockIdx.x] = tot;
+}
+
+// Exclusive scan of each tile, offset by the (already scanned) tile prefix.
+// in and out may alias.  If total != null, thread 0 of the last tile writes
+// the grand total.
+__global__ void __launch_bounds__(kScanWG) k_scan_down(const uint32_t* in, int64_t n, const uint32_t* partial_scan,
+                                                       uint32_t* out, uint32_t* total) {
+    __shared__ uint32_t ws[kScanWG / 64];
+    int64_t base = (int64_t)blockIdx.x * kScanTile;
+    uint32_t v[kScanIPT];
+    uint32_t s = 0;
+#pragma unroll
+    for (int i = 0; i < kScanIPT; ++i) {
+        int64_t idx = base + (int64_t)threadIdx.x * kScanIPT + i;
+        v[i] = idx < n ? in[idx] : 0u;
+        s += v[i];
+    }
+    uint32_t tot;
+    uint32_t pre = block_exclusive_scan(s, ws, tot);
+    uint32_t off = (partial_scan ? partial_scan[blockIdx.x] : 0u) + pre;
+#pragma unroll
+    for (int i = 0; i < kScanIPT; ++i) {
+        int64_t idx = base + (int64_t)threadIdx.x * kScanIPT + i;
+        if (idx < n) out[idx] = off;
+        off += v[i];
+    }
+    if (total && blockIdx.x == gridDim.x - 1 && threadIdx.x == kScanWG - 1) *total = off;
+}
+
+int64_t scan_temp_words(int64_t n) {
+    int64_t words = 0;
+    while (n > kScanTile) {
+        int64_t tiles = (n + kScanTile - 1) / kScanTile;
+        words += tiles;
+        n = tiles;
+    }
+    return words + 64;
+}
+
+// Exclusive scan in place over n words; temp holds scan_temp_words(n).
+// total (device, may be null) receives the sum.
+void exclusive_scan(uint32_t* data, int64_t n, uint32_t* temp, uint32_t* total, hipStream_t st) {
+    if (n <= 0) return;
+    if (n <= kScanTile) {
+        hipLaunchKernelGGL(k_scan_down, dim3(1), dim3(kScanWG), 0, st, data, n, (const uint32_t*)nullptr, data, total);
+        return;
+    }
+    int64_t tiles = (n + kScanTile - 1) / kScanTile;
+    uint32_t* partial = temp;
+    hipLaunchKernelGGL(k_scan_reduce, dim3((unsigned)tiles), dim3(kScanWG), 0, st, data, n, partial);
+    exclusive_scan(partial, tiles, temp + tiles, nullptr, st);
+    hipLaunchKernelGGL(k_scan_down, dim3((unsigned)tiles), dim3(kScanWG), 0, st, data, n, (const uint32_t*)partial,
+                       data, total);
+}
+
+int64_t radix_tiles(int64_t n) { return (n + kSortTile - 1) / kSortTile; }
+int64_t radix_hist_words(int64_t n) { return radix_tiles(n) * kRadix; }
+
+int radix_passes(int key_bits) { return (key_bits + kRadixBits - 1) / kRadixBits; }
+
+// Sorts (k0,v0) using (k1,v1) as ping-pong; returns 0 if the result is in
+// (k0,v0), 1 if in (k1,v1).
+int radix_sort_pairs(uint32_t* k0, uint32_t* v0, uint32_t* k1, uint32_t* v1, int64_t n, int key_bits, uint32_t* hist,
+                     uint32_t* scan_temp, hipStream_t st, const ProfileHook& prof) {
+    if (n <= 0) return 0;
+    int64_t tiles = radix_tiles(n);
+    int passes = radix_passes(key_bits);
+    uint32_t *ki = k0, *vi = v0, *ko = k1, *vo = v1;
+    for (int p = 0; p < passes; ++p) {
+        int shift = p * kRadixBits;
+        int bits = key_bits - shift < kRadixBits ? key_bits - shift : kRadixBits;
+        uint32_t mask = (1u << bits) - 1u;
+        prof.begin("sort_hist");
+        hipLaunchKernelGGL(k_radix_hist, dim3((unsigned)tiles), dim3(kSortWG), 0, st, ki, n, shift, mask, hist, tiles);
+        prof.end("sort_hist");
+        prof.begin("sort_scan");
+        exclusive_scan(hist, tiles * kRadix, scan_temp, nullptr, st);
+        prof.end("sort_scan");
+        prof.begin("sort_scatter");
+        hipLaunchKernelGGL(k_radix_scatter, dim3((unsigned)tiles), dim3(kSortWG), 0, st, ki, vi, ko, vo, n, shift, mask,
+                           hist, tiles);
+        prof.end("sort_scatter");
+        uint32_t* t;
+        t = ki; ki = ko; ko = t;
+        t = vi; vi = vo; vo = t;
+    }
+    return (passes & 1) ? 1 : 0;
+}
+
+}  // namespace mpm

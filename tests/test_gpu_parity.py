@@ -1,0 +1,196 @@
+"""GPU parity: the HIP step against the CPU oracle on identical seeded inputs.
+
+Tolerances (FP32 throughout):
+  * one step: particle x, v, Fe, Fp and grid mass / momentum / force within
+    1e-5 normwise relative error (max|gpu - oracle| / max|oracle|);
+  * 100 steps: particle mass exact, max position difference <= 1e-3 * dx;
+    the FP32-oracle vs FP64-oracle error is printed beside it.
+"""
+import numpy as np
+import pytest
+
+from mpm_amd import native as N
+from mpm_amd import scenes
+from mpm_amd.solver import ParticleInput, Solver, SolverDesc
+from oracle import pyoracle
+from tests.helpers import grid_errors, particle_errors, rel_err, report
+
+pytestmark = pytest.mark.gpu
+
+TOL1 = 1e-5
+
+
+def _one_step(sc, tag, check_grid=True, pcollide=False):
+    print(f"[{tag}] N={sc.particles.count} res={sc.desc.resolution} interp={sc.desc.interp} "
+          f"transfer={sc.desc.transfer}", flush=True)
+    gpu = Solver(sc.desc, sc.particles)
+    ref = pyoracle.OracleSolver(sc.desc, sc.particles, 32)
+    if pcollide:
+        gpu.set_particles_collision(True)
+        ref.set_particle_collision(True)
+    # initialize(): volume0
+    v0 = rel_err(gpu.read_particles().volume0, ref.read_particles().volume0)
+    gpu.step(1)
+    ref.step(1)
+    errs = particle_errors(gpu.read_particles(), ref.read_particles())
+    errs["volume0"] = v0
+    if check_grid:
+        errs.update({"grid_" + k: v for k, v in grid_errors(gpu.read_grid(), ref.read_grid(),
+                                                            ("mass", "velocity", "force")).items()})
+    report(tag, errs)
+    gpu.close()
+    return errs
+
+
+def test_snow_cube_config0_one_step():
+    sc = scenes.snow_cube(seed=0)  # 16,384 particles, 64^3, PIC-FLIP 0.95, quadratic, dt 1e-4
+    errs = _one_step(sc, "cfg0")
+    assert all(v <= TOL1 for v in errs.values()), errs
+
+
+def test_floor_impact_pic_flip():
+    # cube resting just above the floor, moving down: exercises grid collision
+    sc = scenes.snow_cube(seed=11, n=6000, res=32, lo=0.02, hi=0.2, velocity=(0.3, -2.0, 0.1), dt=2e-4)
+    gpu = Solver(sc.desc, sc.particles)
+    ref = pyoracle.OracleSolver(sc.desc, sc.particles, 32)
+    for k in range(3):
+        gpu.step(1)
+        ref.step(1)
+        errs = particle_errors(gpu.read_particles(), ref.read_particles())
+        report(f"floor step {k + 1}", errs)
+        assert all(v <= TOL1 for v in errs.values()), errs
+
+
+def test_two_balls_apic_cubic():
+    sc = scenes.two_balls(seed=1, res=32, ppc=4)
+    errs = _one_step(sc, "two_balls_apic_cubic")
+    assert all(v <= TOL1 for v in errs.values()), errs
+
+
+def test_slab_apic_quadratic_random_velocity():
+    sc = scenes.apic_slab(seed=2, nx=24, ny=12, nz=24, res=32)
+    errs = _one_step(sc, "slab_apic_quad")
+    assert all(v <= TOL1 for v in errs.values()), errs
+
+
+def test_linear_pic_flip():
+    sc = scenes.snow_cube(seed=5, n=3000, res=32, interp=N.LINEAR, velocity=(0.5, 0.2, -0.3))
+    errs = _one_step(sc, "linear_pic")
+    assert all(v <= TOL1 for v in errs.values()), errs
+
+
+def test_cubic_pic_flip_odd_resolution():
+    sc = scenes.snow_cube(seed=6, n=3000, res=32, interp=N.CUBIC_BSPLINE, velocity=(0.5, 0.2, -0.3))
+    sc.desc.resolution = (30, 37, 41)  # not a multiple of the 4x4x4 block
+    errs = _one_step(sc, "cubic_pic_odd_res")
+    assert all(v <= TOL1 for v in errs.values()), errs
+
+
+def test_neo_hookean_and_particle_collision():
+    sc = scenes.snow_cube(seed=7, n=3000, res=32, lo=0.0, hi=0.15, velocity=(-1.0, -1.0, 0.0))
+    p = sc.particles
+    p.model[: p.count // 2] = N.NEO_HOOKEAN
+    errs = _one_step(sc, "neo_hookean_pcollide", pcollide=True)
+    assert all(v <= TOL1 for v in errs.values()), errs
+
+
+def test_deformed_state_apic():
+    """Start from a non-trivial Fe / Fp / B so that stress and plasticity
+    are active in the compared step."""
+    sc = scenes.apic_slab(seed=8, nx=16, ny=8, nz=16, res=32)
+    rng = scenes.Rng(99)
+    gpu = Solver(sc.desc, sc.particles)
+    st = gpu.read_particles()
+    n = st.count
+    st.Fe = (np.eye(3)[None] + rng.uniform(9 * n, -0.02, 0.02).reshape(n, 3, 3)).astype(np.float32)
+    st.Fp = (np.eye(3)[None] + rng.uniform(9 * n, -0.01, 0.01).reshape(n, 3, 3)).astype(np.float32)
+    st.B = rng.uniform(9 * n, -1e-4, 1e-4).reshape(n, 3, 3).astype(np.float32)
+    gpu.write_particles(st)
+    ref = pyoracle.OracleSolver(sc.desc, st, 32, run_initialize=False)
+    gpu.step(1)
+    ref.step(1)
+    errs = particle_errors(gpu.read_particles(), ref.read_particles(), ("position", "velocity", "Fe", "Fp", "B"))
+    errs.update({"grid_" + k: v for k, v in grid_errors(gpu.read_grid(), ref.read_grid(),
+                                                        ("mass", "velocity", "force")).items()})
+    report("deformed_apic", errs)
+    assert all(v <= TOL1 for v in errs.values()), errs
+
+
+def test_config0_hundred_steps():
+    sc = scenes.snow_cube(seed=0)
+    gpu = Solver(sc.desc, sc.particles)
+    ref = pyoracle.OracleSolver(sc.desc, sc.particles, 32)
+    ref64 = pyoracle.OracleSolver(sc.desc, sc.particles, 64)
+    gpu.step(100)
+    ref.step(100)
+    ref64.step(100)
+    g = gpu.read_particles()
+    r = ref.read_particles()
+    r64 = ref64.read_particles64()
+    dx = sc.desc.stride
+    drift = float(np.abs(g.position.astype(np.float64) - r.position).max())
+    drift64 = float(np.abs(g.position.astype(np.float64) - r64["position"]).max())
+    o32_64 = float(np.abs(r.position.astype(np.float64) - r64["position"]).max())
+    print(f"[cfg0 x100] drift vs FP32 oracle {drift:.3e} ({drift / dx:.2e} dx); vs FP64 {drift64:.3e}; "
+          f"FP32 oracle vs FP64 {o32_64:.3e}; vel rel {rel_err(g.velocity, r.velocity):.3e}", flush=True)
+    assert np.array_equal(g.mass, r.mass)
+    assert float(g.mass.astype(np.float64).sum()) == float(sc.particles.mass.astype(np.float64).sum())
+    assert drift <= 1e-3 * dx
+
+
+def test_substeps_match_oracle_and_fused():
+    """The reference's public sub-step sequence (simulateOneStep body) on the
+    GPU equals the oracle's and the fused mpm_step."""
+    sc = scenes.snow_cube(seed=3, n=4000, res=32, velocity=(0.2, -0.5, 0.1))
+    a = Solver(sc.desc, sc.particles)
+    b = Solver(sc.desc, sc.particles)
+    ref = pyoracle.OracleSolver(sc.desc, sc.particles, 32)
+    dt = sc.desc.dt
+    a.resetGrid()
+    a.particlesToGrid()
+    ref.substep("reset")
+    ref.substep("p2g")
+    errs = grid_errors(a.read_grid(), ref.read_grid(), ("mass", "velocity", "force"))
+    report("after P2G", errs)
+    assert all(v <= TOL1 for v in errs.values()), errs
+    a.computeExternalForces()
+    a.updateGridVelocities(dt)
+    a.gridCollision(dt)
+    a.solveLinearSystem(dt)
+    ref.substep("gravity")
+    ref.substep("update", dt)
+    ref.substep("collision", dt)
+    errs = grid_errors(a.read_grid(), ref.read_grid())
+    report("after grid update", errs)
+    assert all(v <= TOL1 for v in errs.values()), errs
+    a.gridToParticles(dt)
+    a.advectParticles(dt)
+    ref.substep("g2p", dt, sc.desc.blend)
+    ref.substep("advect", dt)
+    b.step(1)
+    pa, pb, pr = a.read_particles(), b.read_particles(), ref.read_particles()
+    e1 = particle_errors(pa, pr)
+    e2 = particle_errors(pa, pb)
+    report("substeps vs oracle", e1)
+    report("substeps vs fused", e2)
+    assert all(v <= TOL1 for v in e1.values()), e1
+    assert all(v <= TOL1 for v in e2.values()), e2
+
+
+def test_graph_replay_matches_eager():
+    sc = scenes.snow_cube(seed=4, n=5000, res=32, transfer=N.APIC, velocity=(0.3, -0.4, 0.2))
+    a = Solver(sc.desc, sc.particles)
+    b = Solver(sc.desc, sc.particles)
+    h = sc.desc.stride
+    for k in range(4):
+        a.step(1)
+        b.step_graph(1)
+        b.synchronize()
+        pa, pb = a.read_particles(), b.read_particles()
+        e = particle_errors(pa, pb, ("position", "velocity", "Fe", "Fp"))
+        # B ~ v h (it is ~0 for a near-uniform velocity field): scale by max|v| h
+        e["B"] = float(np.abs(pa.B.astype(np.float64) - pb.B).max() / (np.abs(pb.velocity).max() * h))
+        report(f"graph replay {k + 1}", e)
+        assert all(v <= 1e-6 for v in e.values()), e
+    sa, sb = a.stats(), b.stats()
+    assert sa["steps"] == sb["steps"] == 4

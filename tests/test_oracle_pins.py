@@ -1,0 +1,328 @@
+"""Pins of the CPU oracle: each oracle function is checked against something
+the mathematics fixes independently of the oracle's own formulas (closed
+forms, identities, LAPACK, finite differences, hand-computed cases)."""
+import numpy as np
+import pytest
+
+from mpm_amd import native as N
+from oracle import pyoracle as O
+from tests.svd_cases import rot, special_matrices
+
+FLT_EPS = np.float32(np.finfo(np.float32).eps)
+
+
+# ------------------------------------------------------------ B-spline weights
+@pytest.mark.parametrize("prec", [32, 64])
+def test_bspline_closed_forms(prec):
+    x = np.array([0.0, 0.5, -0.5, 1.0, -1.0, 1.5, -1.5, 2.0, 0.25])
+    q, qd = O.weights(N.QUADRATIC_BSPLINE, x, prec)
+    np.testing.assert_allclose(q[:8], [0.75, 0.5, 0.5, 0.125, 0.125, 0.0, 0.0, 0.0], atol=1e-7)
+    np.testing.assert_allclose(qd[:5], [0.0, -1.0, 1.0, -0.5, 0.5], atol=1e-7)
+    c, cd = O.weights(N.CUBIC_BSPLINE, x, prec)
+    np.testing.assert_allclose(c[[0, 3, 4, 7]], [2 / 3, 1 / 6, 1 / 6, 0.0], atol=1e-7)
+    np.testing.assert_allclose(cd[[0, 3, 4, 7]], [0.0, -0.5, 0.5, 0.0], atol=1e-7)
+    l, ld = O.weights(N.LINEAR, x, prec)
+    np.testing.assert_allclose(l[[0, 1, 2, 3, 8]], [1.0, 0.5, 0.5, 0.0, 0.75], atol=1e-7)
+    np.testing.assert_allclose(ld[[0, 1, 2, 3, 8]], [0.0, -1.0, 1.0, 0.0, -1.0], atol=1e-7)
+
+
+@pytest.mark.parametrize("interp,rng,k", [(N.LINEAR, 1, None), (N.QUADRATIC_BSPLINE, 2, 0.25),
+                                          (N.CUBIC_BSPLINE, 2, 1 / 3)])
+def test_bspline_moments(interp, rng, k):
+    """Partition of unity, zero first moment, sum of derivatives zero and the
+    APIC second moment k (1/4 quadratic, 1/3 cubic) that fixes D = k h^2 I."""
+    u = np.random.default_rng(0).uniform(0, 1, 500)
+    off = np.arange(-rng - 1, rng + 2)
+    d = (u[:, None] - off[None, :]).ravel()
+    w, wd = O.weights(interp, d, 64)
+    w = w.reshape(u.size, -1)
+    wd = wd.reshape(u.size, -1)
+    dist = off[None, :] - u[:, None]
+    np.testing.assert_allclose(w.sum(1), 1.0, atol=1e-12)
+    np.testing.assert_allclose((w * dist).sum(1), 0.0, atol=1e-12)
+    np.testing.assert_allclose(wd.sum(1), 0.0, atol=1e-12)
+    if k is not None:
+        np.testing.assert_allclose((w * dist * dist).sum(1), k, atol=1e-12)
+    w32, _ = O.weights(interp, d, 32)
+    np.testing.assert_allclose(w32.reshape(u.size, -1).sum(1), 1.0, atol=2e-6)
+
+
+@pytest.mark.parametrize("interp", [N.LINEAR, N.QUADRATIC_BSPLINE, N.CUBIC_BSPLINE])
+def test_bspline_derivative_is_finite_difference(interp):
+    x = np.random.default_rng(1).uniform(-2.2, 2.2, 400)
+    brk = np.array([-2, -1.5, -1, -0.5, 0, 0.5, 1, 1.5, 2])
+    x = x[np.min(np.abs(x[:, None] - brk[None, :]), 1) > 1e-3]
+    eps = 1e-6
+    _, d = O.weights(interp, x, 64)
+    wp, _ = O.weights(interp, x + eps, 64)
+    wm, _ = O.weights(interp, x - eps, 64)
+    np.testing.assert_allclose(d, (wp - wm) / (2 * eps), atol=1e-6)
+
+
+def test_quadratic_fp32_matches_single_rounding():
+    """For |x| < 0.5 the reference evaluates 0.75 - fl(x*x) in double and
+    rounds once; the FP32 restatement must give exactly that float."""
+    x = np.random.default_rng(2).uniform(-0.49, 0.49, 1000).astype(np.float32)
+    w32, _ = O.weights(N.QUADRATIC_BSPLINE, x.astype(np.float64), 32)
+    expect = (0.75 - (x * x).astype(np.float64)).astype(np.float32)
+    assert np.array_equal(w32.astype(np.float32), expect)
+
+
+# ---------------------------------------------------------------------- SVD
+def _polar(A):
+    U, s, Vt = np.linalg.svd(A)
+    D = np.diag([1.0, 1.0, np.sign(np.linalg.det(U @ Vt)) or 1.0])
+    return U @ D @ Vt
+
+
+def test_svd_against_lapack_and_invariants():
+    rng = np.random.default_rng(3)
+    A = np.concatenate([special_matrices(), rng.uniform(-1, 1, (500, 3, 3)),
+                        np.eye(3) + rng.uniform(-0.03, 0.03, (500, 3, 3))])
+    U, s, V = O.svd3(A, 64)
+    for i in range(A.shape[0]):
+        a = A[i]
+        sn = np.linalg.svd(a, compute_uv=False)
+        scale = max(np.abs(a).max(), 1e-300)
+        np.testing.assert_allclose(np.abs(s[i]), sn, atol=1e-12 * scale + 1e-300)
+        np.testing.assert_allclose(U[i] @ np.diag(s[i]) @ V[i].T, a, atol=1e-12 * scale + 1e-300)
+        np.testing.assert_allclose(U[i] @ U[i].T, np.eye(3), atol=1e-12)
+        np.testing.assert_allclose(V[i] @ V[i].T, np.eye(3), atol=1e-12)
+        assert np.linalg.det(U[i]) > 0 and np.linalg.det(V[i]) > 0
+        assert s[i, 0] >= s[i, 1] >= abs(s[i, 2]) - 1e-12 * scale
+        d = np.linalg.det(a)
+        if abs(d) > 1e-9 * scale ** 3:
+            assert np.sign(s[i, 2]) == np.sign(d)
+            if d > 0 and sn[0] - sn[1] > 1e-6 and sn[1] - sn[2] > 1e-6:
+                np.testing.assert_allclose(U[i] @ V[i].T, _polar(a), atol=1e-9)
+
+
+def test_svd_fp32_rounds_fp64():
+    A = np.eye(3)[None] + np.random.default_rng(4).uniform(-0.02, 0.02, (200, 3, 3))
+    U, s, V = O.svd3(A, 32)
+    U6, s6, V6 = O.svd3(A.astype(np.float32).astype(np.float64), 64)
+    np.testing.assert_allclose(s, s6, atol=1e-7)
+
+
+# ------------------------------------------------------------------- stress
+def _fc_energy(F, Fp, mu0, la0, xi):
+    Jp = np.linalg.det(Fp)
+    mu = mu0 * np.exp(xi * (1 - Jp))
+    la = la0 * np.exp(xi * (1 - Jp))
+    s = np.linalg.svd(F, compute_uv=False) * np.sign([1, 1, np.linalg.det(F)])
+    J = np.linalg.det(F)
+    return mu * ((s - 1) ** 2).sum() + 0.5 * la * (J - 1) ** 2
+
+
+def _nh_energy(F, Fp, mu0, la0, xi):
+    Jp = np.linalg.det(Fp)
+    mu = mu0 * np.exp(xi * (1 - Jp))
+    la = la0 * np.exp(xi * (1 - Jp))
+    J = np.linalg.det(F)
+    return mu * (0.5 * (np.trace(F.T @ F) - 3) - np.log(J)) + 0.5 * la * np.log(J) ** 2
+
+
+def _fd_PFt(energy, F, *args):
+    P = np.zeros((3, 3))
+    e = 1e-6
+    for i in range(3):
+        for j in range(3):
+            Fp_ = F.copy(); Fp_[i, j] += e
+            Fm_ = F.copy(); Fm_[i, j] -= e
+            P[i, j] = (energy(Fp_, *args) - energy(Fm_, *args)) / (2 * e)
+    return P @ F.T
+
+
+@pytest.mark.parametrize("model,energy", [(N.FIXED_COROTATED, _fc_energy), (N.NEO_HOOKEAN, _nh_energy)])
+def test_stress_is_energy_derivative(model, energy):
+    """P F^T of the oracle equals the finite-difference derivative of the
+    reference's own energy definitions (src/material_point3d.cu:73-107)."""
+    rng = np.random.default_rng(5)
+    mu0, la0, xi = 58333.33, 38888.89, 10.0
+    for _ in range(20):
+        F = rot(rng.normal(size=3), rng.uniform(0, 3)) @ np.diag(rng.uniform(0.9, 1.1, 3)) @ \
+            rot(rng.normal(size=3), rng.uniform(0, 3)).T
+        Fp = np.eye(3) + rng.uniform(-0.01, 0.01, (3, 3))
+        got = O.stress(F, Fp, mu0, la0, xi, model, 64)
+        want = _fd_PFt(energy, F, Fp, mu0, la0, xi)
+        np.testing.assert_allclose(got, want, rtol=1e-5, atol=1e-4 * mu0 * 1e-3)
+
+
+def test_stress_closed_forms():
+    mu0, la0, xi = 58333.33, 38888.89, 10.0
+    R = rot([1, 2, 3], 0.9)
+    assert np.abs(O.stress(R, np.eye(3), mu0, la0, xi, N.FIXED_COROTATED, 64)).max() < 1e-8
+    s = 1.01
+    got = O.stress(s * np.eye(3), np.eye(3), mu0, la0, xi, N.FIXED_COROTATED, 64)
+    np.testing.assert_allclose(got, (2 * mu0 * (s - 1) * s + la0 * (s ** 3 - 1) * s ** 3) * np.eye(3), rtol=1e-12)
+    a = 0.99  # hardening: Jp = a^3 scales mu, lambda by exp(xi (1 - a^3))
+    got_h = O.stress(s * np.eye(3), a * np.eye(3), mu0, la0, xi, N.FIXED_COROTATED, 64)
+    np.testing.assert_allclose(got_h, got * np.exp(xi * (1 - a ** 3)), rtol=1e-12)
+
+
+def test_lame_from_young_poisson():
+    from mpm_amd.solver import ParticleInput, SolverDesc
+    p = ParticleInput(np.array([[0.5, 0.5, 0.5]]), np.zeros((1, 3)), 1.0, 10.0, 1.4e5, 0.2, 0.025, 0.0075, 0)
+    o = O.OracleSolver(SolverDesc(resolution=(16, 16, 16), stride=1 / 16), p, 64)
+    st = o.read_particles()
+    np.testing.assert_allclose(st.mu0[0], 1.4e5 / (2 * 1.2), rtol=1e-7)
+    np.testing.assert_allclose(st.lambda0[0], 1.4e5 * 0.2 / (1.2 * 0.6), rtol=1e-7)
+
+
+# ---------------------------------------------------------------- collision
+G0 = np.zeros(3, np.float32)
+G1 = np.ones(3, np.float32)
+
+
+@pytest.mark.parametrize("pos,vel,mu,expect", [
+    ((0.5, 0.5, 0.5), (1.0, -2.0, 0.0), 0.9, (1.0, -2.0, 0.0)),     # interior: untouched
+    ((0.5, 0.0, 0.5), (1.0, -2.0, 0.0), 0.9, (0.0, 0.0, 0.0)),      # floor, sticks
+    ((0.5, 0.0, 0.5), (3.0, -1.0, 0.0), 0.5, (2.5, 0.0, 0.0)),      # floor, slides with friction
+    ((0.5, 0.0, 0.5), (0.3, 1.0, 0.0), 0.5, (0.3, 1.0, 0.0)),       # separating
+    ((0.0, 0.0, 0.5), (-1.0, -1.0, 0.5), 0.9, (0.0, 0.0, 0.0)),     # corner, unnormalised normal (1,1,0)
+    ((0.0, 0.0, 0.5), (-1.0, -1.0, 0.5), 0.1, tuple(np.array([1.0, 1.0, 0.5]) * (1 - 0.2 / 1.5))),
+    ((0.99, 0.5, 0.5), (5.0, 0.0, 0.0), 0.9, (5.0, 0.0, 0.0)),      # just inside the upper wall
+    ((1.0, 0.5, 0.5), (5.0, 1.0, 0.0), 0.0, (0.0, 1.0, 0.0)),       # at the upper wall, frictionless
+])
+def test_boundary_collision_cases(pos, vel, mu, expect):
+    got = O.collide(np.array(pos), np.array(vel), G0, G1, mu)
+    np.testing.assert_allclose(got, expect, atol=1e-6)
+
+
+# ---------------------------------------------------------- initialize / step
+def _lattice(res, ppc_axis, lo_cell, hi_cell):
+    h = 1.0 / res
+    c = np.arange(lo_cell * ppc_axis, hi_cell * ppc_axis) + 0.5
+    g = np.stack(np.meshgrid(c, c, c, indexing="ij"), -1).reshape(-1, 3) / ppc_axis * h
+    return g
+
+
+def test_initialize_lattice_volume():
+    """Regular lattice with n^3 particles per cell: interior V0 = h^3 / n^3."""
+    from mpm_amd.solver import ParticleInput, SolverDesc
+    res, n = 16, 2
+    pos = _lattice(res, n, 1, 15)  # interior margin >= 4 cells (cubic stencil reach)
+    p = ParticleInput(pos, np.zeros_like(pos), 1e-3, 10.0, 1.4e5, 0.2, 0.025, 0.0075, 0)
+    for interp in (N.QUADRATIC_BSPLINE, N.CUBIC_BSPLINE):
+        d = SolverDesc(resolution=(res,) * 3, stride=1 / res, interp=interp)
+        st = O.OracleSolver(d, p, 64).read_particles()
+        inner = np.all((pos > 6.5 / res) & (pos < 9.5 / res), axis=1)
+        np.testing.assert_allclose(st.volume0[inner], (1 / res) ** 3 / n ** 3, rtol=1e-6)
+
+
+@pytest.mark.parametrize("transfer", [N.PIC_FLIP, N.APIC])
+def test_single_particle_free_fall(transfer):
+    from mpm_amd.solver import ParticleInput, SolverDesc
+    d = SolverDesc(resolution=(32, 32, 32), stride=1 / 32, transfer=transfer, dt=1e-3, gravity=9.8,
+                   interp=N.QUADRATIC_BSPLINE)
+    p = ParticleInput(np.array([[0.5, 0.6, 0.4]]), np.zeros((1, 3)), 1.0, 10.0, 1.4e5, 0.2, 0.025, 0.0075, 0)
+    for prec, tol in ((64, 1e-12), (32, 2e-6)):
+        o = O.OracleSolver(d, p, prec)
+        n = 20
+        o.step(n)
+        st = o.read_particles()
+        np.testing.assert_allclose(st.velocity[0], [0, -9.8 * n * 1e-3, 0], atol=tol * 10)
+        np.testing.assert_allclose(st.position[0, 1], 0.6 - 9.8 * 1e-6 * n * (n + 1) / 2, atol=tol)
+        np.testing.assert_allclose(st.Fe[0], np.eye(3), atol=tol)
+
+
+def test_p2g_conserves_mass_and_momentum():
+    from mpm_amd import scenes
+    for transfer in (N.PIC_FLIP, N.APIC):
+        sc = scenes.snow_cube(seed=9, n=2000, res=32, transfer=transfer, velocity=(0.3, -0.2, 0.5))
+        o = O.OracleSolver(sc.desc, sc.particles, 64)
+        o.substep("reset")
+        o.substep("p2g")
+        g = o.read_grid()
+        st = o.read_particles()
+        m = st.mass.astype(np.float64)
+        np.testing.assert_allclose(g["mass"].astype(np.float64).sum(), m.sum(), rtol=1e-6)
+        np.testing.assert_allclose(g["velocity"].astype(np.float64).sum(0), (m[:, None] * st.velocity).sum(0),
+                                   rtol=1e-6, atol=1e-9)
+
+
+def test_apic_affine_field_is_exact():
+    """APIC reproduces an affine velocity field: with v_p = a + A x_p and
+    B_p = A D (so C = A), one transfer gives back v_p = a + A x_p and
+    B_p = A D (no stress, no gravity, interior particles)."""
+    from mpm_amd.solver import ParticleState, SolverDesc
+    res = 32
+    h = 1 / res
+    pos = _lattice(res, 2, 10, 22)
+    n = pos.shape[0]
+    a = np.array([0.1, -0.2, 0.05])
+    A = np.array([[0.0, 0.3, -0.1], [0.2, 0.0, 0.4], [-0.3, 0.1, 0.0]])
+    for interp, k in ((N.QUADRATIC_BSPLINE, 0.25), (N.CUBIC_BSPLINE, 1 / 3)):
+        D = k * h * h
+        st = ParticleState.empty(n)
+        st.position[:] = pos
+        st.velocity[:] = a + pos @ A.T
+        st.mass[:] = 1e-3
+        st.volume0[:] = h ** 3 / 8
+        st.Fe[:] = np.eye(3)
+        st.Fp[:] = np.eye(3)
+        st.mu0[:] = 0.0
+        st.lambda0[:] = 0.0
+        st.hardening[:] = 10
+        st.critical_compression[:] = 0.5
+        st.critical_stretch[:] = 0.5
+        st.B[:] = A * D
+        d = SolverDesc(resolution=(res,) * 3, stride=h, transfer=N.APIC, interp=interp, dt=1e-6, gravity=0.0)
+        o = O.OracleSolver(d, st, 64, run_initialize=False)
+        o.substep("reset")
+        o.substep("p2g")
+        o.substep("gravity")
+        o.substep("update", 1e-6)
+        o.substep("collision", 1e-6)
+        o.substep("g2p", 1e-6, 0.0)
+        r = o.read_particles64()
+        inner = np.all((pos > 13 * h) & (pos < 19 * h), axis=1)
+        np.testing.assert_allclose(r["velocity"][inner], (a + pos @ A.T)[inner], atol=1e-7)
+        np.testing.assert_allclose(r["B"][inner], np.broadcast_to(A * D, (inner.sum(), 3, 3)), atol=1e-10)
+
+
+def test_fp32_oracle_tracks_fp64():
+    from mpm_amd import scenes
+    sc = scenes.snow_cube(seed=0, n=3000, res=32)
+    o32 = O.OracleSolver(sc.desc, sc.particles, 32)
+    o64 = O.OracleSolver(sc.desc, sc.particles, 64)
+    o32.step(10)
+    o64.step(10)
+    a = o32.read_particles()
+    b = o64.read_particles64()
+    assert np.abs(a.position - b["position"]).max() < 1e-6
+    assert np.abs(a.Fe - b["Fe"]).max() < 1e-5
+
+
+# ---------------------------------------------------------------- keys/sort
+def _keys_numpy(desc, pos):
+    """Independent restatement of the cell key in numpy float32."""
+    shift = {N.LINEAR: 0.0, N.QUADRATIC_BSPLINE: 0.5, N.CUBIC_BSPLINE: 1.0}[desc.interp]
+    width = {N.LINEAR: 2, N.QUADRATIC_BSPLINE: 3, N.CUBIC_BSPLINE: 4}[desc.interp]
+    res = np.array(desc.resolution)
+    kbn = ((res + 3) >> 2) + 1
+    with np.errstate(invalid="ignore"):
+        u = pos.astype(np.float32) / np.float32(desc.stride)
+        fb = np.floor(u - np.float32(shift))
+        ok = np.all((fb >= -(width - 1)) & (fb <= res - 1), axis=1)
+    b = np.where(ok[:, None], fb, 0).astype(np.int64)
+    blk = ((b + 4) >> 2)
+    lin = blk[:, 0] + kbn[0] * (blk[:, 1] + kbn[1] * blk[:, 2])
+    cell = ((b + 4) & 3)
+    key = lin * 64 + cell[:, 0] + 4 * cell[:, 1] + 16 * cell[:, 2]
+    esc = int(kbn.prod()) * 64
+    return np.where(ok, key, esc).astype(np.uint32)
+
+
+@pytest.mark.parametrize("interp", [N.LINEAR, N.QUADRATIC_BSPLINE, N.CUBIC_BSPLINE])
+def test_keys_and_stable_sort(interp):
+    from mpm_amd.solver import SolverDesc
+    rng = np.random.default_rng(6)
+    d = SolverDesc(resolution=(20, 33, 17), stride=1 / 32, interp=interp)
+    pos = rng.uniform(-0.2, 1.2, (5000, 3)).astype(np.float32)
+    pos[:50] = (np.round(pos[:50] * 64) / 64).astype(np.float32)  # exact cell-boundary values
+    pos[50, 0] = np.nan
+    k = O.keys(d, pos)
+    assert np.array_equal(k, _keys_numpy(d, pos))
+    order = O.stable_argsort(k)
+    assert np.array_equal(order, np.argsort(k, kind="stable").astype(np.uint32))
