@@ -18,7 +18,6 @@
 
 namespace mpm {
 
-constexpr int kP2GWG = 128;
 constexpr int kG2PWG = 128;
 
 // ------------------------------------------------------------------ keys
@@ -201,8 +200,45 @@ __device__ __forceinline__ void store_m3(float* base, int64_t N, uint32_t p, con
 // particlesToGrid: src/mpm_solver3d.cu:237-294 (PIC-FLIP),
 // src/apic_mpm_solver3d.cu:245-304 (APIC); MODE 1 is initialize()'s
 // P2GMass (src/mpm_solver3d.cu:136-171).
+//
+// One wave per work item.  Lanes hold consecutive sorted particles, so the
+// particles of one cell sit in consecutive lanes and share all W^3 target
+// nodes.  Each contribution is first summed over the lanes of its cell
+// (segmented reduction with DPP wave shifts), then only the first lane of
+// each cell adds it to the LDS tile with a plain read-add-write: within one
+// instruction the writing lanes belong to different cells, so they hit
+// different nodes, and the single wave orders successive writes.  This
+// avoids LDS float atomics entirely.
+
+// Lane i receives lane i+1's value (0 for lane 63): DPP wave_shl:1.
+__device__ __forceinline__ float wave_shl1(float v) {
+    return __int_as_float(__builtin_amdgcn_update_dpp(0, __float_as_int(v), 0x130, 0xF, 0xF, true));
+}
+
+// Segmented sum toward the first lane of each segment (tree over offsets
+// 1, 2, 4, ...): after step j a lane holds the sum of the next 2^(j+1)
+// lanes of its segment.  mk[j] marks the lanes whose segment continues at
+// lane + 2^j; nsteps = number of non-empty levels (uniform).
+template <int NV>
+__device__ __forceinline__ void segment_sum(float (&v)[NV], const uint64_t (&mk)[6], int nsteps, int lane) {
+#pragma unroll
+    for (int j = 0; j < 6; ++j) {
+        if (j >= nsteps) break;
+        const bool take = (mk[j] >> lane) & 1ull;
+#pragma unroll
+        for (int f = 0; f < NV; ++f) {
+            float t;
+            if (j == 0) t = wave_shl1(v[f]);
+            else if (j == 1) t = wave_shl1(wave_shl1(v[f]));
+            else if (j == 2) t = wave_shl1(wave_shl1(wave_shl1(wave_shl1(v[f]))));
+            else t = __shfl_down(v[f], 1 << j, 64);
+            v[f] += take ? t : 0.0f;
+        }
+    }
+}
+
 template <int INTERP, int APIC, int MODE>
-__global__ void __launch_bounds__(kP2GWG) k_p2g(SceneDev S, ParticlesDev P, ItemsDev I) {
+__global__ void __launch_bounds__(64) k_p2g(SceneDev S, ParticlesDev P, ItemsDev I) {
     constexpr int W = Stencil<INTERP>::W;
     constexpr int T = 4 + W - 1;
     constexpr int TN = T * T * T;
@@ -210,13 +246,29 @@ __global__ void __launch_bounds__(kP2GWG) k_p2g(SceneDev S, ParticlesDev P, Item
     __shared__ float tile[NF * TN];
     const uint32_t nitems = *I.count;
     const int64_t N = P.n;
+    const int lane = threadIdx.x;
     for (uint32_t it = blockIdx.x; it < nitems; it += gridDim.x) {
         ItemGeom g = item_geom(S, I, it, nitems);
         if (g.escape) continue;  // no stencil node inside the grid
-        for (int q = threadIdx.x; q < NF * TN; q += kP2GWG) tile[q] = 0.0f;
+        for (int q = lane; q < NF * TN; q += 64) tile[q] = 0.0f;
         __syncthreads();
-        for (uint32_t i = g.s + threadIdx.x; i < g.e; i += kP2GWG) {
-            const uint32_t p = I.perm[i];
+        for (uint32_t base = g.s; base < g.e; base += 64) {
+            const uint32_t i = base + lane;
+            const bool active = i < g.e;
+            const uint32_t ic = active ? i : g.s;
+            // segment structure of this chunk (inactive lanes form their own tail segment)
+            const uint32_t key = active ? I.sorted_keys[ic] : 0xFFFFFFFFu;
+            const uint32_t prev = __shfl_up(key, 1, 64);
+            const bool head = active && (lane == 0 || prev != key);
+            uint64_t mk[6] = {0ull, 0ull, 0ull, 0ull, 0ull, 0ull};
+            int nsteps = 0;
+#pragma unroll
+            for (int j = 0; j < 6; ++j) {
+                const uint32_t other = __shfl_down(key, 1 << j, 64);
+                mk[j] = __ballot(active && lane + (1 << j) < 64 && other == key);
+                if (mk[j] != 0ull) nsteps = j + 1;
+            }
+            const uint32_t p = I.perm[ic];
             const float x0 = P.x[p], x1 = P.x[N + p], x2 = P.x[2 * N + p];
             int bx, by, bz;
             float Nx[W], Ny[W], Nz[W], dNx[W], dNy[W], dNz[W];
@@ -224,7 +276,7 @@ __global__ void __launch_bounds__(kP2GWG) k_p2g(SceneDev S, ParticlesDev P, Item
             axis_weights<INTERP>(x1, S.oy, S.h, S.inv_h, by, Ny, dNy);
             axis_weights<INTERP>(x2, S.oz, S.h, S.inv_h, bz, Nz, dNz);
             const int lx = bx - g.ox, ly = by - g.oy, lz = bz - g.oz;
-            const float m = P.m[p];
+            const float m = active ? P.m[p] : 0.0f;
             if (MODE == 1) {
 #pragma unroll
                 for (int a = 0; a < W; ++a)
@@ -232,8 +284,10 @@ __global__ void __launch_bounds__(kP2GWG) k_p2g(SceneDev S, ParticlesDev P, Item
                     for (int b = 0; b < W; ++b)
 #pragma unroll
                         for (int c = 0; c < W; ++c) {
-                            float w = (Nx[a] * Ny[b]) * Nz[c];
-                            atomicAdd(&tile[((lz + c) * T + (ly + b)) * T + (lx + a)], m * w);
+                            float v[1] = {m * ((Nx[a] * Ny[b]) * Nz[c])};
+                            segment_sum<1>(v, mk, nsteps, lane);
+                            const int q = ((lz + c) * T + (ly + b)) * T + (lx + a);
+                            if (head) tile[q] += v[0];
                         }
                 continue;
             }
@@ -242,8 +296,8 @@ __global__ void __launch_bounds__(kP2GWG) k_p2g(SceneDev S, ParticlesDev P, Item
             load_m3(P.Fp, N, p, Fp);
             const MaterialDev mat = P.mats[P.mat[p]];
             bool fe_changed;
-            particle_stress(Fe, Fp, P.mu0[p], P.lam0[p], P.V0[p], mat, vs, fe_changed);
-            if (fe_changed) store_m3(P.Fe, N, p, Fe);
+            particle_stress(Fe, Fp, P.mu0[p], P.lam0[p], active ? P.V0[p] : 0.0f, mat, vs, fe_changed);
+            if (fe_changed && active) store_m3(P.Fe, N, p, Fe);
             const float v0 = P.v[p], v1 = P.v[N + p], v2 = P.v[2 * N + p];
             float C[3][3];
             float dxn[3][W];
@@ -272,32 +326,31 @@ __global__ void __launch_bounds__(kP2GWG) k_p2g(SceneDev S, ParticlesDev P, Item
                         const float gy = (Nx[a] * dNy[b]) * Nz[c];
                         const float gz = (Nx[a] * Ny[b]) * dNz[c];
                         const float mip = m * w;
-                        const float fx = (vs[0][0] * gx + vs[0][1] * gy) + vs[0][2] * gz;
-                        const float fy = (vs[1][0] * gx + vs[1][1] * gy) + vs[1][2] * gz;
-                        const float fz = (vs[2][0] * gx + vs[2][1] * gy) + vs[2][2] * gz;
-                        float px, py, pz;
+                        float v[7];
+                        v[0] = mip;
                         if (APIC) {
                             const float d0 = dxn[0][a], d1 = dxn[1][b], d2 = dxn[2][c];
-                            px = mip * (v0 + ((C[0][0] * d0 + C[0][1] * d1) + C[0][2] * d2));
-                            py = mip * (v1 + ((C[1][0] * d0 + C[1][1] * d1) + C[1][2] * d2));
-                            pz = mip * (v2 + ((C[2][0] * d0 + C[2][1] * d1) + C[2][2] * d2));
+                            v[1] = mip * (v0 + ((C[0][0] * d0 + C[0][1] * d1) + C[0][2] * d2));
+                            v[2] = mip * (v1 + ((C[1][0] * d0 + C[1][1] * d1) + C[1][2] * d2));
+                            v[3] = mip * (v2 + ((C[2][0] * d0 + C[2][1] * d1) + C[2][2] * d2));
                         } else {
-                            px = v0 * mip;
-                            py = v1 * mip;
-                            pz = v2 * mip;
+                            v[1] = v0 * mip;
+                            v[2] = v1 * mip;
+                            v[3] = v2 * mip;
                         }
+                        v[4] = (vs[0][0] * gx + vs[0][1] * gy) + vs[0][2] * gz;
+                        v[5] = (vs[1][0] * gx + vs[1][1] * gy) + vs[1][2] * gz;
+                        v[6] = (vs[2][0] * gx + vs[2][1] * gy) + vs[2][2] * gz;
+                        segment_sum<7>(v, mk, nsteps, lane);
                         const int q = ((lz + c) * T + (ly + b)) * T + (lx + a);
-                        atomicAdd(&tile[0 * TN + q], mip);
-                        atomicAdd(&tile[1 * TN + q], px);
-                        atomicAdd(&tile[2 * TN + q], py);
-                        atomicAdd(&tile[3 * TN + q], pz);
-                        atomicAdd(&tile[4 * TN + q], fx);
-                        atomicAdd(&tile[5 * TN + q], fy);
-                        atomicAdd(&tile[6 * TN + q], fz);
+                        if (head) {
+#pragma unroll
+                            for (int f = 0; f < 7; ++f) tile[f * TN + q] += v[f];
+                        }
                     }
         }
         __syncthreads();
-        for (int q = threadIdx.x; q < TN; q += kP2GWG) {
+        for (int q = lane; q < TN; q += 64) {
             const int lx = q % T, ly = (q / T) % T, lz = q / (T * T);
             const int gi = g.ox + lx, gj = g.oy + ly, gk = g.oz + lz;
             if (!in_grid(S, gi, gj, gk)) continue;
@@ -320,8 +373,8 @@ __global__ void __launch_bounds__(kP2GWG) k_p2g(SceneDev S, ParticlesDev P, Item
 template <int INTERP, int APIC>
 static void p2g_dispatch_mode(const SceneDev& S, const ParticlesDev& P, const ItemsDev& I, int mode, int wgs,
                               hipStream_t st) {
-    if (mode == 0) hipLaunchKernelGGL((k_p2g<INTERP, APIC, 0>), dim3(wgs), dim3(kP2GWG), 0, st, S, P, I);
-    else hipLaunchKernelGGL((k_p2g<INTERP, APIC, 1>), dim3(wgs), dim3(kP2GWG), 0, st, S, P, I);
+    if (mode == 0) hipLaunchKernelGGL((k_p2g<INTERP, APIC, 0>), dim3(wgs), dim3(64), 0, st, S, P, I);
+    else hipLaunchKernelGGL((k_p2g<INTERP, APIC, 1>), dim3(wgs), dim3(64), 0, st, S, P, I);
 }
 template <int INTERP>
 static void p2g_dispatch_apic(const SceneDev& S, const ParticlesDev& P, const ItemsDev& I, int apic, int mode, int wgs,

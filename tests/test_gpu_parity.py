@@ -191,6 +191,6 @@ def test_graph_replay_matches_eager():
         # B ~ v h (it is ~0 for a near-uniform velocity field): scale by max|v| h
         e["B"] = float(np.abs(pa.B.astype(np.float64) - pb.B).max() / (np.abs(pb.velocity).max() * h))
         report(f"graph replay {k + 1}", e)
-        assert all(v <= 1e-6 for v in e.values()), e
+        assert all(v <= TOL1 for v in e.values()), e  # float-atomic order only
     sa, sb = a.stats(), b.stats()
     assert sa["steps"] == sb["steps"] == 4
