@@ -83,7 +83,21 @@ def main(argv=None):
     ap.add_argument("--graph", action="store_true", help="replay the step as a captured hipGraph")
     ap.add_argument("--no-cpu-baseline", action="store_true")
     ap.add_argument("--seed", type=int, default=None)
+    ap.add_argument("--calibrate-copy", type=int, default=0, metavar="MIB",
+                    help="only run a plain device copy of MIB MiB (HBM counter calibration) and exit")
     a = ap.parse_args(argv)
+
+    if a.calibrate_copy:
+        import torch
+
+        n = a.calibrate_copy * (1 << 20) // 4
+        src = torch.ones(n, dtype=torch.float32, device="cuda")
+        dst = torch.empty_like(src)
+        for _ in range(3):
+            dst.copy_(src)
+        torch.cuda.synchronize()
+        print(json.dumps({"calibration_copy_bytes_each_way": n * 4, "launches": 3}), flush=True)
+        return 0
 
     if a.gpus != 1:
         print(f"bench.py runs on one GPU only (asked for --gpus {a.gpus}); multi-GPU is not part of this build",

@@ -62,7 +62,10 @@ struct Solver {
     MaterialDev* mats_dev = nullptr;
     int64_t nsb = 0;
     int key_bits = 1;
-    uint32_t *k0 = nullptr, *v0 = nullptr, *k1 = nullptr, *v1 = nullptr;
+    uint32_t* kin = nullptr;   // sort input keys (slot order); written by k_keys or the fused G2P
+    uint32_t* iota = nullptr;  // sort input values: slot index
+    uint32_t *ka = nullptr, *va = nullptr, *kb = nullptr, *vb = nullptr;  // ping-pong / result
+    bool keys_valid = false;   // kin matches the current positions
     uint32_t* hist = nullptr;
     uint32_t* scan_temp = nullptr;
     uint32_t* flags = nullptr;
@@ -70,6 +73,7 @@ struct Solver {
     uint32_t* counters = nullptr;                 // [0] item count
     unsigned long long* acc = nullptr;            // [0] active sum, [1] scratch, [2] escaped
     bool sort_valid = false, grid_dirty = false, vstar_dirty = false, count_on_clear = false;
+    bool stress_valid = false;  // P.tau matches the particle state
     int phase = PH_CLEAR;
     int64_t steps = 0;
     int item_wgs = 1024;
@@ -307,6 +311,7 @@ int alloc_particles(Solver& s, ParticlesDev& P, int64_t n) {
     if ((rc = dalloc(s, (void**)&P.V0, f))) return rc;
     if ((rc = dalloc(s, (void**)&P.mu0, f))) return rc;
     if ((rc = dalloc(s, (void**)&P.lam0, f))) return rc;
+    if ((rc = dalloc(s, (void**)&P.tau, 6 * f))) return rc;
     if ((rc = dalloc(s, (void**)&P.mat, (size_t)std::max<int64_t>(n, 1) * 2))) return rc;
     if ((rc = dalloc(s, (void**)&P.pid, (size_t)std::max<int64_t>(n, 1) * 4))) return rc;
     P.mats = s.mats_dev;
@@ -314,7 +319,7 @@ int alloc_particles(Solver& s, ParticlesDev& P, int64_t n) {
 }
 void free_particles(Solver& s, ParticlesDev& P) {
     dfree(s, P.x); dfree(s, P.v); dfree(s, P.B); dfree(s, P.Fe); dfree(s, P.Fp); dfree(s, P.m);
-    dfree(s, P.V0); dfree(s, P.mu0); dfree(s, P.lam0); dfree(s, P.mat); dfree(s, P.pid);
+    dfree(s, P.V0); dfree(s, P.mu0); dfree(s, P.lam0); dfree(s, P.tau); dfree(s, P.mat); dfree(s, P.pid);
 }
 
 // Host layout staging on the device for pack/unpack.
@@ -354,13 +359,14 @@ void free_staging(Solver& s, Staging& g) {
 void run_sort(Solver& s) {
     ProfileHook h = hook(s);
     if (s.N <= 0) { s.sort_valid = true; return; }
-    {
+    if (!s.keys_valid) {
         Scope sc(h, "keys");
-        launch_keys(s.S, s.P, s.interp, s.k0, s.v0, s.st);
+        launch_keys(s.S, s.P, s.interp, s.kin, s.st);
+        s.keys_valid = true;
     }
-    int sel = radix_sort_pairs(s.k0, s.v0, s.k1, s.v1, s.N, s.key_bits, s.hist, s.scan_temp, s.st, h);
-    s.I.sorted_keys = sel ? s.k1 : s.k0;
-    s.I.perm = sel ? s.v1 : s.v0;
+    int sel = radix_sort_pairs(s.kin, s.iota, s.ka, s.va, s.kb, s.vb, s.N, s.key_bits, s.hist, s.scan_temp, s.st, h);
+    s.I.sorted_keys = sel ? s.kb : s.ka;
+    s.I.perm = sel ? s.vb : s.va;
     s.I.start = s.items;
     s.I.count = s.counters;
     s.I.n = s.N;
@@ -381,9 +387,18 @@ void run_clear(Solver& s) {
     s.phase = PH_CLEAR;
 }
 
+void ensure_stress(Solver& s) {
+    if (s.stress_valid) return;
+    ProfileHook h = hook(s);
+    Scope sc(h, "stress");
+    launch_stress(s.P, s.st);
+    s.stress_valid = true;
+}
+
 void run_fused_step(Solver& s) {
     ProfileHook h = hook(s);
     if (s.grid_dirty) run_clear(s);
+    ensure_stress(s);
     run_sort(s);
     {
         Scope sc(h, "p2g");
@@ -391,12 +406,13 @@ void run_fused_step(Solver& s) {
     }
     {
         Scope sc(h, "g2p");
-        launch_g2p(s.S, s.P, s.I, s.interp, s.apic, 0, 1, s.S.particle_collision, s.item_wgs, s.st);
+        launch_g2p(s.S, s.P, s.I, s.interp, s.apic, 0, 1, s.S.particle_collision, s.item_wgs, s.st, s.kin);
     }
     s.grid_dirty = true;
     s.count_on_clear = true;
     s.phase = PH_RAW;
     s.sort_valid = false;
+    s.keys_valid = true;  // the G2P wrote the keys of the advected positions
     s.steps += 1;
 }
 
@@ -408,6 +424,7 @@ void run_initialize(Solver& s) {
     s.grid_dirty = true;
     s.count_on_clear = false;
     s.phase = PH_MASS;
+    s.stress_valid = false;  // volume0 changed
 }
 
 // Materialise the grid as the reference holds it after a step (gravity
@@ -438,10 +455,13 @@ int build(Solver& s, HostState& H, int run_init) {
     HIP_TRY(hipMemsetAsync(s.S.touched, 0, (size_t)s.nsb * 4, s.st));
     // sort buffers
     const int64_t n = std::max<int64_t>(s.N, 1);
-    if ((rc = dalloc(s, (void**)&s.k0, (size_t)n * 4))) return rc;
-    if ((rc = dalloc(s, (void**)&s.v0, (size_t)n * 4))) return rc;
-    if ((rc = dalloc(s, (void**)&s.k1, (size_t)n * 4))) return rc;
-    if ((rc = dalloc(s, (void**)&s.v1, (size_t)n * 4))) return rc;
+    if ((rc = dalloc(s, (void**)&s.kin, (size_t)n * 4))) return rc;
+    if ((rc = dalloc(s, (void**)&s.iota, (size_t)n * 4))) return rc;
+    if ((rc = dalloc(s, (void**)&s.ka, (size_t)n * 4))) return rc;
+    if ((rc = dalloc(s, (void**)&s.va, (size_t)n * 4))) return rc;
+    if ((rc = dalloc(s, (void**)&s.kb, (size_t)n * 4))) return rc;
+    if ((rc = dalloc(s, (void**)&s.vb, (size_t)n * 4))) return rc;
+    launch_iota(s.iota, s.N, s.st);
     if ((rc = dalloc(s, (void**)&s.flags, (size_t)n * 4))) return rc;
     if ((rc = dalloc(s, (void**)&s.items, (size_t)n * 4))) return rc;
     int64_t hw = radix_hist_words(n);
@@ -483,6 +503,7 @@ int build(Solver& s, HostState& H, int run_init) {
         free_particles(s, s.P);
         s.P = Q;
         s.sort_valid = false;
+        s.keys_valid = false;
     }
     HIP_TRY(hipGetLastError());
     if (run_init) run_initialize(s);
@@ -633,6 +654,11 @@ MPM_API int mpm_step_graph(void* handle, int n) {
     // A grid left by initialize / sub-steps is cleared outside the graph
     // (without counting); the captured step always starts with a clear.
     if (s->grid_dirty && !s->count_on_clear) run_clear(*s);
+    ensure_stress(*s);  // outside the graph: replays rely on G2P keeping it current
+    if (!s->keys_valid) {  // likewise the sort input keys
+        launch_keys(s->S, s->P, s->interp, s->kin, s->st);
+        s->keys_valid = true;
+    }
     if (!s->gexec) {
         bool prof = s->prof;
         s->prof = false;
@@ -676,6 +702,7 @@ MPM_API int mpm_reset_grid(void* handle) {
 
 MPM_API int mpm_particles_to_grid(void* handle) {
     GET_SOLVER(s, handle);
+    ensure_stress(*s);
     if (!s->sort_valid) run_sort(*s);
     launch_p2g(s->S, s->P, s->I, s->interp, s->apic, 0, s->item_wgs, s->st);
     s->grid_dirty = true;
@@ -737,6 +764,7 @@ MPM_API int mpm_advect_particles(void* handle, float dt) {
     GET_SOLVER(s, handle);
     launch_advect(s->P, dt, s->st);
     s->sort_valid = false;
+    s->keys_valid = false;
     return finish(*s);
 }
 
@@ -845,6 +873,8 @@ MPM_API int mpm_write_particles(void* handle, const mpm_particle_state* in) {
     free_staging(*s, g);
     if (e != hipSuccess) return fail(MPM_ERR_HIP, std::string("write particles: ") + hipGetErrorString(e));
     s->sort_valid = false;
+    s->keys_valid = false;
+    s->stress_valid = false;
     return finish(*s);
 }
 

@@ -67,6 +67,7 @@ struct ParticlesDev {
     float* V0;     // N
     float* mu0;    // N
     float* lam0;   // N
+    float* tau;    // 6N cached -V0 * P(Fe) Fe^T (xx, xy, xz, yy, yz, zz) for the next P2G
     uint16_t* mat; // N
     uint32_t* pid; // N original particle id
     const MaterialDev* mats;

@@ -25,6 +25,16 @@ MPM_HD float mpm_rsqrt(float x) {
     return y * (1.5f - 0.5f * x * y * y);
 }
 
+// Reciprocal to ~1 ulp (hardware v_rcp_f32) where correct rounding does not
+// matter: Jacobi tangents, clamped singular values.
+MPM_HD float mpm_rcp(float x) {
+#if defined(__HIP_DEVICE_COMPILE__)
+    return __builtin_amdgcn_rcpf(x);
+#else
+    return 1.0f / x;
+#endif
+}
+
 // ------------------------------------------------------------ 3x3 helpers
 // Matrices are float[3][3] row-major, fully unrolled so they stay in VGPRs.
 MPM_HD void m3_mul(const float a[3][3], const float b[3][3], float c[3][3]) {
@@ -60,7 +70,7 @@ MPM_HD float m3_det(const float m[3][3]) {
 // removes the orthogonality drift that rounding leaves after many
 // rotations.  Every decision is a select, so the wave never diverges.
 
-#define MPM_SVD_SWEEPS 5
+#define MPM_SVD_SWEEPS 4
 
 // Jacobi rotation J(p,q) zeroing S_pq (J_pp = J_qq = c, J_pq = -J_qp = s;
 // S <- J^T S J, V <- V J); spp/sqq/spq are S_pp, S_qq, S_pq and srp/srq the
@@ -68,7 +78,7 @@ MPM_HD float m3_det(const float m[3][3]) {
 MPM_HD void jacobi_pair(float& spp, float& sqq, float& spq, float& srp, float& srq, float V[3][3], int p, int q) {
     const float d = sqq - spp;
     const float den = fabsf(d) + sqrtf(d * d + 4.0f * spq * spq);
-    const float t = den > 0.0f ? ((d >= 0.0f ? 2.0f : -2.0f) * spq) / den : 0.0f;  // tan(theta)
+    const float t = den > 0.0f ? ((d >= 0.0f ? 2.0f : -2.0f) * spq) * mpm_rcp(den) : 0.0f;  // tan(theta)
     const float c = mpm_rsqrt(1.0f + t * t);
     const float s = t * c;
     spp = spp - t * spq;
@@ -223,17 +233,18 @@ template <> struct Stencil<MPM_INTERP_CUBIC_BSPLINE> { static constexpr int W = 
 // (src/mpm_solver3d.cu:251-255) selects the same nodes except when
 // x/h - r is an integer, where the extra node has weight 0.
 template <int INTERP>
-MPM_HD void axis_weights(float x, float origin, float h, float inv_h, int& base,
-                                             float N[Stencil<INTERP>::W], float dN[Stencil<INTERP>::W]) {
+MPM_HD void axis_weights(float x, float origin, float h, float inv_h, int& base, float N[Stencil<INTERP>::W],
+                         float dN[Stencil<INTERP>::W]) {
     constexpr int W = Stencil<INTERP>::W;
-    float u = x / h;
+    const float u = x / h;  // IEEE division: the same base cell as the sort key
     float fb = floorf(u - Stencil<INTERP>::SHIFT);
     fb = fminf(fmaxf(fb, -1.0e9f), 1.0e9f);
     base = (int)fb;
+    // d_j = (x - (origin + h (base + j))) / h, evaluated as u - origin/h - (base + j)
+    const float u0 = u - origin * inv_h;
 #pragma unroll
     for (int j = 0; j < W; ++j) {
-        float gp = origin + h * (float)(base + j);
-        float d = (x - gp) / h;
+        const float d = u0 - (fb + (float)j);
         float n, dn;
         bspline<INTERP>(d, n, dn);
         N[j] = n;

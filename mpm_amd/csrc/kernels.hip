@@ -22,25 +22,32 @@ constexpr int kG2PWG = 128;
 
 // ------------------------------------------------------------------ keys
 template <int INTERP>
-__global__ void k_keys(SceneDev S, ParticlesDev P, uint32_t* __restrict__ keys, uint32_t* __restrict__ vals) {
+__global__ void k_keys(SceneDev S, ParticlesDev P, uint32_t* __restrict__ keys) {
     int64_t i = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
     if (i >= P.n) return;
     const int64_t N = P.n;
     keys[i] = cell_key(S, P.x[i], P.x[N + i], P.x[2 * N + i], Stencil<INTERP>::SHIFT, Stencil<INTERP>::W);
-    vals[i] = (uint32_t)i;
 }
 
-void launch_keys(const SceneDev& S, const ParticlesDev& P, int interp, uint32_t* keys, uint32_t* vals,
-                 hipStream_t st) {
+void launch_keys(const SceneDev& S, const ParticlesDev& P, int interp, uint32_t* keys, hipStream_t st) {
     if (P.n <= 0) return;
     dim3 g((unsigned)((P.n + 255) / 256)), b(256);
     switch (interp) {
-        case MPM_INTERP_LINEAR: hipLaunchKernelGGL(k_keys<MPM_INTERP_LINEAR>, g, b, 0, st, S, P, keys, vals); break;
+        case MPM_INTERP_LINEAR: hipLaunchKernelGGL(k_keys<MPM_INTERP_LINEAR>, g, b, 0, st, S, P, keys); break;
         case MPM_INTERP_QUADRATIC_BSPLINE:
-            hipLaunchKernelGGL(k_keys<MPM_INTERP_QUADRATIC_BSPLINE>, g, b, 0, st, S, P, keys, vals);
+            hipLaunchKernelGGL(k_keys<MPM_INTERP_QUADRATIC_BSPLINE>, g, b, 0, st, S, P, keys);
             break;
-        default: hipLaunchKernelGGL(k_keys<MPM_INTERP_CUBIC_BSPLINE>, g, b, 0, st, S, P, keys, vals); break;
+        default: hipLaunchKernelGGL(k_keys<MPM_INTERP_CUBIC_BSPLINE>, g, b, 0, st, S, P, keys); break;
     }
+}
+
+__global__ void k_iota(uint32_t* p, int64_t n) {
+    int64_t i = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+    if (i < n) p[i] = (uint32_t)i;
+}
+void launch_iota(uint32_t* p, int64_t n, hipStream_t st) {
+    if (n <= 0) return;
+    hipLaunchKernelGGL(k_iota, dim3((unsigned)((n + 255) / 256)), dim3(256), 0, st, p, n);
 }
 
 // ----------------------------------------------------------------- items
@@ -94,28 +101,27 @@ __device__ __forceinline__ bool in_grid(const SceneDev& S, int i, int j, int k) 
 
 // ------------------------------------------------------------------ stress
 // -V0 * P(Fe) Fe^T of one particle: plasticLame (src/material_point3d.cu:158-162)
-// with fixed-corotated (:110-128) or Neo-Hookean (:131-155) stress.
-// With Fe = U diag(s) V^T (U, V rotations) the reference's expressions are
+// with fixed-corotated (:110-128) or Neo-Hookean (:131-155) stress, from an
+// SVD Fe = U diag(s) V^T (U, V rotations).  The reference's expressions are
 // evaluated in the principal frame, which avoids the cancellation in
 // Fe - R (|Fe - R| ~ 1e-2 for snow):
 //   fixed corotated: 2 mu (Fe - R) Fe^T + la (J-1) J I
 //                  = U diag(2 mu (s_i - 1) s_i) U^T + la (J-1) J I
 //   Neo-Hookean:     mu (Fe Fe^T - I) + la log(J) I
 //                  = U diag(mu (s_i^2 - 1)) U^T + la log(J) I
-// with J = s0 s1 s2 = det Fe.  For Neo-Hookean the singular values are first
-// raised to FLT_EPSILON and the projected Fe is returned in Fe (the
-// reference writes it back into the particle, :142); fe_changed tells the
-// caller to store it.
-__device__ __forceinline__ void particle_stress(float Fe[3][3], const float Fp[3][3], float mu0, float lam0, float V0,
-                                                const MaterialDev& mat, float vs[3][3], bool& fe_changed) {
+// with J = s0 s1 s2 = det Fe.  The result is symmetric; tau holds xx, xy,
+// xz, yy, yz, zz.  For Neo-Hookean the singular values are first raised to
+// FLT_EPSILON and the projected Fe = U diag(s) V^T is returned in Fe (the
+// reference writes it back into the particle, :142).
+__device__ __forceinline__ void stress_from_svd(const float U[3][3], const float sv[3], const float V[3][3],
+                                                const float Fp[3][3], float mu0, float lam0, float V0,
+                                                const MaterialDev& mat, float tau[6], float Fe[3][3]) {
     const float Jp = m3_det(Fp);
     const float factor = expf(mat.hardening * (1.0f - Jp));
     const float mu = mu0 * factor, la = lam0 * factor;
-    float U[3][3], V[3][3], s[3];
-    svd3(Fe, U, s, V);
+    float s[3] = {sv[0], sv[1], sv[2]};
     float d[3], c;
     if (mat.model == MPM_MODEL_NEO_HOOKEAN) {
-        fe_changed = true;
 #pragma unroll
         for (int i = 0; i < 3; ++i) s[i] = fmaxf(s[i], MPM_FLT_EPS);
         float US[3][3];
@@ -129,29 +135,27 @@ __device__ __forceinline__ void particle_stress(float Fe[3][3], const float Fp[3
         for (int i = 0; i < 3; ++i) d[i] = mu * (s[i] * s[i] - 1.0f);
         c = la * logf(J);
     } else {
-        fe_changed = false;
         const float J = s[0] * s[1] * s[2];
 #pragma unroll
         for (int i = 0; i < 3; ++i) d[i] = (2.0f * mu) * ((s[i] - 1.0f) * s[i]);
         c = la * (J - 1.0f) * J;
     }
-    float P[3][3];
+    const int ii[6] = {0, 0, 0, 1, 1, 2}, jj[6] = {0, 1, 2, 1, 2, 2};
 #pragma unroll
-    for (int i = 0; i < 3; ++i)
-#pragma unroll
-        for (int j = 0; j < 3; ++j)
-            P[i][j] = (U[i][0] * d[0] * U[j][0] + U[i][1] * d[1] * U[j][1]) + U[i][2] * d[2] * U[j][2] +
-                      (i == j ? c : 0.0f);
-#pragma unroll
-    for (int i = 0; i < 3; ++i)
-#pragma unroll
-        for (int j = 0; j < 3; ++j) vs[i][j] = -(V0 * P[i][j]);
+    for (int k = 0; k < 6; ++k) {
+        const int i = ii[k], j = jj[k];
+        const float P = (U[i][0] * d[0] * U[j][0] + U[i][1] * d[1] * U[j][1]) + U[i][2] * d[2] * U[j][2] +
+                        (i == j ? c : 0.0f);
+        tau[k] = -(V0 * P);
+    }
 }
 
 // updateDeformationGradient, src/material_point3d.cu:53-70: trial Fe, SVD,
 // clamp singular values to [1-theta_c, 1+theta_s], split into Fe and Fp.
+// Returns the SVD of the new Fe (U, clamped s, V) for the stress.
 __device__ __forceinline__ void update_deformation(float Fe[3][3], float Fp[3][3], const float grad[3][3], float dt,
-                                                   const MaterialDev& mat) {
+                                                   const MaterialDev& mat, float U[3][3], float s[3],
+                                                   float V[3][3]) {
     float L[3][3], Fn[3][3];
 #pragma unroll
     for (int i = 0; i < 3; ++i)
@@ -160,14 +164,13 @@ __device__ __forceinline__ void update_deformation(float Fe[3][3], float Fp[3][3
     m3_mul(L, Fe, Fn);
     float F[3][3];
     m3_mul(Fn, Fp, F);
-    float U[3][3], V[3][3], s[3];
     svd3(Fn, U, s, V);
-    float hi = 1.0f + mat.crit_s, lo = 1.0f - mat.crit_c;
+    const float hi = 1.0f + mat.crit_s, lo = 1.0f - mat.crit_c;
     float is[3];
 #pragma unroll
     for (int i = 0; i < 3; ++i) {
         s[i] = fmaxf(fminf(s[i], hi), lo);
-        is[i] = 1.0f / s[i];
+        is[i] = mpm_rcp(s[i]);
     }
     float US[3][3], VSi[3][3];
 #pragma unroll
@@ -216,23 +219,30 @@ __device__ __forceinline__ float wave_shl1(float v) {
 }
 
 // Segmented sum toward the first lane of each segment (tree over offsets
-// 1, 2, 4, ...): after step j a lane holds the sum of the next 2^(j+1)
+// 1, 2, 4, ...): after level j a lane holds the sum of the next 2^(j+1)
 // lanes of its segment.  mk[j] marks the lanes whose segment continues at
-// lane + 2^j; nsteps = number of non-empty levels (uniform).
+// lane + 2^j; nsteps = number of non-empty levels (uniform).  Level 0 uses a
+// DPP wave shift fused into the add (VALU); deeper levels use ds_bpermute,
+// which runs on the LDS pipe and leaves the VALU free.
 template <int NV>
 __device__ __forceinline__ void segment_sum(float (&v)[NV], const uint64_t (&mk)[6], int nsteps, int lane) {
+    if (nsteps <= 0) return;
+    {
+        const bool take = (mk[0] >> lane) & 1ull;
 #pragma unroll
-    for (int j = 0; j < 6; ++j) {
+        for (int f = 0; f < NV; ++f) {
+            const float s = v[f] + wave_shl1(v[f]);
+            v[f] = take ? s : v[f];
+        }
+    }
+#pragma unroll
+    for (int j = 1; j < 6; ++j) {
         if (j >= nsteps) break;
         const bool take = (mk[j] >> lane) & 1ull;
 #pragma unroll
         for (int f = 0; f < NV; ++f) {
-            float t;
-            if (j == 0) t = wave_shl1(v[f]);
-            else if (j == 1) t = wave_shl1(wave_shl1(v[f]));
-            else if (j == 2) t = wave_shl1(wave_shl1(wave_shl1(wave_shl1(v[f]))));
-            else t = __shfl_down(v[f], 1 << j, 64);
-            v[f] += take ? t : 0.0f;
+            const float t = __shfl_down(v[f], 1 << j, 64);
+            v[f] = take ? v[f] + t : v[f];
         }
     }
 }
@@ -291,13 +301,16 @@ __global__ void __launch_bounds__(64) k_p2g(SceneDev S, ParticlesDev P, ItemsDev
                         }
                 continue;
             }
-            float Fe[3][3], Fp[3][3], vs[3][3];
-            load_m3(P.Fe, N, p, Fe);
-            load_m3(P.Fp, N, p, Fp);
-            const MaterialDev mat = P.mats[P.mat[p]];
-            bool fe_changed;
-            particle_stress(Fe, Fp, P.mu0[p], P.lam0[p], active ? P.V0[p] : 0.0f, mat, vs, fe_changed);
-            if (fe_changed && active) store_m3(P.Fe, N, p, Fe);
+            // cached -V0 P Fe^T (symmetric), written by the previous G2P or k_stress
+            float vs[3][3];
+            {
+                const float act = active ? 1.0f : 0.0f;
+                const float t0 = act * P.tau[p], t1 = act * P.tau[N + p], t2 = act * P.tau[2 * N + p];
+                const float t3 = act * P.tau[3 * N + p], t4 = act * P.tau[4 * N + p], t5 = act * P.tau[5 * N + p];
+                vs[0][0] = t0; vs[0][1] = t1; vs[0][2] = t2;
+                vs[1][0] = t1; vs[1][1] = t3; vs[1][2] = t4;
+                vs[2][0] = t2; vs[2][1] = t4; vs[2][2] = t5;
+            }
             const float v0 = P.v[p], v1 = P.v[N + p], v2 = P.v[2 * N + p];
             float C[3][3];
             float dxn[3][W];
@@ -400,7 +413,8 @@ void launch_p2g(const SceneDev& S, const ParticlesDev& P, const ItemsDev& I, int
 // particlesCollision (:428-445) and advectParticles (:381-391) fused in.
 // SRC 2 is initialize()'s G2PVolume (src/mpm_solver3d.cu:173-214).
 template <int INTERP, int APIC, int SRC>
-__global__ void __launch_bounds__(kG2PWG) k_g2p(SceneDev S, ParticlesDev P, ItemsDev I, int advect, int pcollide) {
+__global__ void __launch_bounds__(kG2PWG) k_g2p(SceneDev S, ParticlesDev P, ItemsDev I, int advect, int pcollide,
+                                                uint32_t* __restrict__ next_keys) {
     constexpr int W = Stencil<INTERP>::W;
     constexpr int T = 4 + W - 1;
     constexpr int TN = T * T * T;
@@ -538,18 +552,26 @@ __global__ void __launch_bounds__(kG2PWG) k_g2p(SceneDev S, ParticlesDev P, Item
             load_m3(P.Fe, N, p, Fe);
             load_m3(P.Fp, N, p, Fp);
             const MaterialDev mat = P.mats[P.mat[p]];
-            update_deformation(Fe, Fp, grad, S.dt, mat);
+            float U[3][3], sv[3], V[3][3], tau[6];
+            update_deformation(Fe, Fp, grad, S.dt, mat, U, sv, V);
+            // stress for the next P2G from the same decomposition
+            stress_from_svd(U, sv, V, Fp, P.mu0[p], P.lam0[p], P.V0[p], mat, tau, Fe);
             store_m3(P.Fe, N, p, Fe);
             store_m3(P.Fp, N, p, Fp);
+#pragma unroll
+            for (int k = 0; k < 6; ++k) P.tau[(int64_t)k * N + p] = tau[k];
             if (APIC) store_m3(P.B, N, p, Bm);
             if (pcollide) boundary_collision(S, x0 + S.dt * n0, x1 + S.dt * n1, x2 + S.dt * n2, n0, n1, n2);
             P.v[p] = n0;
             P.v[N + p] = n1;
             P.v[2 * N + p] = n2;
             if (advect) {
-                P.x[p] = x0 + S.dt * n0;
-                P.x[N + p] = x1 + S.dt * n1;
-                P.x[2 * N + p] = x2 + S.dt * n2;
+                const float y0 = x0 + S.dt * n0, y1 = x1 + S.dt * n1, y2 = x2 + S.dt * n2;
+                P.x[p] = y0;
+                P.x[N + p] = y1;
+                P.x[2 * N + p] = y2;
+                if (next_keys)
+                    next_keys[p] = cell_key(S, y0, y1, y2, Stencil<INTERP>::SHIFT, Stencil<INTERP>::W);
             }
         }
         __syncthreads();
@@ -558,34 +580,57 @@ __global__ void __launch_bounds__(kG2PWG) k_g2p(SceneDev S, ParticlesDev P, Item
 
 template <int INTERP, int APIC>
 static void g2p_dispatch_src(const SceneDev& S, const ParticlesDev& P, const ItemsDev& I, int src, int advect,
-                             int pcollide, int wgs, hipStream_t st) {
+                             int pcollide, int wgs, hipStream_t st, uint32_t* nk) {
     if (src == 0)
-        hipLaunchKernelGGL((k_g2p<INTERP, APIC, 0>), dim3(wgs), dim3(kG2PWG), 0, st, S, P, I, advect, pcollide);
+        hipLaunchKernelGGL((k_g2p<INTERP, APIC, 0>), dim3(wgs), dim3(kG2PWG), 0, st, S, P, I, advect, pcollide, nk);
     else if (src == 1)
-        hipLaunchKernelGGL((k_g2p<INTERP, APIC, 1>), dim3(wgs), dim3(kG2PWG), 0, st, S, P, I, advect, pcollide);
+        hipLaunchKernelGGL((k_g2p<INTERP, APIC, 1>), dim3(wgs), dim3(kG2PWG), 0, st, S, P, I, advect, pcollide, nk);
     else
-        hipLaunchKernelGGL((k_g2p<INTERP, APIC, 2>), dim3(wgs), dim3(kG2PWG), 0, st, S, P, I, advect, pcollide);
+        hipLaunchKernelGGL((k_g2p<INTERP, APIC, 2>), dim3(wgs), dim3(kG2PWG), 0, st, S, P, I, advect, pcollide, nk);
 }
 template <int INTERP>
 static void g2p_dispatch_apic(const SceneDev& S, const ParticlesDev& P, const ItemsDev& I, int apic, int src,
-                              int advect, int pcollide, int wgs, hipStream_t st) {
-    if (apic) g2p_dispatch_src<INTERP, 1>(S, P, I, src, advect, pcollide, wgs, st);
-    else g2p_dispatch_src<INTERP, 0>(S, P, I, src, advect, pcollide, wgs, st);
+                              int advect, int pcollide, int wgs, hipStream_t st, uint32_t* nk) {
+    if (apic) g2p_dispatch_src<INTERP, 1>(S, P, I, src, advect, pcollide, wgs, st, nk);
+    else g2p_dispatch_src<INTERP, 0>(S, P, I, src, advect, pcollide, wgs, st, nk);
 }
 void launch_g2p(const SceneDev& S, const ParticlesDev& P, const ItemsDev& I, int interp, int apic, int src, int advect,
-                int pcollide, int grid_wgs, hipStream_t st) {
+                int pcollide, int grid_wgs, hipStream_t st, uint32_t* next_keys) {
     if (P.n <= 0) return;
     switch (interp) {
         case MPM_INTERP_LINEAR:
-            g2p_dispatch_apic<MPM_INTERP_LINEAR>(S, P, I, apic, src, advect, pcollide, grid_wgs, st);
+            g2p_dispatch_apic<MPM_INTERP_LINEAR>(S, P, I, apic, src, advect, pcollide, grid_wgs, st, next_keys);
             break;
         case MPM_INTERP_QUADRATIC_BSPLINE:
-            g2p_dispatch_apic<MPM_INTERP_QUADRATIC_BSPLINE>(S, P, I, apic, src, advect, pcollide, grid_wgs, st);
+            g2p_dispatch_apic<MPM_INTERP_QUADRATIC_BSPLINE>(S, P, I, apic, src, advect, pcollide, grid_wgs, st,
+                                                            next_keys);
             break;
         default:
-            g2p_dispatch_apic<MPM_INTERP_CUBIC_BSPLINE>(S, P, I, apic, src, advect, pcollide, grid_wgs, st);
+            g2p_dispatch_apic<MPM_INTERP_CUBIC_BSPLINE>(S, P, I, apic, src, advect, pcollide, grid_wgs, st, next_keys);
             break;
     }
+}
+
+// ------------------------------------------------------------ stress cache
+// Stress of the current particle state (after creation, initialize() or a
+// state write); afterwards every G2P keeps it current.
+__global__ void k_stress(ParticlesDev P) {
+    int64_t p = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+    if (p >= P.n) return;
+    const int64_t N = P.n;
+    float Fe[3][3], Fp[3][3], U[3][3], V[3][3], sv[3], tau[6];
+    load_m3(P.Fe, N, (uint32_t)p, Fe);
+    load_m3(P.Fp, N, (uint32_t)p, Fp);
+    const MaterialDev mat = P.mats[P.mat[p]];
+    svd3(Fe, U, sv, V);
+    stress_from_svd(U, sv, V, Fp, P.mu0[p], P.lam0[p], P.V0[p], mat, tau, Fe);
+    if (mat.model == MPM_MODEL_NEO_HOOKEAN) store_m3(P.Fe, N, (uint32_t)p, Fe);
+#pragma unroll
+    for (int k = 0; k < 6; ++k) P.tau[(int64_t)k * N + p] = tau[k];
+}
+void launch_stress(const ParticlesDev& P, hipStream_t st) {
+    if (P.n <= 0) return;
+    hipLaunchKernelGGL(k_stress, dim3((unsigned)((P.n + 127) / 128)), dim3(128), 0, st, P);
 }
 
 // ------------------------------------------------------------ grid passes
@@ -790,6 +835,8 @@ __global__ void k_permute(ParticlesDev S, ParticlesDev D, const uint32_t* __rest
     D.lam0[i] = S.lam0[s];
     D.mat[i] = S.mat[s];
     D.pid[i] = S.pid[s];
+#pragma unroll
+    for (int k = 0; k < 6; ++k) D.tau[k * N + i] = S.tau[k * N + s];
 #pragma unroll
     for (int a = 0; a < 3; ++a) {
         D.x[a * N + i] = S.x[a * N + s];

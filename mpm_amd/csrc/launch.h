@@ -25,15 +25,16 @@ void exclusive_scan(uint32_t* data, int64_t n, uint32_t* temp, uint32_t* total, 
 int64_t radix_tiles(int64_t n);
 int64_t radix_hist_words(int64_t n);
 int radix_passes(int key_bits);
-int radix_sort_pairs(uint32_t* k0, uint32_t* v0, uint32_t* k1, uint32_t* v1, int64_t n, int key_bits, uint32_t* hist,
-                     uint32_t* scan_temp, hipStream_t st, const ProfileHook& prof);
+int radix_sort_pairs(const uint32_t* kin, const uint32_t* vin, uint32_t* ka, uint32_t* va, uint32_t* kb, uint32_t* vb,
+                     int64_t n, int key_bits, uint32_t* hist, uint32_t* scan_temp, hipStream_t st,
+                     const ProfileHook& prof);
 
 // ---- kernels.hip
 // Persistent launch width of the item kernels.
 constexpr int kItemMax = 512;   // particles per work item (max)
 
-void launch_keys(const SceneDev& S, const ParticlesDev& P, int interp, uint32_t* keys, uint32_t* vals,
-                 hipStream_t st);
+void launch_keys(const SceneDev& S, const ParticlesDev& P, int interp, uint32_t* keys, hipStream_t st);
+void launch_iota(uint32_t* p, int64_t n, hipStream_t st);
 void launch_items(const uint32_t* sorted_keys, int64_t n, uint32_t* flags, uint32_t* scan_temp, uint32_t* items,
                   uint32_t* item_count, hipStream_t st);
 
@@ -42,8 +43,10 @@ void launch_p2g(const SceneDev& S, const ParticlesDev& P, const ItemsDev& I, int
                 int grid_wgs, hipStream_t st);
 // src: 0 = fused grid update from P2G fields, 1 = materialised v / v*,
 //      2 = initialize() volume gather
+// next_keys (may be null): with advect, the cell key of each particle's new
+// position is written at its storage slot (input of the next sort).
 void launch_g2p(const SceneDev& S, const ParticlesDev& P, const ItemsDev& I, int interp, int apic, int src,
-                int advect, int pcollide, int grid_wgs, hipStream_t st);
+                int advect, int pcollide, int grid_wgs, hipStream_t st, uint32_t* next_keys = nullptr);
 
 void launch_clear(const SceneDev& S, int64_t storage_blocks, int nfields, unsigned long long* active_nodes,
                   hipStream_t st);
@@ -51,6 +54,7 @@ void launch_count_active(const SceneDev& S, int64_t storage_blocks, unsigned lon
                          hipStream_t st);
 // op: 0 gravity, 1 update velocities, 2 grid collision (touched blocks only)
 void launch_grid_op(const SceneDev& S, int64_t storage_blocks, int op, float dt, hipStream_t st);
+void launch_stress(const ParticlesDev& P, hipStream_t st);
 void launch_particles_collision(const SceneDev& S, const ParticlesDev& P, float dt, hipStream_t st);
 void launch_advect(const ParticlesDev& P, float dt, hipStream_t st);
 

@@ -248,14 +248,16 @@ int64_t radix_hist_words(int64_t n) { return radix_tiles(n) * kRadix; }
 
 int radix_passes(int key_bits) { return (key_bits + kRadixBits - 1) / kRadixBits; }
 
-// Sorts (k0,v0) using (k1,v1) as ping-pong; returns 0 if the result is in
-// (k0,v0), 1 if in (k1,v1).
-int radix_sort_pairs(uint32_t* k0, uint32_t* v0, uint32_t* k1, uint32_t* v1, int64_t n, int key_bits, uint32_t* hist,
-                     uint32_t* scan_temp, hipStream_t st, const ProfileHook& prof) {
+// Sorts (kin,vin) (read only) into (ka,va)/(kb,vb) ping-pong buffers;
+// returns 0 if the result is in (ka,va), 1 if in (kb,vb).
+int radix_sort_pairs(const uint32_t* kin, const uint32_t* vin, uint32_t* ka, uint32_t* va, uint32_t* kb, uint32_t* vb,
+                     int64_t n, int key_bits, uint32_t* hist, uint32_t* scan_temp, hipStream_t st,
+                     const ProfileHook& prof) {
     if (n <= 0) return 0;
     int64_t tiles = radix_tiles(n);
     int passes = radix_passes(key_bits);
-    uint32_t *ki = k0, *vi = v0, *ko = k1, *vo = v1;
+    const uint32_t *ki = kin, *vi = vin;
+    uint32_t *ko = ka, *vo = va;
     for (int p = 0; p < passes; ++p) {
         int shift = p * kRadixBits;
         int bits = key_bits - shift < kRadixBits ? key_bits - shift : kRadixBits;
@@ -270,11 +272,12 @@ int radix_sort_pairs(uint32_t* k0, uint32_t* v0, uint32_t* k1, uint32_t* v1, int
         hipLaunchKernelGGL(k_radix_scatter, dim3((unsigned)tiles), dim3(kSortWG), 0, st, ki, vi, ko, vo, n, shift, mask,
                            hist, tiles);
         prof.end("sort_scatter");
-        uint32_t* t;
-        t = ki; ki = ko; ko = t;
-        t = vi; vi = vo; vo = t;
+        ki = ko;
+        vi = vo;
+        ko = (ko == ka) ? kb : ka;
+        vo = (vo == va) ? vb : va;
     }
-    return (passes & 1) ? 1 : 0;
+    return (passes & 1) ? 0 : 1;
 }
 
 }  // namespace mpm
