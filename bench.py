@@ -74,6 +74,111 @@ def cpu_baseline(scene, seconds_target=15.0):
     }
 
 
+def batch_bench(count, steps, warmup, seed=3):
+    """Uneven scene batch: the same scenes stepped one at a time (the plain
+    simulateOneStep loop) and as one batch (eager launches and hipGraph
+    replay, launch parameters from the autotuner)."""
+    from mpm_amd import native as N
+    from mpm_amd import scenes
+    from mpm_amd.batch import Batch
+    from mpm_amd.solver import Solver
+
+    t0 = time.time()
+    scs = scenes.uneven_batch(seed=seed, count=count)
+    ntot = sum(s.particles.count for s in scs)
+    log(f"batch workload: {count} scenes, {ntot} particles, generated in {time.time() - t0:.1f}s")
+    t0 = time.time()
+    solvers = [Solver(s.desc, s.particles) for s in scs]
+    for s in solvers:
+        s.synchronize()
+    log(f"batch scenes created in {time.time() - t0:.1f}s")
+    # plain loop: every scene's own step, one scene after another
+    for _ in range(warmup):
+        for s in solvers:
+            s.step(1)
+    for s in solvers:
+        s.synchronize()
+    rounds = []
+    for _ in range(steps):
+        q0 = time.perf_counter()
+        for s in solvers:
+            s.step(1)
+        for s in solvers:
+            s.synchronize()
+        rounds.append(1e3 * (time.perf_counter() - q0))
+    rounds = np.array(rounds)
+    plain = ntot * steps / (rounds.sum() * 1e-3)
+    # one scene at a time, synchronised: that scene's own step latency
+    solo = []
+    for s in solvers[: min(64, count)]:
+        q0 = time.perf_counter()
+        s.step(1)
+        s.synchronize()
+        solo.append(1e3 * (time.perf_counter() - q0))
+    log(f"plain loop: {rounds.mean():.2f} ms per round of {count} scenes, {plain:.3e} p-u/s")
+    b = Batch(N.QUADRATIC_BSPLINE, N.APIC)
+    for s in solvers:
+        b.add(s)
+    tune = b.autotune(trial_steps=2)
+    log(f"autotune: item_max={tune.item_max} g2p_threads={tune.g2p_threads} workgroups={tune.workgroups} "
+        f"({tune.step_ms:.3f} ms/step)")
+    out = {}
+    for mode in ("eager", "graph"):
+        run = b.step if mode == "eager" else b.step_graph
+        run(warmup)
+        b.synchronize()
+        b.timing(True)
+        q0 = time.perf_counter()
+        run(steps)
+        b.synchronize()
+        el = time.perf_counter() - q0
+        t = b.read_times()
+        b.timing(False)
+        out[mode] = {"value": ntot * steps / el, "ms_per_step": 1e3 * el / steps,
+                     "step_ms_p50": float(np.percentile(t, 50)), "step_ms_p99": float(np.percentile(t, 99))}
+        log(f"batched ({mode}): {1e3 * el / steps:.2f} ms/step, {ntot * steps / el:.3e} p-u/s")
+    info = b.info()
+    b.close()
+    for s in solvers:
+        s.close()
+    best = max(out.values(), key=lambda d: d["value"])
+    return {
+        "workload": f"uneven_batch: {count} scenes, N log-uniform [2k,200k], grids 32^3/64^3/128^3, APIC quadratic",
+        "scenes": count, "particles": ntot, "steps": steps,
+        "plain": {"value": plain, "round_ms_p50": float(np.percentile(rounds, 50)),
+                  "round_ms_p99": float(np.percentile(rounds, 99)),
+                  "scene_step_ms_p50": float(np.percentile(solo, 50)),
+                  "scene_step_ms_p99": float(np.percentile(solo, 99))},
+        "batched": out["eager"], "batched_graph": out["graph"],
+        "speedup_vs_plain": best["value"] / plain,
+        "tune": {"item_max": tune.item_max, "g2p_threads": tune.g2p_threads, "workgroups": tune.workgroups,
+                 "bucket": tune.bucket},
+        "work_items": info["work_items"], "key_bits": info["key_bits"],
+    }
+
+
+def serve_bench(window_s, seed=4, rate=20.0):
+    """Poisson request stream (seeded) over a window of window_s seconds,
+    served to completion under a 200 GB HBM budget."""
+    from mpm_amd import scenes
+    from mpm_amd.serve import Server, requests_from_scenes
+
+    t0 = time.time()
+    reqs = requests_from_scenes(scenes.request_stream(seed=seed, rate=rate, duration=window_s))
+    log(f"serve workload: {len(reqs)} requests over {window_s}s, generated in {time.time() - t0:.1f}s")
+    srv = Server(budget_bytes=200e9, tick_steps=16)
+    try:
+        rep = srv.run(reqs)
+    finally:
+        srv.close()
+    d = rep.summary()
+    log(f"served {rep.completed} requests in {rep.wall_s:.1f}s: {rep.particle_updates_per_s:.3e} p-u/s, "
+        f"request p50/p99 {rep.request_s_p50:.2f}/{rep.request_s_p99:.2f}s")
+    d["workload"] = (f"request_stream: Poisson {rate}/s for {window_s}s, N log-uniform [5k,500k], "
+                     f"steps U[200,2000], PIC-FLIP/APIC 50/50, 64^3-256^3, 200 GB budget")
+    return d
+
+
 def main(argv=None):
     ap = argparse.ArgumentParser()
     ap.add_argument("--gpus", type=int, default=1)
@@ -83,6 +188,10 @@ def main(argv=None):
     ap.add_argument("--graph", action="store_true", help="replay the step as a captured hipGraph")
     ap.add_argument("--no-cpu-baseline", action="store_true")
     ap.add_argument("--seed", type=int, default=None)
+    ap.add_argument("--batch-scenes", type=int, default=256, help="scenes of the batched run (0: skip)")
+    ap.add_argument("--batch-steps", type=int, default=20)
+    ap.add_argument("--serve-seconds", type=float, default=0.0,
+                    help="arrival window of the served request stream (0: skip)")
     ap.add_argument("--calibrate-copy", type=int, default=0, metavar="MIB",
                     help="only run a plain device copy of MIB MiB (HBM counter calibration) and exit")
     a = ap.parse_args(argv)
@@ -174,6 +283,10 @@ def main(argv=None):
         lat.append(1e3 * (time.perf_counter() - q0))
     lat = np.array(lat)
 
+    solver.close()
+    batched = batch_bench(a.batch_scenes, a.batch_steps, 3) if a.batch_scenes > 0 else None
+    served = serve_bench(a.serve_seconds) if a.serve_seconds > 0 else None
+
     cpu = None
     if not a.no_cpu_baseline:
         log("cpu baseline (serial oracle on a bounded sample) ...")
@@ -204,9 +317,10 @@ def main(argv=None):
         "kernels": kernels,
         "roofline": roof,
         "cpu_baseline": cpu,
+        "batched": batched,
+        "served": served,
     }
     print(json.dumps(out), flush=True)
-    solver.close()
     return 0
 
 

@@ -231,6 +231,68 @@ MPM_API int mpm_profile_reset(void* handle);
 /* Fills up to max entries; *count gets the number of kernels tracked. */
 MPM_API int mpm_profile_read(void* handle, mpm_kernel_time* out, int max, int* count);
 
+/* ---------------------------------------------------------------- batches */
+/* No counterpart in the reference, which runs one scene per solver object
+ * (src/mpm_solver3d.cu:77-108 simulateOneStep).  A batch advances many scene
+ * handles of one interpolation and transfer scheme with one set of launches
+ * per step: a batched grid clear, one radix sort over all scenes' keys
+ * (scene-offset key ranges), and the tiled P2G/G2P kernels reading each work
+ * item's scene from a device table.  Per scene the result equals that
+ * scene's own mpm_step up to the summation order of float atomics.
+ *
+ * While a member, a scene handle runs on the batch's stream: its own calls
+ * (reads, sub-steps, mpm_step) stay valid and are ordered with the batch's
+ * steps.  mpm_destroy of a member removes it from the batch first.        */
+
+/* Launch parameters of a batch step. */
+typedef struct mpm_tune {
+    int32_t item_max;     /* particles per work item: multiple of 64 in [64, 65536] */
+    int32_t g2p_threads;  /* threads per G2P workgroup: 64, 128 or 256             */
+    int32_t workgroups;   /* persistent workgroups of P2G/G2P; 0 = 8 per CU        */
+    int32_t bucket;       /* out: size bucket, floor(log2(particles))              */
+    double step_ms;       /* out (autotune): measured ms per step of the choice    */
+} mpm_tune;
+
+typedef struct mpm_batch_stats {
+    int64_t scenes;
+    int64_t particles;
+    int64_t steps;           /* batch steps taken                        */
+    int64_t storage_blocks;  /* 4x4x4 grid blocks over all scenes        */
+    int64_t key_bits;        /* bits of the batch key space (sort width) */
+    int64_t work_items;      /* work items of the last step              */
+    int64_t device_bytes;    /* batch-owned device memory                */
+    double last_step_ms;     /* host time of the last step call (enqueue) */
+    mpm_tune tune;           /* parameters in use                        */
+} mpm_batch_stats;
+
+MPM_API int mpm_batch_create(int32_t interp, int32_t transfer, void** out_batch);
+/* Removes the members (they stay valid scene handles) and frees the batch. */
+MPM_API void mpm_batch_destroy(void* batch);
+/* MPM_ERR_INVALID if the scene's interp/transfer differ from the batch's,
+ * MPM_ERR_STATE if it is already in a batch. */
+MPM_API int mpm_batch_add(void* batch, void* scene);
+MPM_API int mpm_batch_remove(void* batch, void* scene);
+MPM_API int mpm_batch_info(void* batch, mpm_batch_stats* out);
+/* n steps of every member (asynchronous). */
+MPM_API int mpm_batch_step(void* batch, int n);
+/* Same, replaying one step captured as a hipGraph (recaptured after a
+ * membership or tuning change). */
+MPM_API int mpm_batch_step_graph(void* batch, int n);
+MPM_API int mpm_batch_synchronize(void* batch);
+/* Fixes the launch parameters; NULL returns to the tuning table. */
+MPM_API int mpm_batch_set_tune(void* batch, const mpm_tune* tune);
+/* Times candidate parameters on the batch's own scenes (every trial step
+ * is a real step of every member: 1 + trial_steps per candidate, at most 7
+ * candidates), keeps the fastest for the batch and records it in the
+ * process-wide table for this (interp, transfer, size bucket). */
+MPM_API int mpm_batch_autotune(void* batch, int trial_steps, mpm_tune* out);
+MPM_API int mpm_tune_lookup(int32_t interp, int32_t transfer, int64_t particles, mpm_tune* out);
+/* Per-step device time (hipEvents on the batch stream) while enabled. */
+MPM_API int mpm_batch_timing(void* batch, int enable);
+/* Copies up to max recorded step times (ms) and clears the record;
+ * with ms == NULL only *count is set. */
+MPM_API int mpm_batch_read_times(void* batch, float* ms, int max, int* count);
+
 /* ------------------------------------------------------------- test hooks */
 /* Device 3x3 SVD (the solver's own) on n row-major matrices:
  * A = U diag(s) V^T, U,V rotations, s[0] >= s[1] >= |s[2]|. */

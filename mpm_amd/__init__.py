@@ -10,9 +10,10 @@ from .native import (APIC, CUBIC_BSPLINE, FIXED_COROTATED, LINEAR, NEO_HOOKEAN, 
                      MpmError)
 from .solver import (APICMPMSolver3D, MPMSolver3D, ParticleInput, ParticleState, Solver, SolverDesc,
                      debug_svd3)
+from .batch import Batch, Tune, tune_lookup
 
 __all__ = [
     "native", "Solver", "SolverDesc", "ParticleInput", "ParticleState", "MPMSolver3D", "APICMPMSolver3D",
     "debug_svd3", "MpmError", "LINEAR", "QUADRATIC_BSPLINE", "CUBIC_BSPLINE", "FIXED_COROTATED", "NEO_HOOKEAN",
-    "PIC_FLIP", "APIC",
+    "PIC_FLIP", "APIC", "Batch", "Tune", "tune_lookup",
 ]

@@ -1,0 +1,625 @@
+// batch.hip -- many independent scenes advanced by one set of launches.
+//
+// A batch holds scene handles of one interpolation and transfer scheme.
+// Each member keeps its own particle store and grid; the batch adds one
+// global key space in which scene s owns the key range
+//   [key_offset_s, key_offset_s + escape_key_s + 64)   (64-aligned),
+// and one global slot space in which it owns [pstart_s, pstart_s + n_s).
+// One step is then five launch groups over all scenes together:
+//   batched clear of every touched grid block -> one stable radix sort of
+//   all keys -> work items (with their scene index) -> P2G -> G2P
+// where P2G and G2P are the persistent tiled kernels of the single-scene
+// path reading their scene's descriptors from a device table.  Results per
+// scene equal the scene's own mpm_step up to the order of the float
+// atomics in P2G.  The step can be captured once as a hipGraph and
+// replayed.  Launch parameters (particles per work item, G2P workgroup
+// size, persistent workgroup count) come from a tuning table keyed by
+// (interpolation, transfer, size bucket), filled by mpm_batch_autotune.
+#include <hip/hip_runtime.h>
+
+#include <algorithm>
+#include <chrono>
+#include <cstring>
+#include <map>
+#include <mutex>
+#include <new>
+#include <string>
+#include <tuple>
+#include <vector>
+
+#include "launch.h"
+#include "solver_impl.h"
+
+namespace mpm {
+namespace host {
+
+struct Tune {
+    int item_max = kItemMax;
+    int g2p_wg = 128;
+    int wgs = 0;  // 0: 8 per compute unit
+    bool operator==(const Tune& o) const { return item_max == o.item_max && g2p_wg == o.g2p_wg && wgs == o.wgs; }
+};
+
+struct TuneEntry {
+    Tune t;
+    double ms = 0.0;
+};
+
+std::mutex g_tune_mu;
+std::map<std::tuple<int, int, int>, TuneEntry> g_tune;  // (interp, apic, bucket) -> choice
+
+int size_bucket(int64_t n) {
+    int b = 0;
+    while (b < 62 && (n >> (b + 1)) != 0) ++b;
+    return b;
+}
+
+struct Batch {
+    int interp = 0, apic = 0;
+    int device = 0;
+    int cus = 1;
+    hipStream_t st = nullptr;
+    std::vector<Solver*> members;
+    std::vector<uint64_t> synced_version;  // member version whose keys are in kin
+    // layout
+    bool layout_valid = false;
+    int64_t ntot = 0, cap = 0;
+    int64_t total_blocks = 0;
+    int key_bits = 1;
+    std::vector<SceneEntry> tab_host;
+    std::vector<uint32_t> koff_host;
+    std::vector<int64_t> bstart_host;
+    SceneEntry* tab = nullptr;
+    uint32_t* key_offsets = nullptr;
+    int64_t* block_start = nullptr;
+    int tab_cap = 0;
+    uint32_t *kin = nullptr, *iota = nullptr, *ka = nullptr, *va = nullptr, *kb = nullptr, *vb = nullptr;
+    uint32_t *flags = nullptr, *items = nullptr, *item_scene = nullptr, *hist = nullptr, *scan_temp = nullptr;
+    uint32_t* counters = nullptr;
+    ItemsDev I{};
+    Tune tune;
+    bool tune_fixed = false;  // set explicitly: the tuning table is not consulted
+    // graph
+    hipGraph_t graph = nullptr;
+    hipGraphExec_t gexec = nullptr;
+    Tune graph_tune;
+    // stats and timing
+    int64_t steps = 0;
+    double last_step_ms = 0.0;
+    bool timing = false;
+    std::vector<std::pair<hipEvent_t, hipEvent_t>> pending;
+    std::vector<hipEvent_t> event_pool;
+    std::vector<float> step_ms;
+    std::vector<void*> allocs;
+    int64_t device_bytes = 0;
+};
+
+int balloc(Batch& b, void** p, size_t bytes) {
+    if (bytes == 0) bytes = 256;
+    hipError_t e = hipMalloc(p, bytes);
+    if (e != hipSuccess) {
+        *p = nullptr;
+        (void)hipGetLastError();
+        return fail(MPM_ERR_OOM, "hipMalloc(" + std::to_string(bytes) + " B) failed: " + hipGetErrorString(e));
+    }
+    b.allocs.push_back(*p);
+    b.device_bytes += (int64_t)bytes;
+    return MPM_OK;
+}
+
+void bfree(Batch& b, void*& p) {
+    if (!p) return;
+    for (size_t i = 0; i < b.allocs.size(); ++i)
+        if (b.allocs[i] == p) { b.allocs.erase(b.allocs.begin() + (long)i); break; }
+    (void)hipFree(p);
+    p = nullptr;
+}
+
+void drop_graph(Batch& b) {
+    if (b.gexec) { (void)hipGraphExecDestroy(b.gexec); b.gexec = nullptr; }
+    if (b.graph) { (void)hipGraphDestroy(b.graph); b.graph = nullptr; }
+}
+
+int effective_wgs(const Batch& b) { return b.tune.wgs > 0 ? b.tune.wgs : std::max(64, b.cus * 8); }
+
+// Per-particle buffers sized for at least n particles (grown by 1.5x so
+// that a stream of joins does not reallocate every time).
+int ensure_capacity(Batch& b, int64_t n) {
+    if (n <= b.cap && b.kin) return MPM_OK;
+    (void)hipStreamSynchronize(b.st);
+    int64_t c = std::max<int64_t>(std::max<int64_t>(n, 1), b.cap + b.cap / 2);
+    uint32_t** bufs[] = {&b.kin, &b.iota, &b.ka, &b.va, &b.kb, &b.vb, &b.flags, &b.items, &b.item_scene};
+    for (auto pp : bufs) bfree(b, *(void**)pp);
+    bfree(b, *(void**)&b.hist);
+    bfree(b, *(void**)&b.scan_temp);
+    int rc;
+    for (auto pp : bufs)
+        if ((rc = balloc(b, (void**)pp, (size_t)c * 4))) return rc;
+    const int64_t hw = radix_hist_words(c);
+    if ((rc = balloc(b, (void**)&b.hist, (size_t)hw * 4))) return rc;
+    const int64_t tw = std::max(scan_temp_words(hw), scan_temp_words(c));
+    if ((rc = balloc(b, (void**)&b.scan_temp, (size_t)tw * 4))) return rc;
+    b.cap = c;
+    return MPM_OK;
+}
+
+SceneEntry make_entry(const Solver& s, int64_t pstart, uint32_t koff, uint32_t index) {
+    SceneEntry e;
+    std::memset(&e, 0, sizeof(e));  // padding too: entries are compared bytewise
+    std::memcpy(&e.S, &s.S, sizeof(SceneDev));
+    std::memcpy(&e.P, &s.P, sizeof(ParticlesDev));
+    e.pstart = pstart;
+    e.active = s.acc;  // acc[0]: active-node sum over steps
+    e.key_offset = koff;
+    e.index = index;
+    return e;
+}
+
+// Recompute the global layout of the members, (re)upload the tables and
+// the sort input (keys with offsets, identity values).
+int relayout(Batch& b) {
+    drop_graph(b);
+    const int ns = (int)b.members.size();
+    b.tab_host.resize((size_t)ns);
+    b.koff_host.resize((size_t)ns);
+    b.bstart_host.assign((size_t)ns + 1, 0);
+    b.synced_version.assign((size_t)ns, ~0ull);
+    int64_t n = 0, blocks = 0;
+    uint64_t koff = 0;
+    for (int i = 0; i < ns; ++i) {
+        const Solver& s = *b.members[(size_t)i];
+        const uint64_t span = (uint64_t)s.S.escape_key + 64u;
+        if (koff + span > (1ull << 32))
+            return fail(MPM_ERR_UNSUPPORTED, "batch key space exhausted (the scenes' grids total more than 2^32 keys)");
+        b.tab_host[(size_t)i] = make_entry(s, n, (uint32_t)koff, (uint32_t)i);
+        b.koff_host[(size_t)i] = (uint32_t)koff;
+        b.bstart_host[(size_t)i] = blocks;
+        n += s.N;
+        blocks += s.nsb;
+        koff += span;
+    }
+    b.bstart_host[(size_t)ns] = blocks;
+    if (n >= (int64_t)0xFFFFFFFFll) return fail(MPM_ERR_UNSUPPORTED, "batch holds too many particles");
+    b.ntot = n;
+    b.total_blocks = blocks;
+    int bits = 1;
+    while (bits < 32 && (koff > 0 ? ((koff - 1) >> bits) != 0 : false)) ++bits;
+    b.key_bits = bits;
+    int rc;
+    if ((rc = ensure_capacity(b, n))) return rc;
+    if (ns > b.tab_cap || !b.tab) {
+        (void)hipStreamSynchronize(b.st);
+        bfree(b, *(void**)&b.tab);
+        bfree(b, *(void**)&b.key_offsets);
+        bfree(b, *(void**)&b.block_start);
+        const int c = std::max(ns, std::max(16, b.tab_cap * 2));
+        if ((rc = balloc(b, (void**)&b.tab, (size_t)c * sizeof(SceneEntry)))) return rc;
+        if ((rc = balloc(b, (void**)&b.key_offsets, (size_t)c * 4))) return rc;
+        if ((rc = balloc(b, (void**)&b.block_start, (size_t)(c + 1) * 8))) return rc;
+        b.tab_cap = c;
+    }
+    if (ns > 0) {
+        HIP_TRY(hipMemcpyAsync(b.tab, b.tab_host.data(), (size_t)ns * sizeof(SceneEntry), hipMemcpyHostToDevice, b.st));
+        HIP_TRY(hipMemcpyAsync(b.key_offsets, b.koff_host.data(), (size_t)ns * 4, hipMemcpyHostToDevice, b.st));
+    }
+    HIP_TRY(hipMemcpyAsync(b.block_start, b.bstart_host.data(), (size_t)(ns + 1) * 8, hipMemcpyHostToDevice, b.st));
+    launch_iota(b.iota, n, b.st);
+    b.I = ItemsDev{};
+    b.I.start = b.items;
+    b.I.count = b.counters;
+    b.I.scene = b.item_scene;
+    b.I.n = n;
+    if (!b.tune_fixed) {
+        std::lock_guard<std::mutex> g(g_tune_mu);
+        auto it = g_tune.find({b.interp, b.apic, size_bucket(n)});
+        b.tune = it != g_tune.end() ? it->second.t : Tune{};
+    }
+    HIP_TRY(hipStreamSynchronize(b.st));  // host tables must outlive the copies
+    b.layout_valid = true;
+    return MPM_OK;
+}
+
+// Bring every member's device state in line with the batch before a step:
+// table entries (device pointers change on e.g. a material upload), grids
+// left dirty by the member's own sub-steps, the stress cache, and the sort
+// keys of members whose particles moved outside the batch.
+int sync_members(Batch& b) {
+    if (!b.layout_valid) {
+        int rc = relayout(b);
+        if (rc) return rc;
+    }
+    bool upload = false;
+    for (size_t i = 0; i < b.members.size(); ++i) {
+        Solver& s = *b.members[i];
+        SceneEntry e = make_entry(s, b.tab_host[i].pstart, b.tab_host[i].key_offset, (uint32_t)i);
+        if (std::memcmp(&e, &b.tab_host[i], sizeof(SceneEntry)) != 0) {
+            b.tab_host[i] = e;
+            upload = true;
+        }
+        if (s.grid_dirty && (!s.count_on_clear || s.vstar_dirty)) run_clear(s);
+        ensure_stress(s);
+        if (b.synced_version[i] != s.version) {
+            launch_keys(s.S, s.P, b.interp, b.kin + b.tab_host[i].pstart, b.st, b.tab_host[i].key_offset);
+            b.synced_version[i] = s.version;
+        }
+    }
+    if (upload) {
+        drop_graph(b);
+        HIP_TRY(hipMemcpyAsync(b.tab, b.tab_host.data(), b.members.size() * sizeof(SceneEntry), hipMemcpyHostToDevice,
+                               b.st));
+        HIP_TRY(hipStreamSynchronize(b.st));
+    }
+    return MPM_OK;
+}
+
+void enqueue_step(Batch& b) {
+    const int ns = (int)b.members.size();
+    launch_clear_batch(b.tab, b.block_start, ns, b.total_blocks, GF_PRIMARY, b.st);
+    const int sel = radix_sort_pairs(b.kin, b.iota, b.ka, b.va, b.kb, b.vb, b.ntot, b.key_bits, b.hist, b.scan_temp,
+                                     b.st, ProfileHook{});
+    b.I.sorted_keys = sel ? b.kb : b.ka;
+    b.I.perm = sel ? b.vb : b.va;
+    launch_items(b.I.sorted_keys, b.ntot, b.flags, b.scan_temp, b.items, b.counters, b.st, (uint32_t)b.tune.item_max,
+                 b.key_offsets, ns, b.item_scene);
+    const SceneDev S0{};
+    const ParticlesDev P0{};
+    const int wgs = effective_wgs(b);
+    launch_p2g(S0, P0, b.I, b.interp, b.apic, 0, wgs, b.st, b.tab);
+    launch_g2p(S0, P0, b.I, b.interp, b.apic, 0, 1, 0, wgs, b.st, b.kin, b.tab, b.tune.g2p_wg);
+}
+
+// Host-side bookkeeping of the members after n batch steps.
+void after_steps(Batch& b, int n) {
+    for (size_t i = 0; i < b.members.size(); ++i) {
+        Solver& s = *b.members[i];
+        s.steps += n;
+        s.version += (uint64_t)n;
+        b.synced_version[i] = s.version;  // the batch G2P wrote the keys of the new positions
+        s.grid_dirty = true;
+        s.count_on_clear = true;
+        s.phase = PH_RAW;
+        s.sort_valid = false;
+        s.keys_valid = false;  // the member's own sort input is stale
+    }
+    b.steps += n;
+}
+
+hipEvent_t take_event(Batch& b) {
+    if (!b.event_pool.empty()) {
+        hipEvent_t e = b.event_pool.back();
+        b.event_pool.pop_back();
+        return e;
+    }
+    hipEvent_t e = nullptr;
+    if (hipEventCreate(&e) != hipSuccess) return nullptr;
+    return e;
+}
+
+void collect_times(Batch& b) {
+    if (b.pending.empty()) return;
+    (void)hipStreamSynchronize(b.st);
+    for (auto& pr : b.pending) {
+        float ms = 0.0f;
+        if (hipEventElapsedTime(&ms, pr.first, pr.second) == hipSuccess) b.step_ms.push_back(ms);
+        b.event_pool.push_back(pr.first);
+        b.event_pool.push_back(pr.second);
+    }
+    b.pending.clear();
+}
+
+// Runs n steps eagerly (graph = false) or as replays of a captured step.
+int run_steps(Batch& b, int n, bool graph) {
+    int rc = sync_members(b);
+    if (rc) return rc;
+    if (b.members.empty() || b.ntot == 0 || n <= 0) {
+        if (!b.members.empty() && n > 0) after_steps(b, n);
+        return MPM_OK;
+    }
+    if (graph && (!b.gexec || !(b.graph_tune == b.tune))) {
+        drop_graph(b);
+        HIP_TRY(hipStreamBeginCapture(b.st, hipStreamCaptureModeThreadLocal));
+        enqueue_step(b);
+        hipError_t e = hipStreamEndCapture(b.st, &b.graph);
+        if (e != hipSuccess) return fail(MPM_ERR_HIP, std::string("batch graph capture: ") + hipGetErrorString(e));
+        HIP_TRY(hipGraphInstantiate(&b.gexec, b.graph, nullptr, nullptr, 0));
+        b.graph_tune = b.tune;
+    }
+    auto t0 = std::chrono::steady_clock::now();
+    for (int i = 0; i < n; ++i) {
+        hipEvent_t e0 = nullptr, e1 = nullptr;
+        if (b.timing) {
+            e0 = take_event(b);
+            e1 = take_event(b);
+            if (e0 && e1) HIP_TRY(hipEventRecord(e0, b.st));
+        }
+        if (graph) HIP_TRY(hipGraphLaunch(b.gexec, b.st));
+        else enqueue_step(b);
+        if (e0 && e1) {
+            HIP_TRY(hipEventRecord(e1, b.st));
+            b.pending.push_back({e0, e1});
+        }
+    }
+    auto t1 = std::chrono::steady_clock::now();
+    b.last_step_ms = std::chrono::duration<double, std::milli>(t1 - t0).count();
+    after_steps(b, n);
+    hipError_t e = hipGetLastError();
+    if (e != hipSuccess) return fail(MPM_ERR_HIP, std::string("batch step: ") + hipGetErrorString(e));
+    return MPM_OK;
+}
+
+int detach(Batch& b, Solver& s) {
+    auto it = std::find(b.members.begin(), b.members.end(), &s);
+    if (it == b.members.end()) return fail(MPM_ERR_NOT_FOUND, "scene is not a member of this batch");
+    HIP_TRY(hipStreamSynchronize(b.st));
+    b.members.erase(it);
+    s.st = s.own_st;
+    s.own_st = nullptr;
+    s.batch = nullptr;
+    s.keys_valid = false;
+    s.sort_valid = false;
+    b.layout_valid = false;
+    drop_graph(b);
+    return MPM_OK;
+}
+
+void batch_destroy(Batch* b) {
+    if (!b) return;
+    if (b->st) (void)hipStreamSynchronize(b->st);
+    while (!b->members.empty()) (void)detach(*b, *b->members.back());
+    drop_graph(*b);
+    for (auto& pr : b->pending) b->event_pool.push_back(pr.first), b->event_pool.push_back(pr.second);
+    for (auto e : b->event_pool) (void)hipEventDestroy(e);
+    for (void* p : b->allocs) (void)hipFree(p);
+    if (b->st) (void)hipStreamDestroy(b->st);
+    delete b;
+}
+
+// Called by mpm_destroy for a scene that is still a member.
+void batch_forget(Batch* b, Solver* s) {
+    if (b && s) (void)detach(*b, *s);
+}
+
+int check_tune(const mpm_tune* t) {
+    if (!t) return fail(MPM_ERR_INVALID, "tune is NULL");
+    if (t->item_max < 64 || t->item_max > 65536 || (t->item_max % 64) != 0)
+        return fail(MPM_ERR_INVALID, "item_max must be a multiple of 64 in [64, 65536]");
+    if (t->g2p_threads != 64 && t->g2p_threads != 128 && t->g2p_threads != 256)
+        return fail(MPM_ERR_INVALID, "g2p_threads must be 64, 128 or 256");
+    if (t->workgroups < 0 || t->workgroups > (1 << 20)) return fail(MPM_ERR_INVALID, "workgroups out of range");
+    return MPM_OK;
+}
+
+void to_c(const Batch& b, const Tune& t, mpm_tune* out, double ms) {
+    out->item_max = t.item_max;
+    out->g2p_threads = t.g2p_wg;
+    out->workgroups = t.wgs > 0 ? t.wgs : std::max(64, b.cus * 8);
+    out->bucket = size_bucket(b.ntot);
+    out->step_ms = ms;
+}
+
+Batch* as_batch(void* h) { return static_cast<Batch*>(h); }
+
+}  // namespace host
+}  // namespace mpm
+
+using namespace mpm;
+using namespace mpm::host;
+
+#define GET_BATCH(var, h)          \
+    Batch* var = as_batch(h);      \
+    if (!var) return fail(MPM_ERR_INVALID, "batch handle is NULL")
+
+extern "C" {
+
+MPM_API int mpm_batch_create(int32_t interp, int32_t transfer, void** out_batch) {
+    if (!out_batch) return fail(MPM_ERR_INVALID, "out_batch is NULL");
+    *out_batch = nullptr;
+    if (interp < 0 || interp > 2) return fail(MPM_ERR_INVALID, "unknown interpolation type");
+    if (transfer < 0 || transfer > 1) return fail(MPM_ERR_INVALID, "unknown transfer scheme");
+    Batch* b = new (std::nothrow) Batch();
+    if (!b) return fail(MPM_ERR_OOM, "host allocation failed");
+    b->interp = interp;
+    b->apic = transfer == MPM_TRANSFER_APIC;
+    hipError_t e = hipGetDevice(&b->device);
+    hipDeviceProp_t prop;
+    if (e == hipSuccess) e = hipGetDeviceProperties(&prop, b->device);
+    if (e == hipSuccess) e = hipStreamCreateWithFlags(&b->st, hipStreamNonBlocking);
+    if (e == hipSuccess) b->cus = prop.multiProcessorCount;
+    int rc = MPM_OK;
+    if (e != hipSuccess) rc = fail(MPM_ERR_HIP, std::string("batch create: ") + hipGetErrorString(e));
+    if (!rc) rc = balloc(*b, (void**)&b->counters, 64);
+    if (!rc && hipMemsetAsync(b->counters, 0, 64, b->st) != hipSuccess) rc = fail(MPM_ERR_HIP, "batch create: memset");
+    if (!rc) rc = relayout(*b);
+    if (rc) {
+        std::string keep = g_err;
+        batch_destroy(b);
+        g_err = keep;
+        return rc;
+    }
+    *out_batch = b;
+    return MPM_OK;
+}
+
+MPM_API void mpm_batch_destroy(void* batch) { batch_destroy(as_batch(batch)); }
+
+MPM_API int mpm_batch_add(void* batch, void* scene) {
+    GET_BATCH(b, batch);
+    Solver* s = static_cast<Solver*>(scene);
+    if (!s) return fail(MPM_ERR_INVALID, "scene handle is NULL");
+    if (s->batch) return fail(MPM_ERR_STATE, "scene is already a member of a batch");
+    if (s->interp != b->interp || s->apic != b->apic)
+        return fail(MPM_ERR_INVALID, "scene interpolation/transfer differs from the batch's");
+    // Finish the scene's own work, then move it onto the batch stream.
+    if (s->grid_dirty && (!s->count_on_clear || s->vstar_dirty)) run_clear(*s);
+    ensure_stress(*s);
+    HIP_TRY(hipStreamSynchronize(s->st));
+    HIP_TRY(hipStreamSynchronize(b->st));
+    s->own_st = s->st;
+    s->st = b->st;
+    s->batch = b;
+    b->members.push_back(s);
+    b->layout_valid = false;
+    drop_graph(*b);
+    return finish(*s);
+}
+
+MPM_API int mpm_batch_remove(void* batch, void* scene) {
+    GET_BATCH(b, batch);
+    Solver* s = static_cast<Solver*>(scene);
+    if (!s) return fail(MPM_ERR_INVALID, "scene handle is NULL");
+    return detach(*b, *s);
+}
+
+MPM_API int mpm_batch_info(void* batch, mpm_batch_stats* out) {
+    GET_BATCH(b, batch);
+    if (!out) return fail(MPM_ERR_INVALID, "out is NULL");
+    int rc = sync_members(*b);
+    if (rc) return rc;
+    std::memset(out, 0, sizeof(*out));
+    uint32_t items = 0;
+    if (b->ntot > 0 && b->steps > 0) {
+        HIP_TRY(hipMemcpyAsync(&items, b->counters, 4, hipMemcpyDeviceToHost, b->st));
+    }
+    HIP_TRY(hipStreamSynchronize(b->st));
+    out->scenes = (int64_t)b->members.size();
+    out->particles = b->ntot;
+    out->steps = b->steps;
+    out->storage_blocks = b->total_blocks;
+    out->key_bits = b->key_bits;
+    out->work_items = items;
+    out->device_bytes = b->device_bytes;
+    out->last_step_ms = b->last_step_ms;
+    to_c(*b, b->tune, &out->tune, 0.0);
+    return MPM_OK;
+}
+
+MPM_API int mpm_batch_step(void* batch, int n) {
+    GET_BATCH(b, batch);
+    if (n < 0) return fail(MPM_ERR_INVALID, "n < 0");
+    return run_steps(*b, n, false);
+}
+
+MPM_API int mpm_batch_step_graph(void* batch, int n) {
+    GET_BATCH(b, batch);
+    if (n < 0) return fail(MPM_ERR_INVALID, "n < 0");
+    return run_steps(*b, n, true);
+}
+
+MPM_API int mpm_batch_synchronize(void* batch) {
+    GET_BATCH(b, batch);
+    HIP_TRY(hipStreamSynchronize(b->st));
+    hipError_t e = hipGetLastError();
+    if (e != hipSuccess) return fail(MPM_ERR_HIP, std::string("batch: ") + hipGetErrorString(e));
+    return MPM_OK;
+}
+
+MPM_API int mpm_batch_set_tune(void* batch, const mpm_tune* t) {
+    GET_BATCH(b, batch);
+    if (!t) {  // back to the tuning table
+        b->tune_fixed = false;
+        b->layout_valid = false;
+        return MPM_OK;
+    }
+    int rc = check_tune(t);
+    if (rc) return rc;
+    b->tune.item_max = t->item_max;
+    b->tune.g2p_wg = t->g2p_threads;
+    b->tune.wgs = t->workgroups;
+    b->tune_fixed = true;
+    return MPM_OK;
+}
+
+MPM_API int mpm_batch_autotune(void* batch, int trial_steps, mpm_tune* out) {
+    GET_BATCH(b, batch);
+    if (trial_steps < 1) return fail(MPM_ERR_INVALID, "trial_steps < 1");
+    int rc = sync_members(*b);
+    if (rc) return rc;
+    if (b->ntot == 0) return fail(MPM_ERR_STATE, "batch is empty");
+    // Coordinate search from the defaults: particles per work item, then
+    // G2P workgroup size, then persistent workgroups.  Every trial step is
+    // a real step of every scene (the parameters change the work split,
+    // not the arithmetic).
+    const int base_wgs = std::max(64, b->cus * 8);
+    const int item_opts[] = {256, 512, 1024};
+    const int wg_opts[] = {64, 128, 256};
+    const int wgs_opts[] = {base_wgs / 2, base_wgs, base_wgs * 2};
+    hipEvent_t e0 = nullptr, e1 = nullptr;
+    HIP_TRY(hipEventCreate(&e0));
+    HIP_TRY(hipEventCreate(&e1));
+    const bool fixed = b->tune_fixed;
+    b->tune_fixed = true;
+    auto measure = [&](const Tune& t, double& ms) -> int {
+        b->tune = t;
+        int r = run_steps(*b, 1, false);  // warm-up (code objects, caches)
+        if (r) return r;
+        HIP_TRY(hipEventRecord(e0, b->st));
+        r = run_steps(*b, trial_steps, false);
+        if (r) return r;
+        HIP_TRY(hipEventRecord(e1, b->st));
+        HIP_TRY(hipEventSynchronize(e1));
+        float t_ms = 0.0f;
+        HIP_TRY(hipEventElapsedTime(&t_ms, e0, e1));
+        ms = (double)t_ms / trial_steps;
+        return MPM_OK;
+    };
+    Tune best;
+    best.wgs = base_wgs;
+    double best_ms = 0.0;
+    rc = measure(best, best_ms);
+    for (int stage = 0; stage < 3 && !rc; ++stage) {
+        for (int k = 0; k < 3 && !rc; ++k) {
+            Tune t = best;
+            if (stage == 0) t.item_max = item_opts[k];
+            if (stage == 1) t.g2p_wg = wg_opts[k];
+            if (stage == 2) t.wgs = wgs_opts[k];
+            if (t == best) continue;
+            double ms = 0.0;
+            rc = measure(t, ms);
+            if (!rc && ms < best_ms) { best = t; best_ms = ms; }
+        }
+    }
+    (void)hipEventDestroy(e0);
+    (void)hipEventDestroy(e1);
+    b->tune_fixed = fixed;
+    if (rc) return rc;
+    b->tune = best;
+    {
+        std::lock_guard<std::mutex> g(g_tune_mu);
+        g_tune[{b->interp, b->apic, size_bucket(b->ntot)}] = TuneEntry{best, best_ms};
+    }
+    if (out) to_c(*b, best, out, best_ms);
+    return MPM_OK;
+}
+
+MPM_API int mpm_batch_timing(void* batch, int enable) {
+    GET_BATCH(b, batch);
+    collect_times(*b);
+    b->timing = enable != 0;
+    return MPM_OK;
+}
+
+MPM_API int mpm_batch_read_times(void* batch, float* ms, int max, int* count) {
+    GET_BATCH(b, batch);
+    collect_times(*b);
+    const int k = (int)b->step_ms.size();
+    if (count) *count = k;
+    if (!ms) return MPM_OK;  // count only
+    for (int i = 0; i < std::min(k, max); ++i) ms[i] = b->step_ms[(size_t)i];
+    b->step_ms.clear();
+    return MPM_OK;
+}
+
+MPM_API int mpm_tune_lookup(int32_t interp, int32_t transfer, int64_t particles, mpm_tune* out) {
+    if (!out) return fail(MPM_ERR_INVALID, "out is NULL");
+    std::lock_guard<std::mutex> g(g_tune_mu);
+    auto it = g_tune.find({interp, transfer == MPM_TRANSFER_APIC ? 1 : 0, size_bucket(particles)});
+    if (it == g_tune.end()) return fail(MPM_ERR_NOT_FOUND, "no tuning entry for this size bucket");
+    out->item_max = it->second.t.item_max;
+    out->g2p_threads = it->second.t.g2p_wg;
+    out->workgroups = it->second.t.wgs;
+    out->bucket = size_bucket(particles);
+    out->step_ms = it->second.ms;
+    return MPM_OK;
+}
+
+}  // extern "C"

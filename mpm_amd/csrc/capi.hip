@@ -21,75 +21,12 @@
 
 #include "devmath.h"
 #include "launch.h"
+#include "solver_impl.h"
 
 using namespace mpm;
 
-namespace {
-
-thread_local std::string g_err;
-
-int fail(int code, const std::string& msg) {
-    g_err = msg;
-    return code;
-}
-
-#define HIP_TRY(call)                                                                                 \
-    do {                                                                                              \
-        hipError_t e_ = (call);                                                                       \
-        if (e_ != hipSuccess) {                                                                       \
-            return fail(MPM_ERR_HIP, std::string(#call) + ": " + hipGetErrorString(e_) + " at " +     \
-                                         __FILE__ + ":" + std::to_string(__LINE__));                  \
-        }                                                                                             \
-    } while (0)
-
-enum GridPhase { PH_CLEAR = 0, PH_MASS = 1, PH_RAW = 2, PH_SUBSTEP = 3 };
-
-struct Pending {
-    std::string name;
-    hipEvent_t a, b;
-};
-
-struct Solver {
-    mpm_desc desc{};
-    int interp = 0, apic = 0;
-    int64_t N = 0;
-    int device = 0;
-    hipStream_t st = nullptr;
-    SceneDev S{};
-    ParticlesDev P{};
-    ItemsDev I{};
-    std::vector<MaterialDev> mats;
-    MaterialDev* mats_dev = nullptr;
-    int64_t nsb = 0;
-    int key_bits = 1;
-    uint32_t* kin = nullptr;   // sort input keys (slot order); written by k_keys or the fused G2P
-    uint32_t* iota = nullptr;  // sort input values: slot index
-    uint32_t *ka = nullptr, *va = nullptr, *kb = nullptr, *vb = nullptr;  // ping-pong / result
-    bool keys_valid = false;   // kin matches the current positions
-    uint32_t* hist = nullptr;
-    uint32_t* scan_temp = nullptr;
-    uint32_t* flags = nullptr;
-    uint32_t* items = nullptr;
-    uint32_t* counters = nullptr;                 // [0] item count
-    unsigned long long* acc = nullptr;            // [0] active sum, [1] scratch, [2] escaped
-    bool sort_valid = false, grid_dirty = false, vstar_dirty = false, count_on_clear = false;
-    bool stress_valid = false;  // P.tau matches the particle state
-    int phase = PH_CLEAR;
-    int64_t steps = 0;
-    int item_wgs = 1024;
-    double last_step_ms = 0.0;
-    std::vector<void*> allocs;
-    int64_t device_bytes = 0;
-    // profiling
-    bool prof = false;
-    std::vector<Pending> pending;
-    std::vector<hipEvent_t> event_pool;
-    std::map<std::string, std::pair<int64_t, double>> prof_tot;
-    std::vector<std::string> prof_order;
-    // graph
-    hipGraph_t graph = nullptr;
-    hipGraphExec_t gexec = nullptr;
-};
+namespace mpm {
+namespace host {
 
 int dalloc(Solver& s, void** p, size_t bytes) {
     if (bytes == 0) bytes = 256;
@@ -169,12 +106,6 @@ ProfileHook hook(Solver& s) {
     }
     return h;
 }
-struct Scope {
-    const ProfileHook& h;
-    const char* n;
-    Scope(const ProfileHook& h_, const char* n_) : h(h_), n(n_) { h.begin(n); }
-    ~Scope() { h.end(n); }
-};
 
 // ---- validation
 int check_desc(const mpm_desc* d) {
@@ -414,6 +345,7 @@ void run_fused_step(Solver& s) {
     s.sort_valid = false;
     s.keys_valid = true;  // the G2P wrote the keys of the advected positions
     s.steps += 1;
+    s.version += 1;
 }
 
 void run_initialize(Solver& s) {
@@ -514,6 +446,7 @@ int build(Solver& s, HostState& H, int run_init) {
 
 void destroy(Solver* s) {
     if (!s) return;
+    if (s->batch) batch_forget(s->batch, s);
     if (s->st) (void)hipStreamSynchronize(s->st);
     if (s->gexec) (void)hipGraphExecDestroy(s->gexec);
     if (s->graph) (void)hipGraphDestroy(s->graph);
@@ -556,7 +489,10 @@ int create_common(const mpm_desc* desc, HostState& H, int64_t n, const std::vect
     return MPM_OK;
 }
 
-}  // namespace
+}  // namespace host
+}  // namespace mpm
+
+using namespace mpm::host;
 
 extern "C" {
 
@@ -677,6 +613,7 @@ MPM_API int mpm_step_graph(void* handle, int n) {
     auto t0 = std::chrono::steady_clock::now();
     for (int i = 0; i < n; ++i) HIP_TRY(hipGraphLaunch(s->gexec, s->st));
     s->steps += n;
+    s->version += (uint64_t)n;
     s->grid_dirty = true;
     s->count_on_clear = true;
     s->vstar_dirty = true;
@@ -765,6 +702,7 @@ MPM_API int mpm_advect_particles(void* handle, float dt) {
     launch_advect(s->P, dt, s->st);
     s->sort_valid = false;
     s->keys_valid = false;
+    s->version += 1;
     return finish(*s);
 }
 
@@ -874,6 +812,7 @@ MPM_API int mpm_write_particles(void* handle, const mpm_particle_state* in) {
     if (e != hipSuccess) return fail(MPM_ERR_HIP, std::string("write particles: ") + hipGetErrorString(e));
     s->sort_valid = false;
     s->keys_valid = false;
+    s->version += 1;
     s->stress_valid = false;
     return finish(*s);
 }

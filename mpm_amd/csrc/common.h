@@ -73,12 +73,26 @@ struct ParticlesDev {
     const MaterialDev* mats;
 };
 
+// One scene as the item kernels see it.  A batch sorts the particles of
+// all its scenes together: scene s owns global slots [pstart, pstart + n)
+// and global keys key_offset + (its cell key); key_offset is a multiple of
+// 64 so that key >> 6 (the block) never mixes scenes.
+struct SceneEntry {
+    SceneDev S;
+    ParticlesDev P;
+    int64_t pstart;
+    unsigned long long* active;  // the scene's active-node counter
+    uint32_t key_offset;
+    uint32_t index;
+};
+
 // Work items of the block-tiled kernels: a contiguous run of sorted
-// particles that share one key block.
+// particles that share one key block (and so one scene).
 struct ItemsDev {
-    const uint32_t* sorted_keys;  // N
-    const uint32_t* perm;         // N, sorted position -> storage slot
+    const uint32_t* sorted_keys;  // N (global keys)
+    const uint32_t* perm;         // N, sorted position -> global slot
     const uint32_t* start;        // item start positions
     const uint32_t* count;        // device counter: number of items
+    const uint32_t* scene;        // per item scene index (batches) or null
     int64_t n;
 };

@@ -21,23 +21,28 @@ namespace mpm {
 constexpr int kG2PWG = 128;
 
 // ------------------------------------------------------------------ keys
+// Cell key of each particle, plus the scene's key offset in a batch.
 template <int INTERP>
-__global__ void k_keys(SceneDev S, ParticlesDev P, uint32_t* __restrict__ keys) {
+__global__ void k_keys(SceneDev S, ParticlesDev P, uint32_t key_offset, uint32_t* __restrict__ keys) {
     int64_t i = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
     if (i >= P.n) return;
     const int64_t N = P.n;
-    keys[i] = cell_key(S, P.x[i], P.x[N + i], P.x[2 * N + i], Stencil<INTERP>::SHIFT, Stencil<INTERP>::W);
+    keys[i] = key_offset +
+              cell_key(S, P.x[i], P.x[N + i], P.x[2 * N + i], Stencil<INTERP>::SHIFT, Stencil<INTERP>::W);
 }
 
-void launch_keys(const SceneDev& S, const ParticlesDev& P, int interp, uint32_t* keys, hipStream_t st) {
+void launch_keys(const SceneDev& S, const ParticlesDev& P, int interp, uint32_t* keys, hipStream_t st,
+                 uint32_t key_offset) {
     if (P.n <= 0) return;
     dim3 g((unsigned)((P.n + 255) / 256)), b(256);
     switch (interp) {
-        case MPM_INTERP_LINEAR: hipLaunchKernelGGL(k_keys<MPM_INTERP_LINEAR>, g, b, 0, st, S, P, keys); break;
-        case MPM_INTERP_QUADRATIC_BSPLINE:
-            hipLaunchKernelGGL(k_keys<MPM_INTERP_QUADRATIC_BSPLINE>, g, b, 0, st, S, P, keys);
+        case MPM_INTERP_LINEAR:
+            hipLaunchKernelGGL(k_keys<MPM_INTERP_LINEAR>, g, b, 0, st, S, P, key_offset, keys);
             break;
-        default: hipLaunchKernelGGL(k_keys<MPM_INTERP_CUBIC_BSPLINE>, g, b, 0, st, S, P, keys); break;
+        case MPM_INTERP_QUADRATIC_BSPLINE:
+            hipLaunchKernelGGL(k_keys<MPM_INTERP_QUADRATIC_BSPLINE>, g, b, 0, st, S, P, key_offset, keys);
+            break;
+        default: hipLaunchKernelGGL(k_keys<MPM_INTERP_CUBIC_BSPLINE>, g, b, 0, st, S, P, key_offset, keys); break;
     }
 }
 
@@ -51,26 +56,48 @@ void launch_iota(uint32_t* p, int64_t n, hipStream_t st) {
 }
 
 // ----------------------------------------------------------------- items
-__device__ __forceinline__ bool item_head(const uint32_t* sk, int64_t i) {
-    return i == 0 || (sk[i] >> 6) != (sk[i - 1] >> 6) || (i % kItemMax) == 0;
+// A work item starts where the key block changes (in a batch this includes
+// every scene boundary, since scene key ranges are 64-aligned) and every
+// item_max particles within a block.
+__device__ __forceinline__ bool item_head(const uint32_t* sk, int64_t i, uint32_t item_max) {
+    return i == 0 || (sk[i] >> 6) != (sk[i - 1] >> 6) || ((uint64_t)i % item_max) == 0;
 }
-__global__ void k_item_flags(const uint32_t* __restrict__ sk, int64_t n, uint32_t* __restrict__ flags) {
+__global__ void k_item_flags(const uint32_t* __restrict__ sk, int64_t n, uint32_t item_max,
+                             uint32_t* __restrict__ flags) {
     int64_t i = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
-    if (i < n) flags[i] = item_head(sk, i) ? 1u : 0u;
+    if (i < n) flags[i] = item_head(sk, i, item_max) ? 1u : 0u;
 }
-__global__ void k_item_compact(const uint32_t* __restrict__ sk, int64_t n, const uint32_t* __restrict__ pos,
-                               uint32_t* __restrict__ items) {
+// Writes the start position of each item and, for a batch, its scene: the
+// last scene whose key offset is <= the item's first key.
+__global__ void k_item_compact(const uint32_t* __restrict__ sk, int64_t n, uint32_t item_max,
+                               const uint32_t* __restrict__ pos, uint32_t* __restrict__ items,
+                               const uint32_t* __restrict__ key_offsets, int nscenes,
+                               uint32_t* __restrict__ item_scene) {
     int64_t i = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
-    if (i < n && item_head(sk, i)) items[pos[i]] = (uint32_t)i;
+    if (i >= n || !item_head(sk, i, item_max)) return;
+    items[pos[i]] = (uint32_t)i;
+    if (item_scene) {
+        const uint32_t key = sk[i];
+        int lo = 0, hi = nscenes - 1;
+        while (lo < hi) {
+            const int mid = (lo + hi + 1) >> 1;
+            if (key_offsets[mid] <= key) lo = mid;
+            else hi = mid - 1;
+        }
+        item_scene[pos[i]] = (uint32_t)lo;
+    }
 }
 
 void launch_items(const uint32_t* sorted_keys, int64_t n, uint32_t* flags, uint32_t* scan_temp, uint32_t* items,
-                  uint32_t* item_count, hipStream_t st) {
+                  uint32_t* item_count, hipStream_t st, uint32_t item_max, const uint32_t* key_offsets,
+                  int nscenes, uint32_t* item_scene) {
     if (n <= 0) return;
+    if (item_max == 0) item_max = kItemMax;
     dim3 g((unsigned)((n + 255) / 256)), b(256);
-    hipLaunchKernelGGL(k_item_flags, g, b, 0, st, sorted_keys, n, flags);
+    hipLaunchKernelGGL(k_item_flags, g, b, 0, st, sorted_keys, n, item_max, flags);
     exclusive_scan(flags, n, scan_temp, item_count, st);
-    hipLaunchKernelGGL(k_item_compact, g, b, 0, st, sorted_keys, n, flags, items);
+    hipLaunchKernelGGL(k_item_compact, g, b, 0, st, sorted_keys, n, item_max, flags, items, key_offsets, nscenes,
+                       item_scene);
 }
 
 // ------------------------------------------------------- item decoding
@@ -79,11 +106,13 @@ struct ItemGeom {
     bool escape;
     int ox, oy, oz;  // tile origin node
 };
-__device__ __forceinline__ ItemGeom item_geom(const SceneDev& S, const ItemsDev& I, uint32_t it, uint32_t nitems) {
+// key_offset: the scene's offset in the batch key space (0 for one scene).
+__device__ __forceinline__ ItemGeom item_geom(const SceneDev& S, const ItemsDev& I, uint32_t it, uint32_t nitems,
+                                              uint32_t key_offset) {
     ItemGeom g;
     g.s = I.start[it];
     g.e = (it + 1 < nitems) ? I.start[it + 1] : (uint32_t)I.n;
-    uint32_t blk = I.sorted_keys[g.s] >> 6;
+    uint32_t blk = (I.sorted_keys[g.s] - key_offset) >> 6;
     g.escape = blk == (S.escape_key >> 6);
     uint32_t bx = blk % (uint32_t)S.kbx;
     uint32_t rest = blk / (uint32_t)S.kbx;
@@ -93,6 +122,27 @@ __device__ __forceinline__ ItemGeom item_geom(const SceneDev& S, const ItemsDev&
     g.oy = 4 * (int)by - 4;
     g.oz = 4 * (int)bz - 4;
     return g;
+}
+
+// The scene an item belongs to: the kernel arguments for one scene, the
+// batch table entry (read with scalar loads, the index is uniform) for a
+// batch.  pbase is subtracted from the permutation's global slots.
+template <bool BATCH>
+__device__ __forceinline__ void item_scene(const SceneDev& S0, const ParticlesDev& P0, const ItemsDev& I,
+                                           const SceneEntry* __restrict__ tab, uint32_t it, SceneDev& S,
+                                           ParticlesDev& P, uint32_t& pbase, uint32_t& koff) {
+    if constexpr (BATCH) {
+        const uint32_t sc = I.scene[it];
+        S = tab[sc].S;
+        P = tab[sc].P;
+        pbase = (uint32_t)tab[sc].pstart;
+        koff = tab[sc].key_offset;
+    } else {
+        S = S0;
+        P = P0;
+        pbase = 0;
+        koff = 0;
+    }
 }
 
 __device__ __forceinline__ bool in_grid(const SceneDev& S, int i, int j, int k) {
@@ -247,18 +297,23 @@ __device__ __forceinline__ void segment_sum(float (&v)[NV], const uint64_t (&mk)
     }
 }
 
-template <int INTERP, int APIC, int MODE>
-__global__ void __launch_bounds__(64) k_p2g(SceneDev S, ParticlesDev P, ItemsDev I) {
+template <int INTERP, int APIC, int MODE, bool BATCH>
+__global__ void __launch_bounds__(64) k_p2g(SceneDev S0, ParticlesDev P0, ItemsDev I,
+                                            const SceneEntry* __restrict__ tab) {
     constexpr int W = Stencil<INTERP>::W;
     constexpr int T = 4 + W - 1;
     constexpr int TN = T * T * T;
     constexpr int NF = MODE == 0 ? 7 : 1;
     __shared__ float tile[NF * TN];
     const uint32_t nitems = *I.count;
-    const int64_t N = P.n;
     const int lane = threadIdx.x;
     for (uint32_t it = blockIdx.x; it < nitems; it += gridDim.x) {
-        ItemGeom g = item_geom(S, I, it, nitems);
+        SceneDev S;
+        ParticlesDev P;
+        uint32_t pbase, koff;
+        item_scene<BATCH>(S0, P0, I, tab, it, S, P, pbase, koff);
+        const int64_t N = P.n;
+        ItemGeom g = item_geom(S, I, it, nitems, koff);
         if (g.escape) continue;  // no stencil node inside the grid
         for (int q = lane; q < NF * TN; q += 64) tile[q] = 0.0f;
         __syncthreads();
@@ -278,7 +333,7 @@ __global__ void __launch_bounds__(64) k_p2g(SceneDev S, ParticlesDev P, ItemsDev
                 mk[j] = __ballot(active && lane + (1 << j) < 64 && other == key);
                 if (mk[j] != 0ull) nsteps = j + 1;
             }
-            const uint32_t p = I.perm[ic];
+            const uint32_t p = I.perm[ic] - pbase;
             const float x0 = P.x[p], x1 = P.x[N + p], x2 = P.x[2 * N + p];
             int bx, by, bz;
             float Nx[W], Ny[W], Nz[W], dNx[W], dNy[W], dNz[W];
@@ -312,15 +367,20 @@ __global__ void __launch_bounds__(64) k_p2g(SceneDev S, ParticlesDev P, ItemsDev
                 vs[2][0] = t2; vs[2][1] = t4; vs[2][2] = t5;
             }
             const float v0 = P.v[p], v1 = P.v[N + p], v2 = P.v[2 * N + p];
-            float C[3][3];
+            // Tensor-product factorisation of the node contributions
+            // (w = Nx[a] Ny[b] Nz[c]):
+            //   mass     m w                 = (m Nx[a]) yz
+            //   momentum m w (v + C (x_i - x_p)) = mip (cx(a) + cyz(b,c))
+            //   force    vs grad w           = F0(a) yz + F1(a) dyz + F2(a) ydz
             float dxn[3][W];
+            float Cm[3][3];
             if (APIC) {
                 float Bm[3][3];
                 load_m3(P.B, N, p, Bm);
 #pragma unroll
                 for (int i2 = 0; i2 < 3; ++i2)
 #pragma unroll
-                    for (int j2 = 0; j2 < 3; ++j2) C[i2][j2] = Bm[i2][j2] * S.dinv;
+                    for (int j2 = 0; j2 < 3; ++j2) Cm[i2][j2] = Bm[i2][j2] * S.dinv;
 #pragma unroll
                 for (int j = 0; j < W; ++j) {
                     dxn[0][j] = (S.ox + S.h * (float)(bx + j)) - x0;
@@ -328,32 +388,40 @@ __global__ void __launch_bounds__(64) k_p2g(SceneDev S, ParticlesDev P, ItemsDev
                     dxn[2][j] = (S.oz + S.h * (float)(bz + j)) - x2;
                 }
             }
+            float yz[W][W], dyz[W][W], ydz[W][W];
 #pragma unroll
-            for (int a = 0; a < W; ++a)
+            for (int b = 0; b < W; ++b)
+#pragma unroll
+                for (int c = 0; c < W; ++c) {
+                    yz[b][c] = Ny[b] * Nz[c];
+                    dyz[b][c] = dNy[b] * Nz[c];
+                    ydz[b][c] = Ny[b] * dNz[c];
+                }
+#pragma unroll
+            for (int a = 0; a < W; ++a) {
+                const float ma = m * Nx[a];
+                float F0[3], F1[3], F2[3], cx[3];
+#pragma unroll
+                for (int k = 0; k < 3; ++k) {
+                    F0[k] = vs[k][0] * dNx[a];
+                    F1[k] = vs[k][1] * Nx[a];
+                    F2[k] = vs[k][2] * Nx[a];
+                    const float vk = k == 0 ? v0 : (k == 1 ? v1 : v2);
+                    cx[k] = APIC ? vk + Cm[k][0] * dxn[0][a] : vk;
+                }
 #pragma unroll
                 for (int b = 0; b < W; ++b)
 #pragma unroll
                     for (int c = 0; c < W; ++c) {
-                        const float w = (Nx[a] * Ny[b]) * Nz[c];
-                        const float gx = (dNx[a] * Ny[b]) * Nz[c];
-                        const float gy = (Nx[a] * dNy[b]) * Nz[c];
-                        const float gz = (Nx[a] * Ny[b]) * dNz[c];
-                        const float mip = m * w;
+                        const float mip = ma * yz[b][c];
                         float v[7];
                         v[0] = mip;
-                        if (APIC) {
-                            const float d0 = dxn[0][a], d1 = dxn[1][b], d2 = dxn[2][c];
-                            v[1] = mip * (v0 + ((C[0][0] * d0 + C[0][1] * d1) + C[0][2] * d2));
-                            v[2] = mip * (v1 + ((C[1][0] * d0 + C[1][1] * d1) + C[1][2] * d2));
-                            v[3] = mip * (v2 + ((C[2][0] * d0 + C[2][1] * d1) + C[2][2] * d2));
-                        } else {
-                            v[1] = v0 * mip;
-                            v[2] = v1 * mip;
-                            v[3] = v2 * mip;
+#pragma unroll
+                        for (int k = 0; k < 3; ++k) {
+                            const float ck = APIC ? cx[k] + (Cm[k][1] * dxn[1][b] + Cm[k][2] * dxn[2][c]) : cx[k];
+                            v[1 + k] = mip * ck;
+                            v[4 + k] = (F0[k] * yz[b][c] + F1[k] * dyz[b][c]) + F2[k] * ydz[b][c];
                         }
-                        v[4] = (vs[0][0] * gx + vs[0][1] * gy) + vs[0][2] * gz;
-                        v[5] = (vs[1][0] * gx + vs[1][1] * gy) + vs[1][2] * gz;
-                        v[6] = (vs[2][0] * gx + vs[2][1] * gy) + vs[2][2] * gz;
                         segment_sum<7>(v, mk, nsteps, lane);
                         const int q = ((lz + c) * T + (ly + b)) * T + (lx + a);
                         if (head) {
@@ -361,6 +429,7 @@ __global__ void __launch_bounds__(64) k_p2g(SceneDev S, ParticlesDev P, ItemsDev
                             for (int f = 0; f < 7; ++f) tile[f * TN + q] += v[f];
                         }
                     }
+            }
         }
         __syncthreads();
         for (int q = lane; q < TN; q += 64) {
@@ -384,26 +453,27 @@ __global__ void __launch_bounds__(64) k_p2g(SceneDev S, ParticlesDev P, ItemsDev
 }
 
 template <int INTERP, int APIC>
-static void p2g_dispatch_mode(const SceneDev& S, const ParticlesDev& P, const ItemsDev& I, int mode, int wgs,
-                              hipStream_t st) {
-    if (mode == 0) hipLaunchKernelGGL((k_p2g<INTERP, APIC, 0>), dim3(wgs), dim3(64), 0, st, S, P, I);
-    else hipLaunchKernelGGL((k_p2g<INTERP, APIC, 1>), dim3(wgs), dim3(64), 0, st, S, P, I);
+static void p2g_dispatch_mode(const SceneDev& S, const ParticlesDev& P, const ItemsDev& I, const SceneEntry* tab,
+                              int mode, int wgs, hipStream_t st) {
+    if (tab) hipLaunchKernelGGL((k_p2g<INTERP, APIC, 0, true>), dim3(wgs), dim3(64), 0, st, S, P, I, tab);
+    else if (mode == 0) hipLaunchKernelGGL((k_p2g<INTERP, APIC, 0, false>), dim3(wgs), dim3(64), 0, st, S, P, I, tab);
+    else hipLaunchKernelGGL((k_p2g<INTERP, APIC, 1, false>), dim3(wgs), dim3(64), 0, st, S, P, I, tab);
 }
 template <int INTERP>
-static void p2g_dispatch_apic(const SceneDev& S, const ParticlesDev& P, const ItemsDev& I, int apic, int mode, int wgs,
-                              hipStream_t st) {
-    if (apic) p2g_dispatch_mode<INTERP, 1>(S, P, I, mode, wgs, st);
-    else p2g_dispatch_mode<INTERP, 0>(S, P, I, mode, wgs, st);
+static void p2g_dispatch_apic(const SceneDev& S, const ParticlesDev& P, const ItemsDev& I, const SceneEntry* tab,
+                              int apic, int mode, int wgs, hipStream_t st) {
+    if (apic) p2g_dispatch_mode<INTERP, 1>(S, P, I, tab, mode, wgs, st);
+    else p2g_dispatch_mode<INTERP, 0>(S, P, I, tab, mode, wgs, st);
 }
 void launch_p2g(const SceneDev& S, const ParticlesDev& P, const ItemsDev& I, int interp, int apic, int mode,
-                int grid_wgs, hipStream_t st) {
-    if (P.n <= 0) return;
+                int grid_wgs, hipStream_t st, const SceneEntry* tab) {
+    if (I.n <= 0) return;
     switch (interp) {
-        case MPM_INTERP_LINEAR: p2g_dispatch_apic<MPM_INTERP_LINEAR>(S, P, I, apic, mode, grid_wgs, st); break;
+        case MPM_INTERP_LINEAR: p2g_dispatch_apic<MPM_INTERP_LINEAR>(S, P, I, tab, apic, mode, grid_wgs, st); break;
         case MPM_INTERP_QUADRATIC_BSPLINE:
-            p2g_dispatch_apic<MPM_INTERP_QUADRATIC_BSPLINE>(S, P, I, apic, mode, grid_wgs, st);
+            p2g_dispatch_apic<MPM_INTERP_QUADRATIC_BSPLINE>(S, P, I, tab, apic, mode, grid_wgs, st);
             break;
-        default: p2g_dispatch_apic<MPM_INTERP_CUBIC_BSPLINE>(S, P, I, apic, mode, grid_wgs, st); break;
+        default: p2g_dispatch_apic<MPM_INTERP_CUBIC_BSPLINE>(S, P, I, tab, apic, mode, grid_wgs, st); break;
     }
 }
 
@@ -412,20 +482,24 @@ void launch_p2g(const SceneDev& S, const ParticlesDev& P, const ItemsDev& I, int
 // src/apic_mpm_solver3d.cu:326-388 (APIC), with the optional
 // particlesCollision (:428-445) and advectParticles (:381-391) fused in.
 // SRC 2 is initialize()'s G2PVolume (src/mpm_solver3d.cu:173-214).
-template <int INTERP, int APIC, int SRC>
-__global__ void __launch_bounds__(kG2PWG) k_g2p(SceneDev S, ParticlesDev P, ItemsDev I, int advect, int pcollide,
-                                                uint32_t* __restrict__ next_keys) {
+template <int INTERP, int APIC, int SRC, bool BATCH, int WG>
+__global__ void __launch_bounds__(WG) k_g2p(SceneDev S0, ParticlesDev P0, ItemsDev I, const SceneEntry* __restrict__ tab,
+                                            int advect, int pcollide, uint32_t* __restrict__ next_keys) {
     constexpr int W = Stencil<INTERP>::W;
     constexpr int T = 4 + W - 1;
     constexpr int TN = T * T * T;
     constexpr int NF = SRC == 2 ? 1 : (APIC ? 3 : 6);
     __shared__ float tile[NF * TN];
     const uint32_t nitems = *I.count;
-    const int64_t N = P.n;
     for (uint32_t it = blockIdx.x; it < nitems; it += gridDim.x) {
-        ItemGeom g = item_geom(S, I, it, nitems);
+        SceneDev S;
+        ParticlesDev P;
+        uint32_t pbase, koff;
+        item_scene<BATCH>(S0, P0, I, tab, it, S, P, pbase, koff);
+        const int64_t N = P.n;
+        ItemGeom g = item_geom(S, I, it, nitems, koff);
         if (!g.escape) {
-            for (int q = threadIdx.x; q < TN; q += kG2PWG) {
+            for (int q = threadIdx.x; q < TN; q += WG) {
                 const int lx = q % T, ly = (q / T) % T, lz = q / (T * T);
                 const int gi = g.ox + lx, gj = g.oy + ly, gk = g.oz + lz;
                 float vals[NF];
@@ -461,8 +535,8 @@ __global__ void __launch_bounds__(kG2PWG) k_g2p(SceneDev S, ParticlesDev P, Item
             }
         }
         __syncthreads();
-        for (uint32_t i = g.s + threadIdx.x; i < g.e; i += kG2PWG) {
-            const uint32_t p = I.perm[i];
+        for (uint32_t i = g.s + threadIdx.x; i < g.e; i += WG) {
+            const uint32_t p = I.perm[i] - pbase;
             const float x0 = P.x[p], x1 = P.x[N + p], x2 = P.x[2 * N + p];
             int bx = 0, by = 0, bz = 0;
             float Nx[W], Ny[W], Nz[W], dNx[W], dNy[W], dNz[W];
@@ -509,35 +583,56 @@ __global__ void __launch_bounds__(kG2PWG) k_g2p(SceneDev S, ParticlesDev P, Item
                         dxn[2][j] = (S.oz + S.h * (float)(bz + j)) - x2;
                     }
                 }
+                // Tensor-product factorisation: w = Nx[a] (Ny[b] Nz[c]) and
+                // grad w = (dNx[a] yz, Nx[a] dNy[b] Nz[c], Nx[a] Ny[b] dNz[c]) / h,
+                // so the (b, c) sums are formed first for each a.
+                float yz[W][W], dyz[W][W], ydz[W][W];
 #pragma unroll
-                for (int a = 0; a < W; ++a)
+                for (int b = 0; b < W; ++b)
+#pragma unroll
+                    for (int c = 0; c < W; ++c) {
+                        yz[b][c] = Ny[b] * Nz[c];
+                        dyz[b][c] = dNy[b] * Nz[c];
+                        ydz[b][c] = Ny[b] * dNz[c];
+                    }
+#pragma unroll
+                for (int a = 0; a < W; ++a) {
+                    float S0[3] = {0.0f, 0.0f, 0.0f}, Sy[3] = {0.0f, 0.0f, 0.0f}, Sz[3] = {0.0f, 0.0f, 0.0f};
+                    float Sdy[3] = {0.0f, 0.0f, 0.0f}, Sdz[3] = {0.0f, 0.0f, 0.0f}, SF[3] = {0.0f, 0.0f, 0.0f};
 #pragma unroll
                     for (int b = 0; b < W; ++b)
 #pragma unroll
                         for (int c = 0; c < W; ++c) {
-                            const float w = (Nx[a] * Ny[b]) * Nz[c];
-                            const float gx = (dNx[a] * Ny[b]) * Nz[c];
-                            const float gy = (Nx[a] * dNy[b]) * Nz[c];
-                            const float gz = (Nx[a] * Ny[b]) * dNz[c];
                             const int q = ((lz + c) * T + (ly + b)) * T + (lx + a);
-                            const float w0 = tile[q], w1 = tile[TN + q], w2 = tile[2 * TN + q];
-                            const float vi0 = w0 * w, vi1 = w1 * w, vi2 = w2 * w;
-                            vpic0 += vi0; vpic1 += vi1; vpic2 += vi2;
-                            if (!APIC) {
-                                vf0 += (w0 - tile[3 * TN + q]) * w;
-                                vf1 += (w1 - tile[4 * TN + q]) * w;
-                                vf2 += (w2 - tile[5 * TN + q]) * w;
-                            }
-                            grad[0][0] += w0 * gx; grad[0][1] += w0 * gy; grad[0][2] += w0 * gz;
-                            grad[1][0] += w1 * gx; grad[1][1] += w1 * gy; grad[1][2] += w1 * gz;
-                            grad[2][0] += w2 * gx; grad[2][1] += w2 * gy; grad[2][2] += w2 * gz;
-                            if (APIC) {
-                                const float d0 = dxn[0][a], d1 = dxn[1][b], d2 = dxn[2][c];
-                                Bm[0][0] += vi0 * d0; Bm[0][1] += vi0 * d1; Bm[0][2] += vi0 * d2;
-                                Bm[1][0] += vi1 * d0; Bm[1][1] += vi1 * d1; Bm[1][2] += vi1 * d2;
-                                Bm[2][0] += vi2 * d0; Bm[2][1] += vi2 * d1; Bm[2][2] += vi2 * d2;
+                            const float wv[3] = {tile[q], tile[TN + q], tile[2 * TN + q]};
+#pragma unroll
+                            for (int k = 0; k < 3; ++k) {
+                                S0[k] += yz[b][c] * wv[k];
+                                Sy[k] += dyz[b][c] * wv[k];
+                                Sz[k] += ydz[b][c] * wv[k];
+                                if (APIC) {
+                                    Sdy[k] += (yz[b][c] * dxn[1][b]) * wv[k];
+                                    Sdz[k] += (yz[b][c] * dxn[2][c]) * wv[k];
+                                } else {
+                                    SF[k] += yz[b][c] * (wv[k] - tile[(3 + k) * TN + q]);
+                                }
                             }
                         }
+                    const float xa = Nx[a], dxa = dNx[a];
+                    vpic0 += xa * S0[0]; vpic1 += xa * S0[1]; vpic2 += xa * S0[2];
+                    if (!APIC) { vf0 += xa * SF[0]; vf1 += xa * SF[1]; vf2 += xa * SF[2]; }
+#pragma unroll
+                    for (int k = 0; k < 3; ++k) {
+                        grad[k][0] += dxa * S0[k];
+                        grad[k][1] += xa * Sy[k];
+                        grad[k][2] += xa * Sz[k];
+                        if (APIC) {
+                            Bm[k][0] += (xa * dxn[0][a]) * S0[k];
+                            Bm[k][1] += xa * Sdy[k];
+                            Bm[k][2] += xa * Sdz[k];
+                        }
+                    }
+                }
             }
             float n0, n1, n2;
             if (APIC) {
@@ -561,7 +656,7 @@ __global__ void __launch_bounds__(kG2PWG) k_g2p(SceneDev S, ParticlesDev P, Item
 #pragma unroll
             for (int k = 0; k < 6; ++k) P.tau[(int64_t)k * N + p] = tau[k];
             if (APIC) store_m3(P.B, N, p, Bm);
-            if (pcollide) boundary_collision(S, x0 + S.dt * n0, x1 + S.dt * n1, x2 + S.dt * n2, n0, n1, n2);
+            if (BATCH ? S.particle_collision : pcollide) boundary_collision(S, x0 + S.dt * n0, x1 + S.dt * n1, x2 + S.dt * n2, n0, n1, n2);
             P.v[p] = n0;
             P.v[N + p] = n1;
             P.v[2 * N + p] = n2;
@@ -571,42 +666,60 @@ __global__ void __launch_bounds__(kG2PWG) k_g2p(SceneDev S, ParticlesDev P, Item
                 P.x[N + p] = y1;
                 P.x[2 * N + p] = y2;
                 if (next_keys)
-                    next_keys[p] = cell_key(S, y0, y1, y2, Stencil<INTERP>::SHIFT, Stencil<INTERP>::W);
+                    next_keys[pbase + p] = koff + cell_key(S, y0, y1, y2, Stencil<INTERP>::SHIFT, Stencil<INTERP>::W);
             }
         }
         __syncthreads();
     }
 }
 
-template <int INTERP, int APIC>
-static void g2p_dispatch_src(const SceneDev& S, const ParticlesDev& P, const ItemsDev& I, int src, int advect,
-                             int pcollide, int wgs, hipStream_t st, uint32_t* nk) {
-    if (src == 0)
-        hipLaunchKernelGGL((k_g2p<INTERP, APIC, 0>), dim3(wgs), dim3(kG2PWG), 0, st, S, P, I, advect, pcollide, nk);
-    else if (src == 1)
-        hipLaunchKernelGGL((k_g2p<INTERP, APIC, 1>), dim3(wgs), dim3(kG2PWG), 0, st, S, P, I, advect, pcollide, nk);
+template <int INTERP, int APIC, int SRC, bool BATCH>
+static void g2p_dispatch_wg(const SceneDev& S, const ParticlesDev& P, const ItemsDev& I, const SceneEntry* tab,
+                            int advect, int pcollide, int wgs, int wg, hipStream_t st, uint32_t* nk) {
+    if (wg == 64)
+        hipLaunchKernelGGL((k_g2p<INTERP, APIC, SRC, BATCH, 64>), dim3(wgs), dim3(64), 0, st, S, P, I, tab, advect,
+                           pcollide, nk);
+    else if (wg == 256)
+        hipLaunchKernelGGL((k_g2p<INTERP, APIC, SRC, BATCH, 256>), dim3(wgs), dim3(256), 0, st, S, P, I, tab,
+                           advect, pcollide, nk);
     else
-        hipLaunchKernelGGL((k_g2p<INTERP, APIC, 2>), dim3(wgs), dim3(kG2PWG), 0, st, S, P, I, advect, pcollide, nk);
+        hipLaunchKernelGGL((k_g2p<INTERP, APIC, SRC, BATCH, kG2PWG>), dim3(wgs), dim3(kG2PWG), 0, st, S, P, I, tab,
+                           advect, pcollide, nk);
+}
+template <int INTERP, int APIC>
+static void g2p_dispatch_src(const SceneDev& S, const ParticlesDev& P, const ItemsDev& I, const SceneEntry* tab,
+                             int src, int advect, int pcollide, int wgs, int wg, hipStream_t st, uint32_t* nk) {
+    if (tab) g2p_dispatch_wg<INTERP, APIC, 0, true>(S, P, I, tab, advect, pcollide, wgs, wg, st, nk);
+    else if (src == 0) g2p_dispatch_wg<INTERP, APIC, 0, false>(S, P, I, tab, advect, pcollide, wgs, wg, st, nk);
+    else if (src == 1)
+        hipLaunchKernelGGL((k_g2p<INTERP, APIC, 1, false, kG2PWG>), dim3(wgs), dim3(kG2PWG), 0, st, S, P, I, tab,
+                           advect, pcollide, nk);
+    else
+        hipLaunchKernelGGL((k_g2p<INTERP, APIC, 2, false, kG2PWG>), dim3(wgs), dim3(kG2PWG), 0, st, S, P, I, tab,
+                           advect, pcollide, nk);
 }
 template <int INTERP>
-static void g2p_dispatch_apic(const SceneDev& S, const ParticlesDev& P, const ItemsDev& I, int apic, int src,
-                              int advect, int pcollide, int wgs, hipStream_t st, uint32_t* nk) {
-    if (apic) g2p_dispatch_src<INTERP, 1>(S, P, I, src, advect, pcollide, wgs, st, nk);
-    else g2p_dispatch_src<INTERP, 0>(S, P, I, src, advect, pcollide, wgs, st, nk);
+static void g2p_dispatch_apic(const SceneDev& S, const ParticlesDev& P, const ItemsDev& I, const SceneEntry* tab,
+                              int apic, int src, int advect, int pcollide, int wgs, int wg, hipStream_t st,
+                              uint32_t* nk) {
+    if (apic) g2p_dispatch_src<INTERP, 1>(S, P, I, tab, src, advect, pcollide, wgs, wg, st, nk);
+    else g2p_dispatch_src<INTERP, 0>(S, P, I, tab, src, advect, pcollide, wgs, wg, st, nk);
 }
 void launch_g2p(const SceneDev& S, const ParticlesDev& P, const ItemsDev& I, int interp, int apic, int src, int advect,
-                int pcollide, int grid_wgs, hipStream_t st, uint32_t* next_keys) {
-    if (P.n <= 0) return;
+                int pcollide, int grid_wgs, hipStream_t st, uint32_t* next_keys, const SceneEntry* tab, int wg) {
+    if (I.n <= 0) return;
     switch (interp) {
         case MPM_INTERP_LINEAR:
-            g2p_dispatch_apic<MPM_INTERP_LINEAR>(S, P, I, apic, src, advect, pcollide, grid_wgs, st, next_keys);
+            g2p_dispatch_apic<MPM_INTERP_LINEAR>(S, P, I, tab, apic, src, advect, pcollide, grid_wgs, wg, st,
+                                                 next_keys);
             break;
         case MPM_INTERP_QUADRATIC_BSPLINE:
-            g2p_dispatch_apic<MPM_INTERP_QUADRATIC_BSPLINE>(S, P, I, apic, src, advect, pcollide, grid_wgs, st,
-                                                            next_keys);
+            g2p_dispatch_apic<MPM_INTERP_QUADRATIC_BSPLINE>(S, P, I, tab, apic, src, advect, pcollide, grid_wgs, wg,
+                                                            st, next_keys);
             break;
         default:
-            g2p_dispatch_apic<MPM_INTERP_CUBIC_BSPLINE>(S, P, I, apic, src, advect, pcollide, grid_wgs, st, next_keys);
+            g2p_dispatch_apic<MPM_INTERP_CUBIC_BSPLINE>(S, P, I, tab, apic, src, advect, pcollide, grid_wgs, wg, st,
+                                                        next_keys);
             break;
     }
 }
@@ -634,8 +747,8 @@ void launch_stress(const ParticlesDev& P, hipStream_t st) {
 }
 
 // ------------------------------------------------------------ grid passes
-// One wave per touched storage block (64 nodes, one per lane); the count of
-// nodes with mass > 0 is reduced per workgroup and added with one atomic.
+// Per-workgroup reduction of a wave-uniform count (lane 0 of each wave
+// contributes), added with one atomic.
 __device__ __forceinline__ void wg_count_add(unsigned long long* dst, unsigned long long mine) {
     __shared__ unsigned long long part[4];
     const int lane = threadIdx.x & 63, wave = threadIdx.x >> 6;
@@ -648,31 +761,78 @@ __device__ __forceinline__ void wg_count_add(unsigned long long* dst, unsigned l
     }
 }
 
+// Grid clear: one lane per storage block reads the touched flags
+// (coalesced); the wave then clears the touched blocks it found, one block
+// (64 nodes, one per lane) at a time, and counts their nodes with mass > 0
+// (the active nodes of the step that wrote them).
+__device__ __forceinline__ unsigned long long clear_block(float* grid, int64_t plane, uint32_t* touched, int64_t b,
+                                                          int nfields, int lane) {
+    const int64_t ni = b * 64 + lane;
+    const unsigned long long c = (unsigned long long)__popcll(__ballot(grid[ni] > 0.0f));
+    for (int f = 0; f < nfields; ++f) grid[(int64_t)f * plane + ni] = 0.0f;
+    if (lane == 0) touched[b] = 0u;
+    return c;
+}
+
 __global__ void __launch_bounds__(256) k_clear(SceneDev S, int64_t nsb, int nfields, unsigned long long* active) {
     const int lane = threadIdx.x & 63;
-    const int64_t wave = ((int64_t)blockIdx.x * blockDim.x + threadIdx.x) >> 6;
-    const int64_t nwaves = ((int64_t)gridDim.x * blockDim.x) >> 6;
+    const int64_t b = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+    uint64_t todo = __ballot(b < nsb && S.touched[b] != 0u);
+    const int64_t wave0 = b - lane;
     unsigned long long cnt = 0;
-    for (int64_t b = wave; b < nsb; b += nwaves) {
-        if (S.touched[b] == 0u) continue;
-        const int64_t ni = b * 64 + lane;
-        const float m = S.grid[ni];
-        cnt += (unsigned long long)__popcll(__ballot(m > 0.0f));
-        for (int f = 0; f < nfields; ++f) S.grid[(int64_t)f * S.plane + ni] = 0.0f;
-        if (lane == 0) S.touched[b] = 0u;
+    while (todo) {
+        const int l = __builtin_ctzll(todo);
+        todo &= todo - 1;
+        cnt += clear_block(S.grid, S.plane, S.touched, wave0 + l, nfields, lane);
     }
-    if (active) wg_count_add(active, cnt);
+    if (active) wg_count_add(active, lane == 0 ? cnt : 0ull);
 }
+
 __global__ void __launch_bounds__(256) k_count_active(SceneDev S, int64_t nsb, unsigned long long* active) {
     const int lane = threadIdx.x & 63;
-    const int64_t wave = ((int64_t)blockIdx.x * blockDim.x + threadIdx.x) >> 6;
-    const int64_t nwaves = ((int64_t)gridDim.x * blockDim.x) >> 6;
+    const int64_t b = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+    uint64_t todo = __ballot(b < nsb && S.touched[b] != 0u);
+    const int64_t wave0 = b - lane;
     unsigned long long cnt = 0;
-    for (int64_t b = wave; b < nsb; b += nwaves) {
-        if (S.touched[b] == 0u) continue;
-        cnt += (unsigned long long)__popcll(__ballot(S.grid[b * 64 + lane] > 0.0f));
+    while (todo) {
+        const int l = __builtin_ctzll(todo);
+        todo &= todo - 1;
+        cnt += (unsigned long long)__popcll(__ballot(S.grid[(wave0 + l) * 64 + lane] > 0.0f));
     }
-    wg_count_add(active, cnt);
+    wg_count_add(active, lane == 0 ? cnt : 0ull);
+}
+
+// Batched clear over the storage blocks of every scene: global block index
+// gb lies in scene s when block_start[s] <= gb < block_start[s + 1].  Each
+// scene's active-node count goes to its own counter.
+__global__ void __launch_bounds__(256) k_clear_batch(const SceneEntry* __restrict__ tab,
+                                                     const int64_t* __restrict__ block_start, int nscenes,
+                                                     int nfields) {
+    const int lane = threadIdx.x & 63;
+    const int64_t gb = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+    const int64_t total = block_start[nscenes];
+    int sc = 0;
+    bool t = false;
+    if (gb < total) {
+        int lo = 0, hi = nscenes - 1;
+        while (lo < hi) {
+            const int mid = (lo + hi + 1) >> 1;
+            if (block_start[mid] <= gb) lo = mid;
+            else hi = mid - 1;
+        }
+        sc = lo;
+        t = tab[sc].S.touched[gb - block_start[sc]] != 0u;
+    }
+    uint64_t todo = __ballot(t);
+    while (todo) {
+        const int l = __builtin_ctzll(todo);
+        todo &= todo - 1;
+        const int s = __builtin_amdgcn_readfirstlane(__shfl(sc, l, 64));
+        const int64_t b = (gb - lane + l) - block_start[s];
+        const unsigned long long c =
+            clear_block(tab[s].S.grid, tab[s].S.plane, tab[s].S.touched, b, nfields, lane);
+        if (lane == 0 && c) atomicAdd(tab[s].active, c);
+    }
 }
 
 static unsigned grid_pass_wgs(int64_t nsb) {
@@ -683,10 +843,18 @@ static unsigned grid_pass_wgs(int64_t nsb) {
 }
 
 void launch_clear(const SceneDev& S, int64_t nsb, int nfields, unsigned long long* active, hipStream_t st) {
-    hipLaunchKernelGGL(k_clear, dim3(grid_pass_wgs(nsb)), dim3(256), 0, st, S, nsb, nfields, active);
+    if (nsb <= 0) return;
+    hipLaunchKernelGGL(k_clear, dim3((unsigned)((nsb + 255) / 256)), dim3(256), 0, st, S, nsb, nfields, active);
+}
+void launch_clear_batch(const SceneEntry* tab, const int64_t* block_start, int nscenes, int64_t total_blocks,
+                        int nfields, hipStream_t st) {
+    if (total_blocks <= 0 || nscenes <= 0) return;
+    hipLaunchKernelGGL(k_clear_batch, dim3((unsigned)((total_blocks + 255) / 256)), dim3(256), 0, st, tab,
+                       block_start, nscenes, nfields);
 }
 void launch_count_active(const SceneDev& S, int64_t nsb, unsigned long long* active, hipStream_t st) {
-    hipLaunchKernelGGL(k_count_active, dim3(grid_pass_wgs(nsb)), dim3(256), 0, st, S, nsb, active);
+    if (nsb <= 0) return;
+    hipLaunchKernelGGL(k_count_active, dim3((unsigned)((nsb + 255) / 256)), dim3(256), 0, st, S, nsb, active);
 }
 
 // Unfused grid sub-steps over touched blocks (untouched nodes hold zero

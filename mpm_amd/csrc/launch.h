@@ -33,23 +33,37 @@ int radix_sort_pairs(const uint32_t* kin, const uint32_t* vin, uint32_t* ka, uin
 // Persistent launch width of the item kernels.
 constexpr int kItemMax = 512;   // particles per work item (max)
 
-void launch_keys(const SceneDev& S, const ParticlesDev& P, int interp, uint32_t* keys, hipStream_t st);
+// key_offset is added to every key (the scene's offset in a batch).
+void launch_keys(const SceneDev& S, const ParticlesDev& P, int interp, uint32_t* keys, hipStream_t st,
+                 uint32_t key_offset = 0);
 void launch_iota(uint32_t* p, int64_t n, hipStream_t st);
+// Work items over sorted keys.  item_max = 0 selects kItemMax.  For a batch,
+// key_offsets[nscenes] (ascending) gives each item's scene in item_scene.
 void launch_items(const uint32_t* sorted_keys, int64_t n, uint32_t* flags, uint32_t* scan_temp, uint32_t* items,
-                  uint32_t* item_count, hipStream_t st);
+                  uint32_t* item_count, hipStream_t st, uint32_t item_max = 0, const uint32_t* key_offsets = nullptr,
+                  int nscenes = 0, uint32_t* item_scene = nullptr);
 
-// mode: 0 = full P2G (mass, momentum, force), 1 = mass only (initialize)
+// mode: 0 = full P2G (mass, momentum, force), 1 = mass only (initialize).
+// With a scene table (tab != null, I.scene set) every item reads its scene
+// from the table (mode 0 only); S and P are then unused.
 void launch_p2g(const SceneDev& S, const ParticlesDev& P, const ItemsDev& I, int interp, int apic, int mode,
-                int grid_wgs, hipStream_t st);
+                int grid_wgs, hipStream_t st, const SceneEntry* tab = nullptr);
 // src: 0 = fused grid update from P2G fields, 1 = materialised v / v*,
 //      2 = initialize() volume gather
 // next_keys (may be null): with advect, the cell key of each particle's new
-// position is written at its storage slot (input of the next sort).
+// position is written at its storage slot (input of the next sort); in a
+// batch at the scene's global slot, plus its key offset.
+// wg: threads per workgroup of the fused G2P (64, 128 or 256).
 void launch_g2p(const SceneDev& S, const ParticlesDev& P, const ItemsDev& I, int interp, int apic, int src,
-                int advect, int pcollide, int grid_wgs, hipStream_t st, uint32_t* next_keys = nullptr);
+                int advect, int pcollide, int grid_wgs, hipStream_t st, uint32_t* next_keys = nullptr,
+                const SceneEntry* tab = nullptr, int wg = 128);
 
 void launch_clear(const SceneDev& S, int64_t storage_blocks, int nfields, unsigned long long* active_nodes,
                   hipStream_t st);
+// Clear of every scene's touched blocks; block_start[nscenes + 1] is the
+// prefix sum of the scenes' storage block counts (on the device).
+void launch_clear_batch(const SceneEntry* tab, const int64_t* block_start, int nscenes, int64_t total_blocks,
+                        int nfields, hipStream_t st);
 void launch_count_active(const SceneDev& S, int64_t storage_blocks, unsigned long long* active_nodes,
                          hipStream_t st);
 // op: 0 gravity, 1 update velocities, 2 grid collision (touched blocks only)

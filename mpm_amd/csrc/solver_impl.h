@@ -1,0 +1,111 @@
+// solver_impl.h -- host-side state of one scene handle, shared by the
+// C-ABI translation units (capi.hip: one scene; batch.hip: many scenes in
+// one launch).  Not part of the public interface.
+#pragma once
+
+#include <hip/hip_runtime.h>
+
+#include <cstdint>
+#include <map>
+#include <string>
+#include <utility>
+#include <vector>
+
+#include "launch.h"
+
+namespace mpm {
+namespace host {
+
+
+inline thread_local std::string g_err;
+
+inline int fail(int code, const std::string& msg) {
+    g_err = msg;
+    return code;
+}
+
+#define HIP_TRY(call)                                                                                 \
+    do {                                                                                              \
+        hipError_t e_ = (call);                                                                       \
+        if (e_ != hipSuccess) {                                                                       \
+            return fail(MPM_ERR_HIP, std::string(#call) + ": " + hipGetErrorString(e_) + " at " +     \
+                                         __FILE__ + ":" + std::to_string(__LINE__));                  \
+        }                                                                                             \
+    } while (0)
+
+enum GridPhase { PH_CLEAR = 0, PH_MASS = 1, PH_RAW = 2, PH_SUBSTEP = 3 };
+
+struct Pending {
+    std::string name;
+    hipEvent_t a, b;
+};
+
+struct Solver {
+    mpm_desc desc{};
+    int interp = 0, apic = 0;
+    int64_t N = 0;
+    int device = 0;
+    hipStream_t st = nullptr;
+    SceneDev S{};
+    ParticlesDev P{};
+    ItemsDev I{};
+    std::vector<MaterialDev> mats;
+    MaterialDev* mats_dev = nullptr;
+    int64_t nsb = 0;
+    int key_bits = 1;
+    uint32_t* kin = nullptr;   // sort input keys (slot order); written by k_keys or the fused G2P
+    uint32_t* iota = nullptr;  // sort input values: slot index
+    uint32_t *ka = nullptr, *va = nullptr, *kb = nullptr, *vb = nullptr;  // ping-pong / result
+    bool keys_valid = false;   // kin matches the current positions
+    uint32_t* hist = nullptr;
+    uint32_t* scan_temp = nullptr;
+    uint32_t* flags = nullptr;
+    uint32_t* items = nullptr;
+    uint32_t* counters = nullptr;                 // [0] item count
+    unsigned long long* acc = nullptr;            // [0] active sum, [1] scratch, [2] escaped
+    bool sort_valid = false, grid_dirty = false, vstar_dirty = false, count_on_clear = false;
+    bool stress_valid = false;  // P.tau matches the particle state
+    int phase = PH_CLEAR;
+    int64_t steps = 0;
+    int item_wgs = 1024;
+    double last_step_ms = 0.0;
+    std::vector<void*> allocs;
+    int64_t device_bytes = 0;
+    // profiling
+    bool prof = false;
+    std::vector<Pending> pending;
+    std::vector<hipEvent_t> event_pool;
+    std::map<std::string, std::pair<int64_t, double>> prof_tot;
+    std::vector<std::string> prof_order;
+    // graph
+    hipGraph_t graph = nullptr;
+    hipGraphExec_t gexec = nullptr;
+    // batch membership: while in a batch the solver works on the batch's
+    // stream (own_st keeps its own)
+    struct Batch* batch = nullptr;
+    hipStream_t own_st = nullptr;
+    // incremented whenever particle positions change (steps, advection,
+    // state writes): a batch recomputes the member's sort keys on a change
+    uint64_t version = 0;
+};
+
+
+int dalloc(Solver& s, void** p, size_t bytes);
+void dfree(Solver& s, void* p);
+ProfileHook hook(Solver& s);
+void prof_collect(Solver& s);
+struct Scope {
+    const ProfileHook& h;
+    const char* n;
+    Scope(const ProfileHook& h_, const char* n_) : h(h_), n(n_) { h.begin(n); }
+    ~Scope() { h.end(n); }
+};
+void run_clear(Solver& s);
+void ensure_stress(Solver& s);
+void destroy(Solver* s);
+int finish(Solver& s);
+// batch.hip: removes a scene that is destroyed while a batch member.
+void batch_forget(struct Batch* b, Solver* s);
+
+}  // namespace host
+}  // namespace mpm

@@ -108,6 +108,30 @@ class MpmKernelTime(C.Structure):
     _fields_ = [("name", C.c_char * 32), ("launches", C.c_int64), ("total_ms", C.c_double)]
 
 
+class MpmTune(C.Structure):
+    _fields_ = [
+        ("item_max", C.c_int32),
+        ("g2p_threads", C.c_int32),
+        ("workgroups", C.c_int32),
+        ("bucket", C.c_int32),
+        ("step_ms", C.c_double),
+    ]
+
+
+class MpmBatchStats(C.Structure):
+    _fields_ = [
+        ("scenes", C.c_int64),
+        ("particles", C.c_int64),
+        ("steps", C.c_int64),
+        ("storage_blocks", C.c_int64),
+        ("key_bits", C.c_int64),
+        ("work_items", C.c_int64),
+        ("device_bytes", C.c_int64),
+        ("last_step_ms", C.c_double),
+        ("tune", MpmTune),
+    ]
+
+
 # Every exported entry point and its signature (also used by the CPU test
 # that checks the library exports what the header declares).
 SIGNATURES = {
@@ -144,6 +168,19 @@ SIGNATURES = {
     "mpm_profile_reset": (C.c_int, [C.c_void_p]),
     "mpm_profile_read": (C.c_int, [C.c_void_p, C.POINTER(MpmKernelTime), C.c_int, C.POINTER(C.c_int)]),
     "mpm_debug_svd3": (C.c_int, [_fp, C.c_int64, _fp, _fp, _fp]),
+    "mpm_batch_create": (C.c_int, [C.c_int32, C.c_int32, C.POINTER(C.c_void_p)]),
+    "mpm_batch_destroy": (None, [C.c_void_p]),
+    "mpm_batch_add": (C.c_int, [C.c_void_p, C.c_void_p]),
+    "mpm_batch_remove": (C.c_int, [C.c_void_p, C.c_void_p]),
+    "mpm_batch_info": (C.c_int, [C.c_void_p, C.POINTER(MpmBatchStats)]),
+    "mpm_batch_step": (C.c_int, [C.c_void_p, C.c_int]),
+    "mpm_batch_step_graph": (C.c_int, [C.c_void_p, C.c_int]),
+    "mpm_batch_synchronize": (C.c_int, [C.c_void_p]),
+    "mpm_batch_set_tune": (C.c_int, [C.c_void_p, C.POINTER(MpmTune)]),
+    "mpm_batch_autotune": (C.c_int, [C.c_void_p, C.c_int, C.POINTER(MpmTune)]),
+    "mpm_tune_lookup": (C.c_int, [C.c_int32, C.c_int32, C.c_int64, C.POINTER(MpmTune)]),
+    "mpm_batch_timing": (C.c_int, [C.c_void_p, C.c_int]),
+    "mpm_batch_read_times": (C.c_int, [C.c_void_p, _fp, C.c_int, C.POINTER(C.c_int)]),
 }
 
 _lib = None
