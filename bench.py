@@ -74,6 +74,61 @@ def cpu_baseline(scene, seconds_target=15.0):
     }
 
 
+def _pmc_pass(counter, args, outdir, tag):
+    """One rocprofv3 counter pass over a short run of this script (child
+    process); returns {kernel name: mean counter value per dispatch}."""
+    import csv
+    import shutil
+    import subprocess
+
+    prof = shutil.which("rocprofv3")
+    if prof is None:
+        return None
+    d = os.path.join(outdir, f"{tag}_{counter}")
+    cmd = [prof, "--pmc", counter, "--output-format", "csv", "-d", d, "-o", tag, "--",
+           sys.executable, os.path.abspath(__file__)] + args
+    env = dict(os.environ, TMPDIR=os.environ.get("TMPDIR", "/tmp"))
+    try:
+        subprocess.run(cmd, stdout=subprocess.DEVNULL, stderr=subprocess.DEVNULL, timeout=240, env=env, check=False)
+    except subprocess.TimeoutExpired:
+        return None
+    sums, counts = {}, {}
+    for root, _, files in os.walk(d):
+        for f in files:
+            if f.endswith("counter_collection.csv"):
+                for r in csv.DictReader(open(os.path.join(root, f))):
+                    k = r["Kernel_Name"]
+                    sums[k] = sums.get(k, 0.0) + float(r["Counter_Value"])
+                    counts[k] = counts.get(k, 0) + 1
+    return {k: sums[k] / counts[k] for k in sums} or None
+
+
+def pmc_traffic(workload, seed_args, outdir):
+    """HBM bytes per launch of P2G and G2P: FETCH_SIZE and WRITE_SIZE (KiB per
+    dispatch) of a 3-step run, each scaled by what the same counter reads
+    for a plain 1 GiB device copy."""
+    run = ["--workload", workload, "--steps", "3", "--warmup", "1", "--no-cpu-baseline", "--batch-scenes", "0",
+           "--no-traffic"] + seed_args
+    copy = ["--calibrate-copy", "1024"]
+    fe, wr = _pmc_pass("FETCH_SIZE", run, outdir, "step"), _pmc_pass("WRITE_SIZE", run, outdir, "step")
+    cfe, cwr = _pmc_pass("FETCH_SIZE", copy, outdir, "copy"), _pmc_pass("WRITE_SIZE", copy, outdir, "copy")
+    if not (fe and wr and cfe and cwr):
+        return None
+    ck = [k for k in cfe if "copyBuffer" in k]
+    if not ck or ck[0] not in cwr:
+        return None
+    kib = 1024 * 1024  # 1 GiB each way
+    sf, sw = kib / cfe[ck[0]], kib / cwr[ck[0]]
+    out = {"fetch_scale": sf, "write_scale": sw}
+    for name, key in (("p2g", "k_p2g<"), ("g2p", "k_g2p<")):
+        ks = [k for k in fe if key in k and ", 0, false" in k and k in wr]
+        if ks:
+            k = max(ks, key=lambda q: fe[q])
+            out[name] = {"kernel": k.split("(")[0], "fetch_kib": fe[k], "write_kib": wr[k],
+                         "bytes": (fe[k] * sf + wr[k] * sw) * 1024.0}
+    return out
+
+
 def batch_bench(count, steps, warmup, seed=3):
     """Uneven scene batch: the same scenes stepped one at a time (the plain
     simulateOneStep loop) and as one batch (eager launches and hipGraph
@@ -190,8 +245,10 @@ def main(argv=None):
     ap.add_argument("--seed", type=int, default=None)
     ap.add_argument("--batch-scenes", type=int, default=256, help="scenes of the batched run (0: skip)")
     ap.add_argument("--batch-steps", type=int, default=20)
-    ap.add_argument("--serve-seconds", type=float, default=0.0,
+    ap.add_argument("--serve-seconds", type=float, default=10.0,
                     help="arrival window of the served request stream (0: skip)")
+    ap.add_argument("--no-traffic", action="store_true",
+                    help="skip the rocprofv3 FETCH_SIZE/WRITE_SIZE passes that fill roofline.traffic")
     ap.add_argument("--calibrate-copy", type=int, default=0, metavar="MIB",
                     help="only run a plain device copy of MIB MiB (HBM counter calibration) and exit")
     a = ap.parse_args(argv)
@@ -215,6 +272,22 @@ def main(argv=None):
 
     from mpm_amd import scenes
     from mpm_amd.solver import Solver
+
+    # counter passes first, in child processes, before this process uses the GPU
+    traffic = None
+    if not a.no_traffic:
+        t0 = time.time()
+        import shutil
+        import tempfile
+
+        outdir = tempfile.mkdtemp(prefix="mpm_bench_pmc_")
+        try:
+            traffic = pmc_traffic(a.workload, [] if a.seed is None else ["--seed", str(a.seed)], outdir)
+        finally:
+            shutil.rmtree(outdir, ignore_errors=True)
+        log(f"pmc traffic passes in {time.time() - t0:.1f}s: "
+            + (", ".join(f"{k}={v['bytes'] / 1e9:.3f} GB" for k, v in traffic.items() if isinstance(v, dict))
+               if traffic else "unavailable"))
 
     kw = {} if a.seed is None else {"seed": a.seed}
     t0 = time.time()
@@ -266,9 +339,11 @@ def main(argv=None):
         mean_s = kernels[dom]["mean_ms"] * 1e-3
         if dom in bp:
             alg = n * bp[dom] + act * GRID_BYTES[dom]
+            tr = traffic.get(dom) if traffic else None
             roof = {"bound": "hbm", "kernel": dom, "achieved": alg / mean_s / 1e9, "peak": HBM_PEAK / 1e9,
-                    "unit": "GB/s", "frac": alg / mean_s / HBM_PEAK, "traffic": None,
-                    "algorithmic_bytes": alg, "mean_ms": kernels[dom]["mean_ms"]}
+                    "unit": "GB/s", "frac": alg / mean_s / HBM_PEAK, "traffic": tr["bytes"] if tr else None,
+                    "algorithmic_bytes": alg, "mean_ms": kernels[dom]["mean_ms"],
+                    "traffic_detail": traffic}
         else:
             roof = {"bound": "hbm", "kernel": dom, "achieved": None, "peak": HBM_PEAK / 1e9, "unit": "GB/s",
                     "frac": None, "traffic": None, "mean_ms": kernels[dom]["mean_ms"]}

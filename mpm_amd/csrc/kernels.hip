@@ -5,9 +5,10 @@
 // of one block.  P2G and G2P are persistent kernels: each workgroup walks
 // the item list and stages the block's node tile (the block plus the
 // stencil halo, T = 4 + W - 1 nodes per axis) in LDS:
-//   P2G: particles scatter mass, momentum and force into the LDS tile with
-//        ds_add_f32; the tile is flushed to HBM with one global float atomic
-//        per non-zero node value.
+//   P2G: particles scatter mass, momentum and force into the LDS tile (cell
+//        sums over a wave-level segmented scan, one plain LDS update per
+//        cell); the tile is flushed to HBM with one global float atomic per
+//        non-zero node value.
 //   G2P: the tile is loaded once per item; in the fused step the grid
 //        velocity update (gravity, p/m, v* = v + dt f/m, boundary collision)
 //        is computed while loading, so there is no separate grid pass.
@@ -19,6 +20,25 @@
 namespace mpm {
 
 constexpr int kG2PWG = 128;
+
+// Occupancy targets (waves per SIMD) of the item kernels; 0 leaves the
+// register allocation to the compiler.
+#ifndef MPM_P2G_WAVES
+#define MPM_P2G_WAVES 0
+#endif
+#ifndef MPM_G2P_WAVES
+#define MPM_G2P_WAVES 0
+#endif
+#if MPM_P2G_WAVES > 0
+#define MPM_P2G_OCC __attribute__((amdgpu_waves_per_eu(MPM_P2G_WAVES)))
+#else
+#define MPM_P2G_OCC
+#endif
+#if MPM_G2P_WAVES > 0
+#define MPM_G2P_OCC __attribute__((amdgpu_waves_per_eu(MPM_G2P_WAVES)))
+#else
+#define MPM_G2P_OCC
+#endif
 
 // ------------------------------------------------------------------ keys
 // Cell key of each particle, plus the scene's key offset in a batch.
@@ -256,49 +276,94 @@ __device__ __forceinline__ void store_m3(float* base, int64_t N, uint32_t p, con
 //
 // One wave per work item.  Lanes hold consecutive sorted particles, so the
 // particles of one cell sit in consecutive lanes and share all W^3 target
-// nodes.  Each contribution is first summed over the lanes of its cell
-// (segmented reduction with DPP wave shifts), then only the first lane of
-// each cell adds it to the LDS tile with a plain read-add-write: within one
-// instruction the writing lanes belong to different cells, so they hit
-// different nodes, and the single wave orders successive writes.  This
-// avoids LDS float atomics entirely.
+// nodes.  Each contribution is first summed over the lanes of its cell (a
+// segmented scan in DPP, below), then only the last lane of each cell adds
+// it to the LDS tile with a plain read-add-write: within one instruction the
+// writing lanes belong to different cells, so they hit different nodes, and
+// the single wave orders successive writes.  This avoids LDS float atomics.
 
-// Lane i receives lane i+1's value (0 for lane 63): DPP wave_shl:1.
-__device__ __forceinline__ float wave_shl1(float v) {
-    return __int_as_float(__builtin_amdgcn_update_dpp(0, __float_as_int(v), 0x130, 0xF, 0xF, true));
+// Segmented inclusive scan toward the LAST lane of each segment, entirely
+// in DPP (VALU, no LDS traffic): Hillis-Steele within each 16-lane row with
+// row_shr:1,2,4,8, then row_bcast:15 (last lane of rows 0 and 2 into rows 1
+// and 3) and row_bcast:31 (lane 31 into rows 2 and 3), as in a wave64 scan.
+// Each step is one v_fmac_f32 with a DPP source per value,
+//   v += shifted(v) * flag,
+// where the per-lane flag is 1.0 when the source lane lies in the same
+// segment and 0.0 otherwise (bound_ctrl gives 0 for sources outside the
+// row; row_mask keeps the rows a broadcast does not target).  The compiler
+// does not fold a DPP move into a tied FMAC, so the step is written as
+// inline assembly; each block starts with s_nop 4, which covers the
+// VALU-write -> DPP-read (2) and VALU-exec-write -> DPP (5) wait states for
+// the values computed just before it.  Within a block the 7 values are
+// independent, so successive blocks need no further waits.  Steps that no
+// lane needs are skipped (uniform branches).
+struct SegPlan {
+    float f1, f2, f4, f8, f15, f31;  // per-lane multipliers of each step
+    bool s1, s2, s4, s8, s15, s31;   // uniform: step needed
+    bool writer;                     // last active lane of its segment
+};
+// headmask: lanes that start a segment (lane 0 always does).
+__device__ __forceinline__ SegPlan seg_plan(uint64_t headmask, bool active, int lane) {
+    SegPlan sp;
+    const uint64_t le = lane == 63 ? ~0ull : ((1ull << (lane + 1)) - 1ull);
+    const int start = 63 - __clzll(headmask & le);
+    const int rowstart = lane & ~15;
+    sp.f1 = start <= lane - 1 ? 1.0f : 0.0f;
+    sp.f2 = start <= lane - 2 ? 1.0f : 0.0f;
+    sp.f4 = start <= lane - 4 ? 1.0f : 0.0f;
+    sp.f8 = start <= lane - 8 ? 1.0f : 0.0f;
+    const bool c15 = ((lane >> 4) & 1) && start < rowstart;
+    const bool c31 = lane >= 32 && start <= 31;
+    sp.f15 = c15 ? 1.0f : 0.0f;
+    sp.f31 = c31 ? 1.0f : 0.0f;
+    const int run = lane - (start > rowstart ? start : rowstart);
+    sp.s1 = __ballot(run >= 1) != 0ull;
+    sp.s2 = __ballot(run >= 2) != 0ull;
+    sp.s4 = __ballot(run >= 4) != 0ull;
+    sp.s8 = __ballot(run >= 8) != 0ull;
+    sp.s15 = __ballot(c15) != 0ull;
+    sp.s31 = __ballot(c31) != 0ull;
+    sp.writer = active && (lane == 63 || ((headmask >> (lane + 1)) & 1ull));
+    return sp;
 }
 
-// Segmented sum toward the first lane of each segment (tree over offsets
-// 1, 2, 4, ...): after level j a lane holds the sum of the next 2^(j+1)
-// lanes of its segment.  mk[j] marks the lanes whose segment continues at
-// lane + 2^j; nsteps = number of non-empty levels (uniform).  Level 0 uses a
-// DPP wave shift fused into the add (VALU); deeper levels use ds_bpermute,
-// which runs on the LDS pipe and leaves the VALU free.
+#define MPM_FMAC_DPP(CTRL) "v_fmac_f32_dpp %0, %0, %1 " CTRL " bank_mask:0xf bound_ctrl:1"
+#define MPM_FMAC_DPP7(CTRL)                                                                    \
+    "s_nop 4\n\t"                                                                              \
+    "v_fmac_f32_dpp %0, %0, %7 " CTRL " bank_mask:0xf bound_ctrl:1\n\t"                       \
+    "v_fmac_f32_dpp %1, %1, %7 " CTRL " bank_mask:0xf bound_ctrl:1\n\t"                       \
+    "v_fmac_f32_dpp %2, %2, %7 " CTRL " bank_mask:0xf bound_ctrl:1\n\t"                       \
+    "v_fmac_f32_dpp %3, %3, %7 " CTRL " bank_mask:0xf bound_ctrl:1\n\t"                       \
+    "v_fmac_f32_dpp %4, %4, %7 " CTRL " bank_mask:0xf bound_ctrl:1\n\t"                       \
+    "v_fmac_f32_dpp %5, %5, %7 " CTRL " bank_mask:0xf bound_ctrl:1\n\t"                       \
+    "v_fmac_f32_dpp %6, %6, %7 " CTRL " bank_mask:0xf bound_ctrl:1"
+
+#define MPM_SCAN_STEP(NV, v, CTRL, flag)                                                          \
+    do {                                                                                        \
+        if constexpr (NV == 7) {                                                                \
+            asm volatile(MPM_FMAC_DPP7(CTRL)                                                    \
+                         : "+v"(v[0]), "+v"(v[1]), "+v"(v[2]), "+v"(v[3]), "+v"(v[4]), "+v"(v[5]), \
+                           "+v"(v[6])                                                           \
+                         : "v"(flag));                                                          \
+        } else {                                                                                \
+            for (int f_ = 0; f_ < NV; ++f_)                                                     \
+                asm volatile("s_nop 4\n\t" MPM_FMAC_DPP(CTRL) : "+v"(v[f_]) : "v"(flag));        \
+        }                                                                                       \
+    } while (0)
+
 template <int NV>
-__device__ __forceinline__ void segment_sum(float (&v)[NV], const uint64_t (&mk)[6], int nsteps, int lane) {
-    if (nsteps <= 0) return;
-    {
-        const bool take = (mk[0] >> lane) & 1ull;
-#pragma unroll
-        for (int f = 0; f < NV; ++f) {
-            const float s = v[f] + wave_shl1(v[f]);
-            v[f] = take ? s : v[f];
-        }
-    }
-#pragma unroll
-    for (int j = 1; j < 6; ++j) {
-        if (j >= nsteps) break;
-        const bool take = (mk[j] >> lane) & 1ull;
-#pragma unroll
-        for (int f = 0; f < NV; ++f) {
-            const float t = __shfl_down(v[f], 1 << j, 64);
-            v[f] = take ? v[f] + t : v[f];
-        }
-    }
+__device__ __forceinline__ void seg_scan(float (&v)[NV], const SegPlan& sp) {
+    static_assert(NV == 7 || NV == 1, "scan blocks are written for 7 or 1 values");
+    if (sp.s1) MPM_SCAN_STEP(NV, v, "row_shr:1 row_mask:0xf", sp.f1);
+    if (sp.s2) MPM_SCAN_STEP(NV, v, "row_shr:2 row_mask:0xf", sp.f2);
+    if (sp.s4) MPM_SCAN_STEP(NV, v, "row_shr:4 row_mask:0xf", sp.f4);
+    if (sp.s8) MPM_SCAN_STEP(NV, v, "row_shr:8 row_mask:0xf", sp.f8);
+    if (sp.s15) MPM_SCAN_STEP(NV, v, "row_bcast:15 row_mask:0xa", sp.f15);
+    if (sp.s31) MPM_SCAN_STEP(NV, v, "row_bcast:31 row_mask:0xc", sp.f31);
 }
 
 template <int INTERP, int APIC, int MODE, bool BATCH>
-__global__ void __launch_bounds__(64) k_p2g(SceneDev S0, ParticlesDev P0, ItemsDev I,
+__global__ void __launch_bounds__(64) MPM_P2G_OCC k_p2g(SceneDev S0, ParticlesDev P0, ItemsDev I,
                                             const SceneEntry* __restrict__ tab) {
     constexpr int W = Stencil<INTERP>::W;
     constexpr int T = 4 + W - 1;
@@ -324,15 +389,8 @@ __global__ void __launch_bounds__(64) k_p2g(SceneDev S0, ParticlesDev P0, ItemsD
             // segment structure of this chunk (inactive lanes form their own tail segment)
             const uint32_t key = active ? I.sorted_keys[ic] : 0xFFFFFFFFu;
             const uint32_t prev = __shfl_up(key, 1, 64);
-            const bool head = active && (lane == 0 || prev != key);
-            uint64_t mk[6] = {0ull, 0ull, 0ull, 0ull, 0ull, 0ull};
-            int nsteps = 0;
-#pragma unroll
-            for (int j = 0; j < 6; ++j) {
-                const uint32_t other = __shfl_down(key, 1 << j, 64);
-                mk[j] = __ballot(active && lane + (1 << j) < 64 && other == key);
-                if (mk[j] != 0ull) nsteps = j + 1;
-            }
+            const SegPlan sp = seg_plan(__ballot(lane == 0 || prev != key), active, lane);
+            const bool writer = sp.writer;
             const uint32_t p = I.perm[ic] - pbase;
             const float x0 = P.x[p], x1 = P.x[N + p], x2 = P.x[2 * N + p];
             int bx, by, bz;
@@ -350,9 +408,9 @@ __global__ void __launch_bounds__(64) k_p2g(SceneDev S0, ParticlesDev P0, ItemsD
 #pragma unroll
                         for (int c = 0; c < W; ++c) {
                             float v[1] = {m * ((Nx[a] * Ny[b]) * Nz[c])};
-                            segment_sum<1>(v, mk, nsteps, lane);
+                            seg_scan<1>(v, sp);
                             const int q = ((lz + c) * T + (ly + b)) * T + (lx + a);
-                            if (head) tile[q] += v[0];
+                            if (writer) tile[q] += v[0];
                         }
                 continue;
             }
@@ -422,9 +480,9 @@ __global__ void __launch_bounds__(64) k_p2g(SceneDev S0, ParticlesDev P0, ItemsD
                             v[1 + k] = mip * ck;
                             v[4 + k] = (F0[k] * yz[b][c] + F1[k] * dyz[b][c]) + F2[k] * ydz[b][c];
                         }
-                        segment_sum<7>(v, mk, nsteps, lane);
+                        seg_scan<7>(v, sp);
                         const int q = ((lz + c) * T + (ly + b)) * T + (lx + a);
-                        if (head) {
+                        if (writer) {
 #pragma unroll
                             for (int f = 0; f < 7; ++f) tile[f * TN + q] += v[f];
                         }
@@ -483,7 +541,7 @@ void launch_p2g(const SceneDev& S, const ParticlesDev& P, const ItemsDev& I, int
 // particlesCollision (:428-445) and advectParticles (:381-391) fused in.
 // SRC 2 is initialize()'s G2PVolume (src/mpm_solver3d.cu:173-214).
 template <int INTERP, int APIC, int SRC, bool BATCH, int WG>
-__global__ void __launch_bounds__(WG) k_g2p(SceneDev S0, ParticlesDev P0, ItemsDev I, const SceneEntry* __restrict__ tab,
+__global__ void __launch_bounds__(WG) MPM_G2P_OCC k_g2p(SceneDev S0, ParticlesDev P0, ItemsDev I, const SceneEntry* __restrict__ tab,
                                             int advect, int pcollide, uint32_t* __restrict__ next_keys) {
     constexpr int W = Stencil<INTERP>::W;
     constexpr int T = 4 + W - 1;
@@ -643,6 +701,23 @@ __global__ void __launch_bounds__(WG) k_g2p(SceneDev S0, ParticlesDev P0, ItemsD
                 n1 = S.blend * vf1 + ob * vpic1;
                 n2 = S.blend * vf2 + ob * vpic2;
             }
+            // Velocity, B, position and next sort key first: they depend on
+            // the gather only, and storing them before the F update frees
+            // their registers for the SVD.
+            if (APIC) store_m3(P.B, N, p, Bm);
+            if (BATCH ? S.particle_collision : pcollide)
+                boundary_collision(S, x0 + S.dt * n0, x1 + S.dt * n1, x2 + S.dt * n2, n0, n1, n2);
+            P.v[p] = n0;
+            P.v[N + p] = n1;
+            P.v[2 * N + p] = n2;
+            if (advect) {
+                const float y0 = x0 + S.dt * n0, y1 = x1 + S.dt * n1, y2 = x2 + S.dt * n2;
+                P.x[p] = y0;
+                P.x[N + p] = y1;
+                P.x[2 * N + p] = y2;
+                if (next_keys)
+                    next_keys[pbase + p] = koff + cell_key(S, y0, y1, y2, Stencil<INTERP>::SHIFT, Stencil<INTERP>::W);
+            }
             float Fe[3][3], Fp[3][3];
             load_m3(P.Fe, N, p, Fe);
             load_m3(P.Fp, N, p, Fp);
@@ -655,19 +730,6 @@ __global__ void __launch_bounds__(WG) k_g2p(SceneDev S0, ParticlesDev P0, ItemsD
             store_m3(P.Fp, N, p, Fp);
 #pragma unroll
             for (int k = 0; k < 6; ++k) P.tau[(int64_t)k * N + p] = tau[k];
-            if (APIC) store_m3(P.B, N, p, Bm);
-            if (BATCH ? S.particle_collision : pcollide) boundary_collision(S, x0 + S.dt * n0, x1 + S.dt * n1, x2 + S.dt * n2, n0, n1, n2);
-            P.v[p] = n0;
-            P.v[N + p] = n1;
-            P.v[2 * N + p] = n2;
-            if (advect) {
-                const float y0 = x0 + S.dt * n0, y1 = x1 + S.dt * n1, y2 = x2 + S.dt * n2;
-                P.x[p] = y0;
-                P.x[N + p] = y1;
-                P.x[2 * N + p] = y2;
-                if (next_keys)
-                    next_keys[pbase + p] = koff + cell_key(S, y0, y1, y2, Stencil<INTERP>::SHIFT, Stencil<INTERP>::W);
-            }
         }
         __syncthreads();
     }
