@@ -248,7 +248,8 @@ MPM_API int mpm_profile_read(void* handle, mpm_kernel_time* out, int max, int* c
 typedef struct mpm_tune {
     int32_t item_max;     /* particles per work item: multiple of 64 in [64, 65536] */
     int32_t g2p_threads;  /* threads per G2P workgroup: 64, 128 or 256             */
-    int32_t workgroups;   /* persistent workgroups of P2G/G2P; 0 = 8 per CU        */
+    int32_t workgroups;   /* persistent workgroups of P2G/G2P; 0 = one resident
+                             round per kernel (its occupancy x CUs)              */
     int32_t bucket;       /* out: size bucket, floor(log2(particles))              */
     double step_ms;       /* out (autotune): measured ms per step of the choice    */
 } mpm_tune;

@@ -36,7 +36,7 @@ namespace host {
 struct Tune {
     int item_max = kItemMax;
     int g2p_wg = 128;
-    int wgs = 0;  // 0: 8 per compute unit
+    int wgs = 0;  // 0: one resident round per kernel (occupancy x CUs)
     bool operator==(const Tune& o) const { return item_max == o.item_max && g2p_wg == o.g2p_wg && wgs == o.wgs; }
 };
 
@@ -120,7 +120,7 @@ void drop_graph(Batch& b) {
     if (b.graph) { (void)hipGraphDestroy(b.graph); b.graph = nullptr; }
 }
 
-int effective_wgs(const Batch& b) { return b.tune.wgs > 0 ? b.tune.wgs : std::max(64, b.cus * 8); }
+int effective_wgs(const Batch& b) { return b.tune.wgs; }
 
 // Per-particle buffers sized for at least n particles (grown by 1.5x so
 // that a stream of joins does not reallocate every time).
@@ -392,7 +392,7 @@ int check_tune(const mpm_tune* t) {
 void to_c(const Batch& b, const Tune& t, mpm_tune* out, double ms) {
     out->item_max = t.item_max;
     out->g2p_threads = t.g2p_wg;
-    out->workgroups = t.wgs > 0 ? t.wgs : std::max(64, b.cus * 8);
+    out->workgroups = t.wgs;
     out->bucket = size_bucket(b.ntot);
     out->step_ms = ms;
 }
@@ -536,13 +536,12 @@ MPM_API int mpm_batch_autotune(void* batch, int trial_steps, mpm_tune* out) {
     if (rc) return rc;
     if (b->ntot == 0) return fail(MPM_ERR_STATE, "batch is empty");
     // Coordinate search from the defaults: particles per work item, then
-    // G2P workgroup size, then persistent workgroups.  Every trial step is
-    // a real step of every scene (the parameters change the work split,
-    // not the arithmetic).
-    const int base_wgs = std::max(64, b->cus * 8);
+    // G2P workgroup size, then persistent workgroups (one resident round,
+    // or a fixed 8 or 16 per CU).  Every trial step is a real step of every
+    // scene (the parameters change the work split, not the arithmetic).
     const int item_opts[] = {256, 512, 1024};
     const int wg_opts[] = {64, 128, 256};
-    const int wgs_opts[] = {base_wgs / 2, base_wgs, base_wgs * 2};
+    const int wgs_opts[] = {0, b->cus * 8, b->cus * 16};
     hipEvent_t e0 = nullptr, e1 = nullptr;
     HIP_TRY(hipEventCreate(&e0));
     HIP_TRY(hipEventCreate(&e1));
@@ -563,7 +562,6 @@ MPM_API int mpm_batch_autotune(void* batch, int trial_steps, mpm_tune* out) {
         return MPM_OK;
     };
     Tune best;
-    best.wgs = base_wgs;
     double best_ms = 0.0;
     rc = measure(best, best_ms);
     for (int stage = 0; stage < 3 && !rc; ++stage) {

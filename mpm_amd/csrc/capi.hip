@@ -374,9 +374,7 @@ int build(Solver& s, HostState& H, int run_init) {
     int rc;
     HIP_TRY(hipGetDevice(&s.device));
     HIP_TRY(hipStreamCreateWithFlags(&s.st, hipStreamNonBlocking));
-    hipDeviceProp_t prop;
-    HIP_TRY(hipGetDeviceProperties(&prop, s.device));
-    s.item_wgs = std::max(64, prop.multiProcessorCount * 8);
+    s.item_wgs = 0;  // persistent kernels: one resident round (occupancy x CUs)
     fill_scene(s);
     if ((rc = upload_materials(s))) return rc;
     // grid

@@ -321,11 +321,14 @@ MPM_HD void grid_node_update(const SceneDev& S, int i, int j, int k, float m, fl
                                                  float& vz, float& wx, float& wy, float& wz) {
     bool heavy = m > MPM_FLT_EPS;
     if (heavy) {
+        // one correctly rounded reciprocal instead of six divisions: each
+        // quotient differs from p/m by at most one ulp
         fy = fmaf(-S.gravity, m, fy);
-        vx = px / m; vy = py / m; vz = pz / m;
-        wx = vx + (S.dt * fx) / m;
-        wy = vy + (S.dt * fy) / m;
-        wz = vz + (S.dt * fz) / m;
+        const float im = 1.0f / m;
+        vx = px * im; vy = py * im; vz = pz * im;
+        wx = vx + (S.dt * fx) * im;
+        wy = vy + (S.dt * fy) * im;
+        wz = vz + (S.dt * fz) * im;
     } else {
         vx = px; vy = py; vz = pz;
         wx = 0.0f; wy = 0.0f; wz = 0.0f;

@@ -14,12 +14,52 @@
 //        is computed while loading, so there is no separate grid pass.
 #include <hip/hip_runtime.h>
 
+#include <map>
+#include <mutex>
+#include <utility>
+
 #include "devmath.h"
 #include "launch.h"
 
 namespace mpm {
 
 constexpr int kG2PWG = 128;
+
+// Persistent launches are sized to one resident round: the occupancy of
+// the kernel instance (VGPRs, LDS, workgroup size) times the compute units.
+// A larger grid leaves a partial second round (a tail), a smaller one idles
+// SIMD slots.  requested > 0 overrides.  Cached per (kernel, device).
+static int device_cus() {
+    int dev = 0;
+    if (hipGetDevice(&dev) != hipSuccess) return 1;
+    static int cus[64] = {0};
+    if (dev < 0 || dev >= 64) return 1;
+    if (!cus[dev] && hipDeviceGetAttribute(&cus[dev], hipDeviceAttributeMultiprocessorCount, dev) != hipSuccess)
+        cus[dev] = 1;
+    return cus[dev] > 0 ? cus[dev] : 1;
+}
+template <typename F>
+static unsigned resident_grid(F* kernel, int block, int requested) {
+    if (requested > 0) return (unsigned)requested;
+    static std::mutex mu;
+    static std::map<std::pair<const void*, int>, int> cache;
+    int dev = 0;
+    (void)hipGetDevice(&dev);
+    std::lock_guard<std::mutex> g(mu);
+    auto key = std::make_pair((const void*)kernel, dev);
+    auto it = cache.find(key);
+    if (it != cache.end()) return (unsigned)it->second;
+    int nb = 0;
+    if (hipOccupancyMaxActiveBlocksPerMultiprocessor(&nb, kernel, block, 0) != hipSuccess || nb < 1) {
+        (void)hipGetLastError();
+        nb = 1;
+    }
+    const int grid = nb * device_cus();
+    cache[key] = grid;
+    return (unsigned)grid;
+}
+#define MPM_PERSISTENT(K, BLOCK, REQ, ST, ...) \
+    hipLaunchKernelGGL(K, dim3(resident_grid(K, BLOCK, REQ)), dim3(BLOCK), 0, ST, __VA_ARGS__)
 
 // Occupancy targets (waves per SIMD) of the item kernels; 0 leaves the
 // register allocation to the compiler.
@@ -513,9 +553,9 @@ __global__ void __launch_bounds__(64) MPM_P2G_OCC k_p2g(SceneDev S0, ParticlesDe
 template <int INTERP, int APIC>
 static void p2g_dispatch_mode(const SceneDev& S, const ParticlesDev& P, const ItemsDev& I, const SceneEntry* tab,
                               int mode, int wgs, hipStream_t st) {
-    if (tab) hipLaunchKernelGGL((k_p2g<INTERP, APIC, 0, true>), dim3(wgs), dim3(64), 0, st, S, P, I, tab);
-    else if (mode == 0) hipLaunchKernelGGL((k_p2g<INTERP, APIC, 0, false>), dim3(wgs), dim3(64), 0, st, S, P, I, tab);
-    else hipLaunchKernelGGL((k_p2g<INTERP, APIC, 1, false>), dim3(wgs), dim3(64), 0, st, S, P, I, tab);
+    if (tab) MPM_PERSISTENT((k_p2g<INTERP, APIC, 0, true>), 64, wgs, st, S, P, I, tab);
+    else if (mode == 0) MPM_PERSISTENT((k_p2g<INTERP, APIC, 0, false>), 64, wgs, st, S, P, I, tab);
+    else MPM_PERSISTENT((k_p2g<INTERP, APIC, 1, false>), 64, wgs, st, S, P, I, tab);
 }
 template <int INTERP>
 static void p2g_dispatch_apic(const SceneDev& S, const ParticlesDev& P, const ItemsDev& I, const SceneEntry* tab,
@@ -739,14 +779,12 @@ template <int INTERP, int APIC, int SRC, bool BATCH>
 static void g2p_dispatch_wg(const SceneDev& S, const ParticlesDev& P, const ItemsDev& I, const SceneEntry* tab,
                             int advect, int pcollide, int wgs, int wg, hipStream_t st, uint32_t* nk) {
     if (wg == 64)
-        hipLaunchKernelGGL((k_g2p<INTERP, APIC, SRC, BATCH, 64>), dim3(wgs), dim3(64), 0, st, S, P, I, tab, advect,
-                           pcollide, nk);
+        MPM_PERSISTENT((k_g2p<INTERP, APIC, SRC, BATCH, 64>), 64, wgs, st, S, P, I, tab, advect, pcollide, nk);
     else if (wg == 256)
-        hipLaunchKernelGGL((k_g2p<INTERP, APIC, SRC, BATCH, 256>), dim3(wgs), dim3(256), 0, st, S, P, I, tab,
-                           advect, pcollide, nk);
+        MPM_PERSISTENT((k_g2p<INTERP, APIC, SRC, BATCH, 256>), 256, wgs, st, S, P, I, tab, advect, pcollide, nk);
     else
-        hipLaunchKernelGGL((k_g2p<INTERP, APIC, SRC, BATCH, kG2PWG>), dim3(wgs), dim3(kG2PWG), 0, st, S, P, I, tab,
-                           advect, pcollide, nk);
+        MPM_PERSISTENT((k_g2p<INTERP, APIC, SRC, BATCH, kG2PWG>), kG2PWG, wgs, st, S, P, I, tab, advect, pcollide,
+                       nk);
 }
 template <int INTERP, int APIC>
 static void g2p_dispatch_src(const SceneDev& S, const ParticlesDev& P, const ItemsDev& I, const SceneEntry* tab,
@@ -754,11 +792,9 @@ static void g2p_dispatch_src(const SceneDev& S, const ParticlesDev& P, const Ite
     if (tab) g2p_dispatch_wg<INTERP, APIC, 0, true>(S, P, I, tab, advect, pcollide, wgs, wg, st, nk);
     else if (src == 0) g2p_dispatch_wg<INTERP, APIC, 0, false>(S, P, I, tab, advect, pcollide, wgs, wg, st, nk);
     else if (src == 1)
-        hipLaunchKernelGGL((k_g2p<INTERP, APIC, 1, false, kG2PWG>), dim3(wgs), dim3(kG2PWG), 0, st, S, P, I, tab,
-                           advect, pcollide, nk);
+        MPM_PERSISTENT((k_g2p<INTERP, APIC, 1, false, kG2PWG>), kG2PWG, wgs, st, S, P, I, tab, advect, pcollide, nk);
     else
-        hipLaunchKernelGGL((k_g2p<INTERP, APIC, 2, false, kG2PWG>), dim3(wgs), dim3(kG2PWG), 0, st, S, P, I, tab,
-                           advect, pcollide, nk);
+        MPM_PERSISTENT((k_g2p<INTERP, APIC, 2, false, kG2PWG>), kG2PWG, wgs, st, S, P, I, tab, advect, pcollide, nk);
 }
 template <int INTERP>
 static void g2p_dispatch_apic(const SceneDev& S, const ParticlesDev& P, const ItemsDev& I, const SceneEntry* tab,

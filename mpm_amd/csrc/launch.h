@@ -43,6 +43,8 @@ void launch_items(const uint32_t* sorted_keys, int64_t n, uint32_t* flags, uint3
                   uint32_t* item_count, hipStream_t st, uint32_t item_max = 0, const uint32_t* key_offsets = nullptr,
                   int nscenes = 0, uint32_t* item_scene = nullptr);
 
+// grid_wgs: persistent workgroups; 0 = one resident round (occupancy of the
+// kernel instance x compute units).
 // mode: 0 = full P2G (mass, momentum, force), 1 = mass only (initialize).
 // With a scene table (tab != null, I.scene set) every item reads its scene
 // from the table (mode 0 only); S and P are then unused.

@@ -67,7 +67,7 @@ struct Solver {
     bool stress_valid = false;  // P.tau matches the particle state
     int phase = PH_CLEAR;
     int64_t steps = 0;
-    int item_wgs = 1024;
+    int item_wgs = 0;  // 0: persistent grids sized by occupancy
     double last_step_ms = 0.0;
     std::vector<void*> allocs;
     int64_t device_bytes = 0;
