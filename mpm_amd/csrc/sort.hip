@@ -1,7 +1,8 @@
 // sort.hip -- stable LSD radix sort of (key, value) uint32 pairs and a
 // device-wide exclusive scan, written for wave64 (gfx950).
 //
-// One pass over an 8-bit digit is three launches:
+// One pass over a digit (8 or 9 bits: 25-bit cell keys sort in 3 passes of
+// 9+8+8 bits instead of 4 of 8) is three launches:
 //   hist    : per-tile digit histogram, stored digit-major (digit * tiles + tile)
 //   scan    : exclusive scan of that table -> global base of (digit, tile)
 //   scatter : per-tile stable ranking (64-key chunks ranked with 8 ballots,
@@ -16,18 +17,19 @@
 
 namespace mpm {
 
-constexpr int kRadixBits = 8;
-constexpr int kRadix = 1 << kRadixBits;
+constexpr int kMaxRadixBits = 9;
+constexpr int kMaxRadix = 1 << kMaxRadixBits;
 constexpr int kSortWG = 256;
 constexpr int kSortIPT = 8;
 constexpr int kSortTile = kSortWG * kSortIPT;  // 2048 keys per tile
 constexpr int kSortChunks = kSortTile / 64;    // 32 chunks of 64 keys
 constexpr int kChunksPerWave = kSortChunks / (kSortWG / 64);
 
+template <int BITS>
 __device__ __forceinline__ uint64_t digit_match(uint32_t digit, uint64_t active) {
     uint64_t m = active;
 #pragma unroll
-    for (int b = 0; b < kRadixBits; ++b) {
+    for (int b = 0; b < BITS; ++b) {
         bool bit = (digit >> b) & 1u;
         uint64_t bb = __ballot(bit);
         m &= bit ? bb : ~bb;
@@ -35,8 +37,10 @@ __device__ __forceinline__ uint64_t digit_match(uint32_t digit, uint64_t active)
     return m;
 }
 
+template <int BITS>
 __global__ void __launch_bounds__(kSortWG) k_radix_hist(const uint32_t* __restrict__ keys, int64_t n, int shift,
                                                         uint32_t mask, uint32_t* __restrict__ hist, int64_t tiles) {
+    constexpr int kRadix = 1 << BITS;
     __shared__ uint32_t h[kRadix];
     const int tid = threadIdx.x;
     const int lane = tid & 63;
@@ -52,7 +56,7 @@ __global__ void __launch_bounds__(kSortWG) k_radix_hist(const uint32_t* __restri
         uint32_t digit = valid ? ((keys[idx] >> shift) & mask) : 0u;
         uint64_t active = __ballot(valid);
         if (active == 0) break;
-        uint64_t m = digit_match(digit, active);
+        uint64_t m = digit_match<BITS>(digit, active);
         int leader = __ffsll((unsigned long long)m) - 1;
         if (valid && lane == leader) atomicAdd(&h[digit], (uint32_t)__popcll(m));
     }
@@ -60,13 +64,17 @@ __global__ void __launch_bounds__(kSortWG) k_radix_hist(const uint32_t* __restri
     for (int d = tid; d < kRadix; d += kSortWG) hist[(int64_t)d * tiles + tile] = h[d];
 }
 
+template <int BITS>
 __global__ void __launch_bounds__(kSortWG) k_radix_scatter(const uint32_t* __restrict__ keys_in,
                                                            const uint32_t* __restrict__ vals_in,
                                                            uint32_t* __restrict__ keys_out,
                                                            uint32_t* __restrict__ vals_out, int64_t n, int shift,
                                                            uint32_t mask, const uint32_t* __restrict__ hist_scan,
                                                            int64_t tiles) {
-    __shared__ uint16_t chunk_off[kSortChunks][kRadix];  // 16 KB
+    constexpr int kRadix = 1 << BITS;
+    constexpr int DPT = kRadix / kSortWG;  // digits per thread (1 or 2)
+    static_assert(DPT >= 1 && DPT * kSortWG == kRadix, "radix must be a multiple of the workgroup size");
+    __shared__ uint16_t chunk_off[kSortChunks][kRadix];  // 16 KB (8 bits) or 32 KB (9 bits)
     __shared__ uint32_t digit_base[kRadix];
     __shared__ uint32_t global_base[kRadix];
     __shared__ uint32_t stage_k[kSortTile];
@@ -94,7 +102,7 @@ __global__ void __launch_bounds__(kSortWG) k_radix_scatter(const uint32_t* __res
         uint32_t v = valid ? vals_in[idx] : 0u;
         uint32_t digit = (k >> shift) & mask;
         uint64_t active = __ballot(valid);
-        uint64_t m = digit_match(digit, active);
+        uint64_t m = digit_match<BITS>(digit, active);
         uint64_t lt = (lane == 0) ? 0ull : ((~0ull) >> (64 - lane));
         my_rank[c] = (uint32_t)__popcll(m & lt);
         int leader = __ffsll((unsigned long long)m) - 1;
@@ -105,17 +113,24 @@ __global__ void __launch_bounds__(kSortWG) k_radix_scatter(const uint32_t* __res
     }
     __syncthreads();
 
-    // Per digit: exclusive prefix over chunks (thread d owns digit d).
+    // Per digit: exclusive prefix over chunks (thread t owns digits
+    // t*DPT .. t*DPT+DPT-1, so digit order follows thread order).
+    uint32_t dtot[DPT];
     uint32_t total = 0;
-    if (tid < kRadix) {
+#pragma unroll
+    for (int j = 0; j < DPT; ++j) {
+        const int d = tid * DPT + j;
+        uint32_t acc = 0;
         for (int c = 0; c < kSortChunks; ++c) {
-            uint32_t cnt = chunk_off[c][tid];
-            chunk_off[c][tid] = (uint16_t)total;
-            total += cnt;
+            uint32_t cnt = chunk_off[c][d];
+            chunk_off[c][d] = (uint16_t)acc;
+            acc += cnt;
         }
-        global_base[tid] = hist_scan[(int64_t)tid * tiles + tile];
+        dtot[j] = acc;
+        total += acc;
+        global_base[d] = hist_scan[(int64_t)d * tiles + tile];
     }
-    // Block exclusive scan of the per-digit totals (kRadix == kSortWG).
+    // Block exclusive scan of the per-thread digit totals.
     uint32_t incl = total;
 #pragma unroll
     for (int off = 1; off < 64; off <<= 1) {
@@ -126,7 +141,14 @@ __global__ void __launch_bounds__(kSortWG) k_radix_scatter(const uint32_t* __res
     __syncthreads();
     uint32_t wave_prefix = 0;
     for (int w = 0; w < wave; ++w) wave_prefix += wave_sums[w];
-    if (tid < kRadix) digit_base[tid] = wave_prefix + incl - total;
+    {
+        uint32_t run = wave_prefix + incl - total;
+#pragma unroll
+        for (int j = 0; j < DPT; ++j) {
+            digit_base[tid * DPT + j] = run;
+            run += dtot[j];
+        }
+    }
     __syncthreads();
 
 #pragma unroll
@@ -244,9 +266,20 @@ void exclusive_scan(uint32_t* data, int64_t n, uint32_t* temp, uint32_t* total, 
 }
 
 int64_t radix_tiles(int64_t n) { return (n + kSortTile - 1) / kSortTile; }
-int64_t radix_hist_words(int64_t n) { return radix_tiles(n) * kRadix; }
+int64_t radix_hist_words(int64_t n) { return radix_tiles(n) * kMaxRadix; }
 
-int radix_passes(int key_bits) { return (key_bits + kRadixBits - 1) / kRadixBits; }
+// Digit widths: 8-bit digits unless 9-bit digits need fewer passes; the
+// bits are split as evenly as possible, widest digits first.
+int radix_passes(int key_bits) {
+    const int p8 = (key_bits + 7) / 8, p9 = (key_bits + 8) / 9;
+    return p9 < p8 ? p9 : p8;
+}
+static int digit_bits(int key_bits, int pass) {
+    const int passes = radix_passes(key_bits);
+    int done = 0;
+    for (int p = 0; p < pass; ++p) done += (key_bits - done + (passes - p) - 1) / (passes - p);
+    return (key_bits - done + (passes - pass) - 1) / (passes - pass);
+}
 
 // Sorts (kin,vin) (read only) into (ka,va)/(kb,vb) ping-pong buffers;
 // returns 0 if the result is in (ka,va), 1 if in (kb,vb).
@@ -258,20 +291,31 @@ int radix_sort_pairs(const uint32_t* kin, const uint32_t* vin, uint32_t* ka, uin
     int passes = radix_passes(key_bits);
     const uint32_t *ki = kin, *vi = vin;
     uint32_t *ko = ka, *vo = va;
+    int shift = 0;
     for (int p = 0; p < passes; ++p) {
-        int shift = p * kRadixBits;
-        int bits = key_bits - shift < kRadixBits ? key_bits - shift : kRadixBits;
-        uint32_t mask = (1u << bits) - 1u;
+        const int bits = digit_bits(key_bits, p);
+        const uint32_t mask = (1u << bits) - 1u;
+        const int radix = bits > 8 ? 512 : 256;
         prof.begin("sort_hist");
-        hipLaunchKernelGGL(k_radix_hist, dim3((unsigned)tiles), dim3(kSortWG), 0, st, ki, n, shift, mask, hist, tiles);
+        if (bits > 8)
+            hipLaunchKernelGGL(k_radix_hist<9>, dim3((unsigned)tiles), dim3(kSortWG), 0, st, ki, n, shift, mask, hist,
+                               tiles);
+        else
+            hipLaunchKernelGGL(k_radix_hist<8>, dim3((unsigned)tiles), dim3(kSortWG), 0, st, ki, n, shift, mask, hist,
+                               tiles);
         prof.end("sort_hist");
         prof.begin("sort_scan");
-        exclusive_scan(hist, tiles * kRadix, scan_temp, nullptr, st);
+        exclusive_scan(hist, tiles * radix, scan_temp, nullptr, st);
         prof.end("sort_scan");
         prof.begin("sort_scatter");
-        hipLaunchKernelGGL(k_radix_scatter, dim3((unsigned)tiles), dim3(kSortWG), 0, st, ki, vi, ko, vo, n, shift, mask,
-                           hist, tiles);
+        if (bits > 8)
+            hipLaunchKernelGGL(k_radix_scatter<9>, dim3((unsigned)tiles), dim3(kSortWG), 0, st, ki, vi, ko, vo, n,
+                               shift, mask, hist, tiles);
+        else
+            hipLaunchKernelGGL(k_radix_scatter<8>, dim3((unsigned)tiles), dim3(kSortWG), 0, st, ki, vi, ko, vo, n,
+                               shift, mask, hist, tiles);
         prof.end("sort_scatter");
+        shift += bits;
         ki = ko;
         vi = vo;
         ko = (ko == ka) ? kb : ka;

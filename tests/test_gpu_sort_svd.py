@@ -56,13 +56,14 @@ def test_sort_escapees_and_boundaries():
     assert np.isfinite(st.position[: lat.shape[0]]).all()
 
 
-def test_sort_large_random():
-    rng = scenes.Rng(77)
+@pytest.mark.parametrize("res", [128, 256])
+def test_sort_large_random(res):
+    # 128^3: 22-bit keys, three 8/7/7-bit passes; 256^3: 25-bit keys,
+    # three 9/8/8-bit passes (512-bin histograms)
     n = 300_001
-    sc = scenes.snow_cube(seed=1, n=n, res=128, lo=0.0, hi=1.0)
+    sc = scenes.snow_cube(seed=1, n=n, res=res, lo=0.0, hi=1.0)
     gpu = Solver(sc.desc, sc.particles)
-    _check_sort(gpu, sc.desc, "sort 300k random")
-    del rng
+    _check_sort(gpu, sc.desc, f"sort 300k random res={res}")
 
 
 def _svd_invariants(A, U, s, V, tol):
