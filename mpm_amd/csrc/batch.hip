@@ -35,7 +35,7 @@ namespace host {
 
 struct Tune {
     int item_max = kItemMax;
-    int g2p_wg = 128;
+    int g2p_wg = 64;
     int wgs = 0;  // 0: one resident round per kernel (occupancy x CUs)
     bool operator==(const Tune& o) const { return item_max == o.item_max && g2p_wg == o.g2p_wg && wgs == o.wgs; }
 };
@@ -128,13 +128,15 @@ int ensure_capacity(Batch& b, int64_t n) {
     if (n <= b.cap && b.kin) return MPM_OK;
     (void)hipStreamSynchronize(b.st);
     int64_t c = std::max<int64_t>(std::max<int64_t>(n, 1), b.cap + b.cap / 2);
-    uint32_t** bufs[] = {&b.kin, &b.iota, &b.ka, &b.va, &b.kb, &b.vb, &b.flags, &b.items, &b.item_scene};
+    uint32_t** bufs[] = {&b.kin, &b.iota, &b.ka, &b.va, &b.kb, &b.vb, &b.flags, &b.item_scene};
     for (auto pp : bufs) bfree(b, *(void**)pp);
+    bfree(b, *(void**)&b.items);
     bfree(b, *(void**)&b.hist);
     bfree(b, *(void**)&b.scan_temp);
     int rc;
     for (auto pp : bufs)
         if ((rc = balloc(b, (void**)pp, (size_t)c * 4))) return rc;
+    if ((rc = balloc(b, (void**)&b.items, (size_t)c * 8))) return rc;  // starts + blocks
     const int64_t hw = radix_hist_words(c);
     if ((rc = balloc(b, (void**)&b.hist, (size_t)hw * 4))) return rc;
     const int64_t tw = std::max(scan_temp_words(hw), scan_temp_words(c));
@@ -206,6 +208,7 @@ int relayout(Batch& b) {
     launch_iota(b.iota, n, b.st);
     b.I = ItemsDev{};
     b.I.start = b.items;
+    b.I.blk = b.items + n;
     b.I.count = b.counters;
     b.I.scene = b.item_scene;
     b.I.n = n;

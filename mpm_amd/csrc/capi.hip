@@ -299,6 +299,7 @@ void run_sort(Solver& s) {
     s.I.sorted_keys = sel ? s.kb : s.ka;
     s.I.perm = sel ? s.vb : s.va;
     s.I.start = s.items;
+    s.I.blk = s.items + s.N;
     s.I.count = s.counters;
     s.I.n = s.N;
     {
@@ -393,7 +394,7 @@ int build(Solver& s, HostState& H, int run_init) {
     if ((rc = dalloc(s, (void**)&s.vb, (size_t)n * 4))) return rc;
     launch_iota(s.iota, s.N, s.st);
     if ((rc = dalloc(s, (void**)&s.flags, (size_t)n * 4))) return rc;
-    if ((rc = dalloc(s, (void**)&s.items, (size_t)n * 4))) return rc;
+    if ((rc = dalloc(s, (void**)&s.items, (size_t)n * 8))) return rc;  // starts + blocks
     int64_t hw = radix_hist_words(n);
     if ((rc = dalloc(s, (void**)&s.hist, (size_t)hw * 4))) return rc;
     int64_t tw = std::max(scan_temp_words(hw), scan_temp_words(n));

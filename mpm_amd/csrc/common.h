@@ -92,6 +92,7 @@ struct ItemsDev {
     const uint32_t* sorted_keys;  // N (global keys)
     const uint32_t* perm;         // N, sorted position -> global slot
     const uint32_t* start;        // item start positions
+    const uint32_t* blk;          // item key block (scene-local), so the item needs no key load
     const uint32_t* count;        // device counter: number of items
     const uint32_t* scene;        // per item scene index (batches) or null
     int64_t n;

@@ -23,7 +23,7 @@
 
 namespace mpm {
 
-constexpr int kG2PWG = 128;
+constexpr int kG2PWG = 64;  // default G2P workgroup: one wave per item
 
 // Persistent launches are sized to one resident round: the occupancy of
 // the kernel instance (VGPRs, LDS, workgroup size) times the compute units.
@@ -61,24 +61,19 @@ static unsigned resident_grid(F* kernel, int block, int requested) {
 #define MPM_PERSISTENT(K, BLOCK, REQ, ST, ...) \
     hipLaunchKernelGGL(K, dim3(resident_grid(K, BLOCK, REQ)), dim3(BLOCK), 0, ST, __VA_ARGS__)
 
-// Occupancy targets (waves per SIMD) of the item kernels; 0 leaves the
-// register allocation to the compiler.
-#ifndef MPM_P2G_WAVES
-#define MPM_P2G_WAVES 0
+// Minimum waves per SIMD asked of the register allocator for G2P.  APIC
+// G2P needs 151-171 VGPRs: a cap of 168 (3 waves per SIMD with 512
+// registers per lane) costs no spills, while 171 would drop it to 2 waves.
+// PIC-FLIP G2P (187-256 VGPRs) would spill 124-252 B per lane under the
+// same cap, so it keeps the compiler's allocation unless MPM_PIC_G2P_WAVES
+// says otherwise.
+#ifndef MPM_PIC_G2P_WAVES
+#define MPM_PIC_G2P_WAVES 1
 #endif
-#ifndef MPM_G2P_WAVES
-#define MPM_G2P_WAVES 0
-#endif
-#if MPM_P2G_WAVES > 0
-#define MPM_P2G_OCC __attribute__((amdgpu_waves_per_eu(MPM_P2G_WAVES)))
-#else
-#define MPM_P2G_OCC
-#endif
-#if MPM_G2P_WAVES > 0
-#define MPM_G2P_OCC __attribute__((amdgpu_waves_per_eu(MPM_G2P_WAVES)))
-#else
-#define MPM_G2P_OCC
-#endif
+template <int APIC>
+struct G2PWaves {
+    static constexpr int value = APIC ? 3 : MPM_PIC_G2P_WAVES;
+};
 
 // ------------------------------------------------------------------ keys
 // Cell key of each particle, plus the scene's key offset in a batch.
@@ -127,25 +122,30 @@ __global__ void k_item_flags(const uint32_t* __restrict__ sk, int64_t n, uint32_
     int64_t i = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
     if (i < n) flags[i] = item_head(sk, i, item_max) ? 1u : 0u;
 }
-// Writes the start position of each item and, for a batch, its scene: the
-// last scene whose key offset is <= the item's first key.
+// Writes the start position of each item, its scene-local key block and,
+// for a batch, its scene: the last scene whose key offset is <= the item's
+// first key.  items[n + k] holds the block of item k.
 __global__ void k_item_compact(const uint32_t* __restrict__ sk, int64_t n, uint32_t item_max,
                                const uint32_t* __restrict__ pos, uint32_t* __restrict__ items,
                                const uint32_t* __restrict__ key_offsets, int nscenes,
                                uint32_t* __restrict__ item_scene) {
     int64_t i = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
     if (i >= n || !item_head(sk, i, item_max)) return;
-    items[pos[i]] = (uint32_t)i;
+    const uint32_t k = pos[i];
+    const uint32_t key = sk[i];
+    items[k] = (uint32_t)i;
+    uint32_t koff = 0;
     if (item_scene) {
-        const uint32_t key = sk[i];
         int lo = 0, hi = nscenes - 1;
         while (lo < hi) {
             const int mid = (lo + hi + 1) >> 1;
             if (key_offsets[mid] <= key) lo = mid;
             else hi = mid - 1;
         }
-        item_scene[pos[i]] = (uint32_t)lo;
+        item_scene[k] = (uint32_t)lo;
+        koff = key_offsets[lo];
     }
+    items[n + k] = (key - koff) >> 6;
 }
 
 void launch_items(const uint32_t* sorted_keys, int64_t n, uint32_t* flags, uint32_t* scan_temp, uint32_t* items,
@@ -172,7 +172,8 @@ __device__ __forceinline__ ItemGeom item_geom(const SceneDev& S, const ItemsDev&
     ItemGeom g;
     g.s = I.start[it];
     g.e = (it + 1 < nitems) ? I.start[it + 1] : (uint32_t)I.n;
-    uint32_t blk = (I.sorted_keys[g.s] - key_offset) >> 6;
+    const uint32_t blk = I.blk[it];
+    (void)key_offset;
     g.escape = blk == (S.escape_key >> 6);
     uint32_t bx = blk % (uint32_t)S.kbx;
     uint32_t rest = blk / (uint32_t)S.kbx;
@@ -403,7 +404,7 @@ __device__ __forceinline__ void seg_scan(float (&v)[NV], const SegPlan& sp) {
 }
 
 template <int INTERP, int APIC, int MODE, bool BATCH>
-__global__ void __launch_bounds__(64) MPM_P2G_OCC k_p2g(SceneDev S0, ParticlesDev P0, ItemsDev I,
+__global__ void __launch_bounds__(64) k_p2g(SceneDev S0, ParticlesDev P0, ItemsDev I,
                                             const SceneEntry* __restrict__ tab) {
     constexpr int W = Stencil<INTERP>::W;
     constexpr int T = 4 + W - 1;
@@ -581,7 +582,7 @@ void launch_p2g(const SceneDev& S, const ParticlesDev& P, const ItemsDev& I, int
 // particlesCollision (:428-445) and advectParticles (:381-391) fused in.
 // SRC 2 is initialize()'s G2PVolume (src/mpm_solver3d.cu:173-214).
 template <int INTERP, int APIC, int SRC, bool BATCH, int WG>
-__global__ void __launch_bounds__(WG) MPM_G2P_OCC k_g2p(SceneDev S0, ParticlesDev P0, ItemsDev I, const SceneEntry* __restrict__ tab,
+__global__ void __launch_bounds__(WG) __attribute__((amdgpu_waves_per_eu(G2PWaves<APIC>::value))) k_g2p(SceneDev S0, ParticlesDev P0, ItemsDev I, const SceneEntry* __restrict__ tab,
                                             int advect, int pcollide, uint32_t* __restrict__ next_keys) {
     constexpr int W = Stencil<INTERP>::W;
     constexpr int T = 4 + W - 1;
@@ -778,13 +779,12 @@ __global__ void __launch_bounds__(WG) MPM_G2P_OCC k_g2p(SceneDev S0, ParticlesDe
 template <int INTERP, int APIC, int SRC, bool BATCH>
 static void g2p_dispatch_wg(const SceneDev& S, const ParticlesDev& P, const ItemsDev& I, const SceneEntry* tab,
                             int advect, int pcollide, int wgs, int wg, hipStream_t st, uint32_t* nk) {
-    if (wg == 64)
-        MPM_PERSISTENT((k_g2p<INTERP, APIC, SRC, BATCH, 64>), 64, wgs, st, S, P, I, tab, advect, pcollide, nk);
+    if (wg == 128)
+        MPM_PERSISTENT((k_g2p<INTERP, APIC, SRC, BATCH, 128>), 128, wgs, st, S, P, I, tab, advect, pcollide, nk);
     else if (wg == 256)
         MPM_PERSISTENT((k_g2p<INTERP, APIC, SRC, BATCH, 256>), 256, wgs, st, S, P, I, tab, advect, pcollide, nk);
     else
-        MPM_PERSISTENT((k_g2p<INTERP, APIC, SRC, BATCH, kG2PWG>), kG2PWG, wgs, st, S, P, I, tab, advect, pcollide,
-                       nk);
+        MPM_PERSISTENT((k_g2p<INTERP, APIC, SRC, BATCH, 64>), 64, wgs, st, S, P, I, tab, advect, pcollide, nk);
 }
 template <int INTERP, int APIC>
 static void g2p_dispatch_src(const SceneDev& S, const ParticlesDev& P, const ItemsDev& I, const SceneEntry* tab,

@@ -37,8 +37,10 @@ constexpr int kItemMax = 512;   // particles per work item (max)
 void launch_keys(const SceneDev& S, const ParticlesDev& P, int interp, uint32_t* keys, hipStream_t st,
                  uint32_t key_offset = 0);
 void launch_iota(uint32_t* p, int64_t n, hipStream_t st);
-// Work items over sorted keys.  item_max = 0 selects kItemMax.  For a batch,
-// key_offsets[nscenes] (ascending) gives each item's scene in item_scene.
+// Work items over sorted keys: items[0..n) receives the item starts and
+// items[n..2n) their key blocks (so items holds 2n words).  item_max = 0
+// selects kItemMax.  For a batch, key_offsets[nscenes] (ascending) gives
+// each item's scene in item_scene.
 void launch_items(const uint32_t* sorted_keys, int64_t n, uint32_t* flags, uint32_t* scan_temp, uint32_t* items,
                   uint32_t* item_count, hipStream_t st, uint32_t item_max = 0, const uint32_t* key_offsets = nullptr,
                   int nscenes = 0, uint32_t* item_scene = nullptr);
@@ -58,7 +60,7 @@ void launch_p2g(const SceneDev& S, const ParticlesDev& P, const ItemsDev& I, int
 // wg: threads per workgroup of the fused G2P (64, 128 or 256).
 void launch_g2p(const SceneDev& S, const ParticlesDev& P, const ItemsDev& I, int interp, int apic, int src,
                 int advect, int pcollide, int grid_wgs, hipStream_t st, uint32_t* next_keys = nullptr,
-                const SceneEntry* tab = nullptr, int wg = 128);
+                const SceneEntry* tab = nullptr, int wg = 64);
 
 void launch_clear(const SceneDev& S, int64_t storage_blocks, int nfields, unsigned long long* active_nodes,
                   hipStream_t st);
