@@ -614,10 +614,10 @@ __global__ void __launch_bounds__(WG) __attribute__((amdgpu_waves_per_eu(G2PWave
                         vals[0] = G[(GF_VSTAR + 0) * pl + ni];
                         vals[1] = G[(GF_VSTAR + 1) * pl + ni];
                         vals[2] = G[(GF_VSTAR + 2) * pl + ni];
-                        if (!APIC) {
-                            vals[3] = G[(GF_VEL + 0) * pl + ni];
-                            vals[4] = G[(GF_VEL + 1) * pl + ni];
-                            vals[5] = G[(GF_VEL + 2) * pl + ni];
+                        if (!APIC) {  // FLIP increment v* - v
+                            vals[3] = vals[0] - G[(GF_VEL + 0) * pl + ni];
+                            vals[4] = vals[1] - G[(GF_VEL + 1) * pl + ni];
+                            vals[5] = vals[2] - G[(GF_VEL + 2) * pl + ni];
                         }
                     } else {
                         float vx, vy, vz, wx, wy, wz;
@@ -626,7 +626,7 @@ __global__ void __launch_bounds__(WG) __attribute__((amdgpu_waves_per_eu(G2PWave
                                          G[(GF_FORCE + 1) * pl + ni], G[(GF_FORCE + 2) * pl + ni], vx, vy, vz, wx, wy,
                                          wz);
                         vals[0] = wx; vals[1] = wy; vals[2] = wz;
-                        if (!APIC) { vals[3] = vx; vals[4] = vy; vals[5] = vz; }
+                        if (!APIC) { vals[3] = wx - vx; vals[4] = wy - vy; vals[5] = wz - vz; }
                     }
                 }
 #pragma unroll
@@ -697,7 +697,7 @@ __global__ void __launch_bounds__(WG) __attribute__((amdgpu_waves_per_eu(G2PWave
 #pragma unroll
                 for (int a = 0; a < W; ++a) {
                     float S0[3] = {0.0f, 0.0f, 0.0f}, Sy[3] = {0.0f, 0.0f, 0.0f}, Sz[3] = {0.0f, 0.0f, 0.0f};
-                    float Sdy[3] = {0.0f, 0.0f, 0.0f}, Sdz[3] = {0.0f, 0.0f, 0.0f}, SF[3] = {0.0f, 0.0f, 0.0f};
+                    float Sdy[3] = {0.0f, 0.0f, 0.0f}, Sdz[3] = {0.0f, 0.0f, 0.0f};
 #pragma unroll
                     for (int b = 0; b < W; ++b)
 #pragma unroll
@@ -712,14 +712,11 @@ __global__ void __launch_bounds__(WG) __attribute__((amdgpu_waves_per_eu(G2PWave
                                 if (APIC) {
                                     Sdy[k] += (yz[b][c] * dxn[1][b]) * wv[k];
                                     Sdz[k] += (yz[b][c] * dxn[2][c]) * wv[k];
-                                } else {
-                                    SF[k] += yz[b][c] * (wv[k] - tile[(3 + k) * TN + q]);
                                 }
                             }
                         }
                     const float xa = Nx[a], dxa = dNx[a];
                     vpic0 += xa * S0[0]; vpic1 += xa * S0[1]; vpic2 += xa * S0[2];
-                    if (!APIC) { vf0 += xa * SF[0]; vf1 += xa * SF[1]; vf2 += xa * SF[2]; }
 #pragma unroll
                     for (int k = 0; k < 3; ++k) {
                         grad[k][0] += dxa * S0[k];
@@ -730,6 +727,24 @@ __global__ void __launch_bounds__(WG) __attribute__((amdgpu_waves_per_eu(G2PWave
                             Bm[k][1] += xa * Sdy[k];
                             Bm[k][2] += xa * Sdz[k];
                         }
+                    }
+                }
+                if (!APIC) {
+                    // FLIP increment sum over v* - v, a second pass over the
+                    // stencil so the main gather holds only the v* sums
+#pragma unroll
+                    for (int a = 0; a < W; ++a) {
+                        float SF[3] = {0.0f, 0.0f, 0.0f};
+#pragma unroll
+                        for (int b = 0; b < W; ++b)
+#pragma unroll
+                            for (int c = 0; c < W; ++c) {
+                                const int q = ((lz + c) * T + (ly + b)) * T + (lx + a);
+#pragma unroll
+                                for (int k = 0; k < 3; ++k) SF[k] += yz[b][c] * tile[(3 + k) * TN + q];
+                            }
+                        const float xa = Nx[a];
+                        vf0 += xa * SF[0]; vf1 += xa * SF[1]; vf2 += xa * SF[2];
                     }
                 }
             }
