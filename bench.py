@@ -174,7 +174,7 @@ def batch_bench(count, steps, warmup, seed=3):
     b = Batch(N.QUADRATIC_BSPLINE, N.APIC)
     for s in solvers:
         b.add(s)
-    tune = b.autotune(trial_steps=2)
+    tune = b.autotune(trial_steps=4)
     log(f"autotune: item_max={tune.item_max} g2p_threads={tune.g2p_threads} workgroups={tune.workgroups} "
         f"({tune.step_ms:.3f} ms/step)")
     out = {}

@@ -151,7 +151,7 @@ SceneEntry make_entry(const Solver& s, int64_t pstart, uint32_t koff, uint32_t i
     std::memcpy(&e.S, &s.S, sizeof(SceneDev));
     std::memcpy(&e.P, &s.P, sizeof(ParticlesDev));
     e.pstart = pstart;
-    e.active = s.acc;  // acc[0]: active-node sum over steps
+    e.active = (decltype(e.active))s.acc;  // acc[0]: active-node sum over steps
     e.key_offset = koff;
     e.index = index;
     return e;

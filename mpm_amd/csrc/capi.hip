@@ -224,7 +224,7 @@ int upload_materials(Solver& s) {
     if (rc) return rc;
     if (!s.mats.empty()) HIP_TRY(hipMemcpyAsync(s.mats_dev, s.mats.data(), s.mats.size() * sizeof(MaterialDev),
                                                 hipMemcpyHostToDevice, s.st));
-    s.P.mats = s.mats_dev;
+    s.P.mats = (decltype(s.P.mats))s.mats_dev;
     return MPM_OK;
 }
 
@@ -245,7 +245,7 @@ int alloc_particles(Solver& s, ParticlesDev& P, int64_t n) {
     if ((rc = dalloc(s, (void**)&P.tau, 6 * f))) return rc;
     if ((rc = dalloc(s, (void**)&P.mat, (size_t)std::max<int64_t>(n, 1) * 2))) return rc;
     if ((rc = dalloc(s, (void**)&P.pid, (size_t)std::max<int64_t>(n, 1) * 4))) return rc;
-    P.mats = s.mats_dev;
+    P.mats = (decltype(P.mats))s.mats_dev;
     return MPM_OK;
 }
 void free_particles(Solver& s, ParticlesDev& P) {

@@ -32,6 +32,18 @@ struct MaterialDev {
     int32_t model;
 };
 
+// Pointers into device memory inside the descriptors below.  In the device
+// pass they are typed as global-address-space pointers, so that accesses
+// through a descriptor read from memory (the batch scene table) compile to
+// global loads and stores; a generic pointer there would make every access
+// a flat access, which also occupies the LDS wait counter.  The host pass
+// sees plain pointers of the same size, so the layout is identical.
+#if defined(__HIP_DEVICE_COMPILE__)
+#define MPM_GLOBAL __attribute__((address_space(1)))
+#else
+#define MPM_GLOBAL
+#endif
+
 // Everything a kernel needs to know about one scene (grid + integrator
 // constants).  Passed by value.
 struct SceneDev {
@@ -51,26 +63,26 @@ struct SceneDev {
     uint32_t escape_key;     // kbx*kby*kbz*64
     int32_t particle_collision;
     int64_t plane;           // floats per grid field plane (= nbx*nby*nbz*64)
-    float* grid;             // GF_ALL (or GF_PRIMARY) planes
-    uint32_t* touched;       // one flag per storage block
+    float MPM_GLOBAL* grid;        // GF_ALL (or GF_PRIMARY) planes
+    uint32_t MPM_GLOBAL* touched;  // one flag per storage block
 };
 
 // Particle SoA (planar: component c of a vector field lives at base + c*N).
 struct ParticlesDev {
     int64_t n;
-    float* x;      // 3N
-    float* v;      // 3N
-    float* B;      // 9N (APIC) or null
-    float* Fe;     // 9N row-major
-    float* Fp;     // 9N
-    float* m;      // N
-    float* V0;     // N
-    float* mu0;    // N
-    float* lam0;   // N
-    float* tau;    // 6N cached -V0 * P(Fe) Fe^T (xx, xy, xz, yy, yz, zz) for the next P2G
-    uint16_t* mat; // N
-    uint32_t* pid; // N original particle id
-    const MaterialDev* mats;
+    float MPM_GLOBAL* x;      // 3N
+    float MPM_GLOBAL* v;      // 3N
+    float MPM_GLOBAL* B;      // 9N (APIC) or null
+    float MPM_GLOBAL* Fe;     // 9N row-major
+    float MPM_GLOBAL* Fp;     // 9N
+    float MPM_GLOBAL* m;      // N
+    float MPM_GLOBAL* V0;     // N
+    float MPM_GLOBAL* mu0;    // N
+    float MPM_GLOBAL* lam0;   // N
+    float MPM_GLOBAL* tau;    // 6N cached -V0 * P(Fe) Fe^T (xx, xy, xz, yy, yz, zz) for the next P2G
+    uint16_t MPM_GLOBAL* mat; // N
+    uint32_t MPM_GLOBAL* pid; // N original particle id
+    const MaterialDev MPM_GLOBAL* mats;
 };
 
 // One scene as the item kernels see it.  A batch sorts the particles of
@@ -81,7 +93,7 @@ struct SceneEntry {
     SceneDev S;
     ParticlesDev P;
     int64_t pstart;
-    unsigned long long* active;  // the scene's active-node counter
+    unsigned long long MPM_GLOBAL* active;  // the scene's active-node counter
     uint32_t key_offset;
     uint32_t index;
 };

@@ -937,15 +937,25 @@ __global__ void __launch_bounds__(256) k_clear_batch(const SceneEntry* __restric
         t = tab[sc].S.touched[gb - block_start[sc]] != 0u;
     }
     uint64_t todo = __ballot(t);
+    // Blocks are in scene order, so a wave's touched blocks form a few runs
+    // of one scene each: their counts are summed per run and added with one
+    // atomic per run (one atomic per block on a single scene's counter
+    // serialises at the L2).
+    int cur = -1;
+    unsigned long long run = 0;
     while (todo) {
         const int l = __builtin_ctzll(todo);
         todo &= todo - 1;
         const int s = __builtin_amdgcn_readfirstlane(__shfl(sc, l, 64));
+        if (s != cur) {
+            if (lane == 0 && run) atomicAdd(tab[cur].active, run);
+            cur = s;
+            run = 0;
+        }
         const int64_t b = (gb - lane + l) - block_start[s];
-        const unsigned long long c =
-            clear_block(tab[s].S.grid, tab[s].S.plane, tab[s].S.touched, b, nfields, lane);
-        if (lane == 0 && c) atomicAdd(tab[s].active, c);
+        run += clear_block(tab[s].S.grid, tab[s].S.plane, tab[s].S.touched, b, nfields, lane);
     }
+    if (lane == 0 && run) atomicAdd(tab[cur].active, run);
 }
 
 static unsigned grid_pass_wgs(int64_t nsb) {
