@@ -264,6 +264,10 @@ __device__ __forceinline__ void stress_from_svd(const float U[3][3], const float
 // updateDeformationGradient, src/material_point3d.cu:53-70: trial Fe, SVD,
 // clamp singular values to [1-theta_c, 1+theta_s], split into Fe and Fp.
 // Returns the SVD of the new Fe (U, clamped s, V) for the stress.
+// The reference's Fp = V diag(1/sc) U^T (Fn Fp) is evaluated as
+// V diag(s/sc) V^T Fp, using U^T Fn = diag(s) V^T for the SVD Fn = U diag(s)
+// V^T: the same matrix in exact arithmetic, two 3x3 products instead of four,
+// and Fn Fp need not be kept through the SVD.
 __device__ __forceinline__ void update_deformation(float Fe[3][3], float Fp[3][3], const float grad[3][3], float dt,
                                                    const MaterialDev& mat, float U[3][3], float s[3],
                                                    float V[3][3]) {
@@ -273,28 +277,31 @@ __device__ __forceinline__ void update_deformation(float Fe[3][3], float Fp[3][3
 #pragma unroll
         for (int j = 0; j < 3; ++j) L[i][j] = (i == j ? 1.0f : 0.0f) + dt * grad[i][j];
     m3_mul(L, Fe, Fn);
-    float F[3][3];
-    m3_mul(Fn, Fp, F);
     svd3(Fn, U, s, V);
     const float hi = 1.0f + mat.crit_s, lo = 1.0f - mat.crit_c;
-    float is[3];
+    float r[3];
 #pragma unroll
     for (int i = 0; i < 3; ++i) {
-        s[i] = fmaxf(fminf(s[i], hi), lo);
-        is[i] = mpm_rcp(s[i]);
+        const float sc = fmaxf(fminf(s[i], hi), lo);
+        r[i] = s[i] * mpm_rcp(sc);
+        s[i] = sc;
     }
-    float US[3][3], VSi[3][3];
+    float US[3][3], VR[3][3];
 #pragma unroll
     for (int i = 0; i < 3; ++i)
 #pragma unroll
         for (int j = 0; j < 3; ++j) {
             US[i][j] = U[i][j] * s[j];
-            VSi[i][j] = V[i][j] * is[j];
+            VR[i][j] = V[i][j] * r[j];
         }
     m3_mul_bt(US, V, Fe);
-    float VSiUt[3][3];
-    m3_mul_bt(VSi, U, VSiUt);
-    m3_mul(VSiUt, F, Fp);
+    float M[3][3], Fp_old[3][3];
+#pragma unroll
+    for (int i = 0; i < 3; ++i)
+#pragma unroll
+        for (int j = 0; j < 3; ++j) Fp_old[i][j] = Fp[i][j];
+    m3_mul_bt(VR, V, M);
+    m3_mul(M, Fp_old, Fp);
 }
 
 __device__ __forceinline__ void load_m3(const float* base, int64_t N, uint32_t p, float M[3][3]) {
