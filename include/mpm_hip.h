@@ -270,7 +270,9 @@ MPM_API int mpm_batch_create(int32_t interp, int32_t transfer, void** out_batch)
 /* Removes the members (they stay valid scene handles) and frees the batch. */
 MPM_API void mpm_batch_destroy(void* batch);
 /* MPM_ERR_INVALID if the scene's interp/transfer differ from the batch's,
- * MPM_ERR_STATE if it is already in a batch. */
+ * MPM_ERR_STATE if it is already in a batch, MPM_ERR_UNSUPPORTED if the
+ * members' cell keys would exceed the 32-bit batch key space (each scene
+ * takes (((rx+3)>>2)+1)*(((ry+3)>>2)+1)*(((rz+3)>>2)+1)*64 + 64 keys). */
 MPM_API int mpm_batch_add(void* batch, void* scene);
 MPM_API int mpm_batch_remove(void* batch, void* scene);
 MPM_API int mpm_batch_info(void* batch, mpm_batch_stats* out);

@@ -452,6 +452,11 @@ MPM_API int mpm_batch_add(void* batch, void* scene) {
     if (s->batch) return fail(MPM_ERR_STATE, "scene is already a member of a batch");
     if (s->interp != b->interp || s->apic != b->apic)
         return fail(MPM_ERR_INVALID, "scene interpolation/transfer differs from the batch's");
+    uint64_t keys = (uint64_t)s->S.escape_key + 64u;
+    for (const Solver* m : b->members) keys += (uint64_t)m->S.escape_key + 64u;
+    if (keys > (1ull << 32))
+        return fail(MPM_ERR_UNSUPPORTED, "batch key space exhausted: the members' grids would total more than 2^32 "
+                                         "cell keys");
     // Finish the scene's own work, then move it onto the batch stream.
     if (s->grid_dirty && (!s->count_on_clear || s->vstar_dirty)) run_clear(*s);
     ensure_stress(*s);

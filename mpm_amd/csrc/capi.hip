@@ -963,18 +963,21 @@ MPM_API int mpm_profile_read(void* handle, mpm_kernel_time* out, int max, int* c
 MPM_API int mpm_debug_svd3(const float* A, int64_t n, float* U, float* sv, float* V) {
     if (n < 0 || (n > 0 && (!A || !U || !sv || !V))) return fail(MPM_ERR_INVALID, "bad arguments");
     if (n == 0) return MPM_OK;
-    float *dA = nullptr, *dU = nullptr, *dS = nullptr, *dV = nullptr;
-    HIP_TRY(hipMalloc((void**)&dA, (size_t)n * 36));
-    HIP_TRY(hipMalloc((void**)&dU, (size_t)n * 36));
-    HIP_TRY(hipMalloc((void**)&dS, (size_t)n * 12));
-    HIP_TRY(hipMalloc((void**)&dV, (size_t)n * 36));
-    HIP_TRY(hipMemcpy(dA, A, (size_t)n * 36, hipMemcpyHostToDevice));
-    launch_debug_svd(dA, n, dU, dS, dV, nullptr);
-    HIP_TRY(hipGetLastError());
-    HIP_TRY(hipMemcpy(U, dU, (size_t)n * 36, hipMemcpyDeviceToHost));
-    HIP_TRY(hipMemcpy(sv, dS, (size_t)n * 12, hipMemcpyDeviceToHost));
-    HIP_TRY(hipMemcpy(V, dV, (size_t)n * 36, hipMemcpyDeviceToHost));
-    (void)hipFree(dA); (void)hipFree(dU); (void)hipFree(dS); (void)hipFree(dV);
+    float* d[4] = {nullptr, nullptr, nullptr, nullptr};  // A, U, s, V
+    const size_t bytes[4] = {(size_t)n * 36, (size_t)n * 36, (size_t)n * 12, (size_t)n * 36};
+    hipError_t e = hipSuccess;
+    for (int i = 0; i < 4 && e == hipSuccess; ++i) e = hipMalloc((void**)&d[i], bytes[i]);
+    if (e == hipSuccess) e = hipMemcpy(d[0], A, bytes[0], hipMemcpyHostToDevice);
+    if (e == hipSuccess) {
+        launch_debug_svd(d[0], n, d[1], d[2], d[3], nullptr);
+        e = hipGetLastError();
+    }
+    if (e == hipSuccess) e = hipMemcpy(U, d[1], bytes[1], hipMemcpyDeviceToHost);
+    if (e == hipSuccess) e = hipMemcpy(sv, d[2], bytes[2], hipMemcpyDeviceToHost);
+    if (e == hipSuccess) e = hipMemcpy(V, d[3], bytes[3], hipMemcpyDeviceToHost);
+    for (float* p : d)
+        if (p) (void)hipFree(p);
+    if (e != hipSuccess) return fail(MPM_ERR_HIP, std::string("debug svd: ") + hipGetErrorString(e));
     return MPM_OK;
 }
 
