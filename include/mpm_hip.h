@@ -36,7 +36,8 @@ enum {
     MPM_ERR_OOM = -3,         /* device allocation failed / budget exceeded */
     MPM_ERR_UNSUPPORTED = -4, /* combination the solver rejects             */
     MPM_ERR_STATE = -5,       /* call not valid in the current state        */
-    MPM_ERR_NOT_FOUND = -6    /* unknown scene id                           */
+    MPM_ERR_NOT_FOUND = -6,   /* unknown scene id                           */
+    MPM_ERR_IO = -7           /* file could not be written                  */
 };
 
 /* ---- enums: same values and order as reference include/enums.h:6-15 */
@@ -215,6 +216,52 @@ MPM_API int mpm_read_grid(void* handle, mpm_grid_host* out);
  * host buffer of 4*count floats (reference updateGLBufferByCPU,
  * src/mpm_solver3d.cu:484-508, without the GL buffer). */
 MPM_API int mpm_read_positions4(void* handle, float* out_xyz1);
+/* The same pack into a caller-owned device buffer of 4*count floats (for
+ * example a vertex buffer mapped for interop), asynchronously on the
+ * solver's stream: updateGLBufferWithCUDA (src/mpm_solver3d.cu:457-482)
+ * without the GL registration. */
+MPM_API int mpm_pack_positions4(void* handle, float* device_out);
+/* Frame export that overlaps later steps: the pack runs on the solver's
+ * stream into a device staging buffer and the device-to-host copy into
+ * host_out (4*count floats) on a second stream.  host_out must stay valid
+ * until mpm_positions4_wait returns; page-locked memory (hipHostMalloc)
+ * makes the copy asynchronous.  A second export waits on the device for
+ * the first copy before reusing the staging buffer. */
+MPM_API int mpm_positions4_async(void* handle, float* host_out);
+MPM_API int mpm_positions4_wait(void* handle);
+
+/* ---------------------------------------------------- density raster
+ * Replacement of writeToOpenVDB (include/mpm_solver3d.h:73,
+ * src/mpm_solver3d.cu:510-572), which rasterises on the host into an
+ * OpenVDB FloatGrid named "density".  Here the raster runs on the device:
+ * density(node) = sum over particles of mass * w / h^3, over the current
+ * positions, the stencil range of P2G (origin ignored, as in the
+ * reference) and the nodes inside [0, res); nodes no particle reaches hold
+ * 0, the reference grid's background value.  It is recomputed only when
+ * the particles changed since the last raster; the reads synchronise. */
+MPM_API int mpm_density_compute(void* handle);  /* asynchronous */
+/* Dense copy: res.x*res.y*res.z floats, node (x,y,z) at x + rx*(y + ry*z). */
+MPM_API int mpm_read_density(void* handle, float* out);
+/* Sparse copy: the 4x4x4 node blocks the raster touched, in storage order.
+ * *count receives their number; with origins and values both NULL nothing
+ * else is written.  Otherwise block b's first node goes to origins[3b..3b+2]
+ * and its 64 densities to values[64b..64b+63], node origin + (i,j,k) at
+ * i + 4j + 16k (nodes beyond res hold 0); MPM_ERR_INVALID if
+ * max_blocks < *count. */
+MPM_API int mpm_read_density_blocks(void* handle, int32_t* origins, float* values, int64_t max_blocks,
+                                    int64_t* count);
+/* Write the sparse blocks to a file (OpenVDB is not used), little-endian:
+ *   0  char[8]  "MPMDENS1"
+ *   8  int32[3] resolution
+ *  20  float[3] grid origin
+ *  32  float    stride h
+ *  36  int32    block edge (4)
+ *  40  int64    block count B
+ *  48  B records { int32 first node[3]; float density[64] } as above.
+ * Node (x,y,z) sits at origin + h*(x,y,z); MPM_ERR_IO on a write error. */
+#define MPM_DENSITY_MAGIC "MPMDENS1"
+MPM_API int mpm_write_density(void* handle, const char* path);
+
 /* Last sort: keys[i] and original particle id ids[i] of sorted position i;
  * seq_ids[j] = original id of the particle at input position j of that sort
  * (the sequence the stable sort was applied to).  Any pointer may be NULL. */

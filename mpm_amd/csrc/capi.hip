@@ -447,6 +447,12 @@ void destroy(Solver* s) {
     if (!s) return;
     if (s->batch) batch_forget(s->batch, s);
     if (s->st) (void)hipStreamSynchronize(s->st);
+    if (s->copy_st) {
+        (void)hipStreamSynchronize(s->copy_st);
+        (void)hipStreamDestroy(s->copy_st);
+    }
+    if (s->pos4_packed) (void)hipEventDestroy(s->pos4_packed);
+    if (s->pos4_copied) (void)hipEventDestroy(s->pos4_copied);
     if (s->gexec) (void)hipGraphExecDestroy(s->gexec);
     if (s->graph) (void)hipGraphDestroy(s->graph);
     for (auto& p : s->pending) s->event_pool.push_back(p.a), s->event_pool.push_back(p.b);
@@ -855,6 +861,179 @@ MPM_API int mpm_read_positions4(void* handle, float* out) {
     dfree(*s, d);
     if (e != hipSuccess) return fail(MPM_ERR_HIP, std::string("read positions: ") + hipGetErrorString(e));
     return finish(*s);
+}
+
+// ---- render export (updateGLBufferWithCUDA / updateGLBufferByCPU,
+// src/mpm_solver3d.cu:457-508)
+
+MPM_API int mpm_pack_positions4(void* handle, float* device_out) {
+    GET_SOLVER(s, handle);
+    if (!device_out) return fail(MPM_ERR_INVALID, "device_out is NULL");
+    launch_positions4(s->P, device_out, s->st);
+    return finish(*s);
+}
+
+MPM_API int mpm_positions4_async(void* handle, float* host_out) {
+    GET_SOLVER(s, handle);
+    if (!host_out) return fail(MPM_ERR_INVALID, "host_out is NULL");
+    if (s->N == 0) return MPM_OK;
+    if (!s->pos4) {
+        if (!s->copy_st) HIP_TRY(hipStreamCreateWithFlags(&s->copy_st, hipStreamNonBlocking));
+        if (!s->pos4_packed) HIP_TRY(hipEventCreateWithFlags(&s->pos4_packed, hipEventDisableTiming));
+        if (!s->pos4_copied) HIP_TRY(hipEventCreateWithFlags(&s->pos4_copied, hipEventDisableTiming));
+        int rc = dalloc(*s, (void**)&s->pos4, (size_t)s->N * 16);
+        if (rc) return rc;
+    }
+    // the staging buffer is reused once the previous copy has read it
+    if (s->pos4_pending) HIP_TRY(hipStreamWaitEvent(s->st, s->pos4_copied, 0));
+    launch_positions4(s->P, s->pos4, s->st);
+    HIP_TRY(hipEventRecord(s->pos4_packed, s->st));
+    HIP_TRY(hipStreamWaitEvent(s->copy_st, s->pos4_packed, 0));
+    HIP_TRY(hipMemcpyAsync(host_out, s->pos4, (size_t)s->N * 16, hipMemcpyDeviceToHost, s->copy_st));
+    HIP_TRY(hipEventRecord(s->pos4_copied, s->copy_st));
+    s->pos4_pending = true;
+    return finish(*s);
+}
+
+MPM_API int mpm_positions4_wait(void* handle) {
+    GET_SOLVER(s, handle);
+    if (s->pos4_pending) {
+        HIP_TRY(hipEventSynchronize(s->pos4_copied));
+        s->pos4_pending = false;
+    }
+    return MPM_OK;
+}
+
+// ---- density raster (writeToOpenVDB, src/mpm_solver3d.cu:510-572)
+
+// The raster's one-plane grid: the scene's geometry with the density
+// buffers in place of the simulation grid.
+static SceneDev density_scene(const Solver& s) {
+    SceneDev D = s.S;
+    D.grid = decltype(D.grid)(s.dens);
+    D.touched = decltype(D.touched)(s.dens_touched);
+    return D;
+}
+
+// Mass-only P2G of the current positions into the density plane (skipped
+// when the particles have not changed since the last raster).
+static int run_density(Solver& s) {
+    if (s.dens && s.dens_version == s.version) return MPM_OK;
+    ProfileHook h = hook(s);
+    Scope sc(h, "density");
+    if (!s.dens) {
+        const size_t sizes[4] = {(size_t)s.S.plane * 4, (size_t)s.nsb * 4, (size_t)(s.nsb + 1) * 4,
+                                 (size_t)scan_temp_words(s.nsb) * 4};
+        void* p[4] = {nullptr, nullptr, nullptr, nullptr};
+        for (int i = 0; i < 4; ++i) {
+            int rc = dalloc(s, &p[i], sizes[i]);
+            if (rc) {
+                for (int j = 0; j < i; ++j) dfree(s, p[j]);
+                return rc;
+            }
+        }
+        HIP_TRY(hipMemsetAsync(p[0], 0, sizes[0], s.st));
+        HIP_TRY(hipMemsetAsync(p[1], 0, sizes[1], s.st));
+        s.dens = (float*)p[0];
+        s.dens_touched = (uint32_t*)p[1];
+        s.dens_rank = (uint32_t*)p[2];
+        s.dens_temp = (uint32_t*)p[3];
+        s.dens_dirty = false;
+    }
+    const SceneDev D = density_scene(s);
+    if (s.dens_dirty) launch_clear(D, s.nsb, 1, nullptr, s.st);
+    if (s.N > 0) {
+        if (!s.sort_valid) run_sort(s);
+        launch_p2g(D, s.P, s.I, s.interp, s.apic, 1, s.item_wgs, s.st);
+    }
+    s.dens_dirty = true;
+    s.dens_version = s.version;
+    return MPM_OK;
+}
+
+MPM_API int mpm_density_compute(void* handle) {
+    GET_SOLVER(s, handle);
+    int rc = run_density(*s);
+    if (rc) return rc;
+    return finish(*s);
+}
+
+MPM_API int mpm_read_density(void* handle, float* out) {
+    GET_SOLVER(s, handle);
+    if (!out) return fail(MPM_ERR_INVALID, "out is NULL");
+    int rc = run_density(*s);
+    if (rc) return rc;
+    const int64_t total = (int64_t)s->S.rx * s->S.ry * s->S.rz;
+    float* d = nullptr;
+    if ((rc = dalloc(*s, (void**)&d, (size_t)total * 4))) return rc;
+    launch_density_dense(density_scene(*s), d, s->st);
+    hipError_t e = hipMemcpyAsync(out, d, (size_t)total * 4, hipMemcpyDeviceToHost, s->st);
+    if (e == hipSuccess) e = hipStreamSynchronize(s->st);
+    dfree(*s, d);
+    if (e != hipSuccess) return fail(MPM_ERR_HIP, std::string("read density: ") + hipGetErrorString(e));
+    return finish(*s);
+}
+
+// Touched blocks of the raster, compacted in storage-block order.
+static int density_blocks(Solver& s, std::vector<int32_t>* origins, std::vector<float>* values, int32_t* h_orig,
+                          float* h_vals, int64_t max_blocks, int64_t* count) {
+    int rc = run_density(s);
+    if (rc) return rc;
+    launch_touched_flags(s.dens_touched, s.nsb, s.dens_rank, s.st);
+    exclusive_scan(s.dens_rank, s.nsb, s.dens_temp, s.dens_rank + s.nsb, s.st);
+    uint32_t total = 0;
+    HIP_TRY(hipMemcpyAsync(&total, s.dens_rank + s.nsb, 4, hipMemcpyDeviceToHost, s.st));
+    HIP_TRY(hipStreamSynchronize(s.st));
+    if (count) *count = (int64_t)total;
+    if (origins) { origins->resize((size_t)total * 3); h_orig = origins->data(); }
+    if (values) { values->resize((size_t)total * 64); h_vals = values->data(); max_blocks = total; }
+    if (!h_orig && !h_vals) return finish(s);
+    if (max_blocks < (int64_t)total)
+        return fail(MPM_ERR_INVALID, "max_blocks (" + std::to_string(max_blocks) + ") < touched blocks (" +
+                                         std::to_string(total) + ")");
+    if (total == 0) return finish(s);
+    int32_t* d_orig = nullptr;
+    float* d_vals = nullptr;
+    if ((rc = dalloc(s, (void**)&d_orig, (size_t)total * 12))) return rc;
+    if ((rc = dalloc(s, (void**)&d_vals, (size_t)total * 256))) { dfree(s, d_orig); return rc; }
+    launch_density_blocks(density_scene(s), s.dens_touched, s.dens_rank, s.nsb, d_orig, d_vals, total, s.st);
+    hipError_t e = hipSuccess;
+    if (h_orig) e = hipMemcpyAsync(h_orig, d_orig, (size_t)total * 12, hipMemcpyDeviceToHost, s.st);
+    if (e == hipSuccess && h_vals) e = hipMemcpyAsync(h_vals, d_vals, (size_t)total * 256, hipMemcpyDeviceToHost, s.st);
+    if (e == hipSuccess) e = hipStreamSynchronize(s.st);
+    dfree(s, d_orig);
+    dfree(s, d_vals);
+    if (e != hipSuccess) return fail(MPM_ERR_HIP, std::string("read density blocks: ") + hipGetErrorString(e));
+    return finish(s);
+}
+
+MPM_API int mpm_read_density_blocks(void* handle, int32_t* origins, float* values, int64_t max_blocks,
+                                    int64_t* count) {
+    GET_SOLVER(s, handle);
+    if (!count) return fail(MPM_ERR_INVALID, "count is NULL");
+    return density_blocks(*s, nullptr, nullptr, origins, values, max_blocks, count);
+}
+
+MPM_API int mpm_write_density(void* handle, const char* path) {
+    GET_SOLVER(s, handle);
+    if (!path) return fail(MPM_ERR_INVALID, "path is NULL");
+    std::vector<int32_t> org;
+    std::vector<float> val;
+    int64_t nb = 0;
+    int rc = density_blocks(*s, &org, &val, nullptr, nullptr, 0, &nb);
+    if (rc) return rc;
+    FILE* f = std::fopen(path, "wb");
+    if (!f) return fail(MPM_ERR_IO, std::string("cannot open ") + path + " for writing");
+    const int32_t res[3] = {s->S.rx, s->S.ry, s->S.rz};
+    const float geo[4] = {s->desc.origin[0], s->desc.origin[1], s->desc.origin[2], s->desc.stride};
+    const int32_t edge = MPM_BLOCK;
+    bool ok = std::fwrite(MPM_DENSITY_MAGIC, 1, 8, f) == 8 && std::fwrite(res, 4, 3, f) == 3 &&
+              std::fwrite(geo, 4, 4, f) == 4 && std::fwrite(&edge, 4, 1, f) == 1 && std::fwrite(&nb, 8, 1, f) == 1;
+    for (int64_t b = 0; ok && b < nb; ++b)
+        ok = std::fwrite(&org[(size_t)b * 3], 4, 3, f) == 3 && std::fwrite(&val[(size_t)b * 64], 4, 64, f) == 64;
+    ok = (std::fclose(f) == 0) && ok;
+    if (!ok) return fail(MPM_ERR_IO, std::string("write failed: ") + path);
+    return MPM_OK;
 }
 
 MPM_API int mpm_read_sort(void* handle, uint32_t* keys, uint32_t* ids, uint32_t* seq, int64_t count) {

@@ -1185,6 +1185,59 @@ void launch_grid_dense(const SceneDev& S, float* mass, float* vel, float* vstar,
                        force, have_vstar);
 }
 
+// ------------------------------------------------------- density raster
+// writeToOpenVDB's density (src/mpm_solver3d.cu:510-563) is the mass-only
+// P2G of the current positions times 1/h^3; these kernels read it out of a
+// one-plane block-major grid D (D.grid = sum of m w per node).
+__global__ void k_density_dense(SceneDev D, float* __restrict__ out) {
+    int64_t i = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+    const int64_t total = (int64_t)D.rx * D.ry * D.rz;
+    if (i >= total) return;
+    const int x = (int)(i % D.rx), y = (int)((i / D.rx) % D.ry), z = (int)(i / ((int64_t)D.rx * D.ry));
+    out[i] = D.grid[node_index(D, x, y, z)] * D.inv_cell_vol;
+}
+void launch_density_dense(const SceneDev& D, float* out, hipStream_t st) {
+    const int64_t total = (int64_t)D.rx * D.ry * D.rz;
+    if (total <= 0) return;
+    hipLaunchKernelGGL(k_density_dense, dim3((unsigned)((total + 255) / 256)), dim3(256), 0, st, D, out);
+}
+
+__global__ void k_touched_flags(const uint32_t* __restrict__ touched, int64_t n, uint32_t* __restrict__ flags) {
+    int64_t i = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+    if (i < n) flags[i] = touched[i] != 0u ? 1u : 0u;
+}
+void launch_touched_flags(const uint32_t* touched, int64_t n, uint32_t* flags, hipStream_t st) {
+    if (n <= 0) return;
+    hipLaunchKernelGGL(k_touched_flags, dim3((unsigned)((n + 255) / 256)), dim3(256), 0, st, touched, n, flags);
+}
+
+// One 64-lane wave per storage block: a touched block b of rank r (exclusive
+// scan of the touched flags) writes its node origin to origin[3r..3r+2] and
+// its 64 densities (x fastest) to values[64r..64r+63]; ranks >= max_blocks
+// are dropped (the caller sizes the output from the count).
+__global__ void __launch_bounds__(256) k_density_blocks(SceneDev D, const uint32_t* __restrict__ touched,
+                                                        const uint32_t* __restrict__ rank, int64_t nsb,
+                                                        int32_t* __restrict__ origin, float* __restrict__ values,
+                                                        int64_t max_blocks) {
+    const int64_t t = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+    const int64_t b = t >> 6;
+    const int l = (int)(t & 63);
+    if (b >= nsb || touched[b] == 0u) return;
+    const int64_t r = rank[b];
+    if (r >= max_blocks) return;
+    if (values) values[r * 64 + l] = D.grid[b * 64 + l] * D.inv_cell_vol;
+    if (origin && l < 3) {
+        const int64_t bx = b % D.nbx, by = (b / D.nbx) % D.nby, bz = b / ((int64_t)D.nbx * D.nby);
+        origin[3 * r + l] = 4 * (int32_t)(l == 0 ? bx : (l == 1 ? by : bz));
+    }
+}
+void launch_density_blocks(const SceneDev& D, const uint32_t* touched, const uint32_t* rank, int64_t nsb,
+                           int32_t* origin, float* values, int64_t max_blocks, hipStream_t st) {
+    if (nsb <= 0 || max_blocks <= 0) return;
+    hipLaunchKernelGGL(k_density_blocks, dim3((unsigned)((nsb * 64 + 255) / 256)), dim3(256), 0, st, D, touched,
+                       rank, nsb, origin, values, max_blocks);
+}
+
 __global__ void __launch_bounds__(256) k_count_escaped(const uint32_t* __restrict__ sk, int64_t n, uint32_t esc,
                                                        unsigned long long* out) {
     unsigned long long cnt = 0;

@@ -88,6 +88,13 @@ void launch_permute(const ParticlesDev& src, const ParticlesDev& dst, const uint
 void launch_gather_u32(const uint32_t* src, const uint32_t* idx, uint32_t* dst, int64_t n, hipStream_t st);
 void launch_grid_dense(const SceneDev& S, float* mass, float* vel, float* vstar, float* force, int have_vstar,
                        hipStream_t st);
+// Density raster readout from a one-plane mass grid D (D.grid = sum m w):
+// dense in node order x + rx (y + ry z), or the touched storage blocks
+// compacted by rank (exclusive scan of launch_touched_flags' output).
+void launch_density_dense(const SceneDev& D, float* out, hipStream_t st);
+void launch_touched_flags(const uint32_t* touched, int64_t n, uint32_t* flags, hipStream_t st);
+void launch_density_blocks(const SceneDev& D, const uint32_t* touched, const uint32_t* rank, int64_t nsb,
+                           int32_t* origin, float* values, int64_t max_blocks, hipStream_t st);
 void launch_count_escaped(const uint32_t* sorted_keys, int64_t n, uint32_t escape_key, unsigned long long* out,
                           hipStream_t st);
 void launch_debug_svd(const float* A, int64_t n, float* U, float* s, float* V, hipStream_t st);

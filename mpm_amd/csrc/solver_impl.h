@@ -87,6 +87,21 @@ struct Solver {
     // incremented whenever particle positions change (steps, advection,
     // state writes): a batch recomputes the member's sort keys on a change
     uint64_t version = 0;
+    // density raster (allocated on first use): one mass plane with its own
+    // touched flags, block ranks (nsb + 1 words, the last is the count) and
+    // scan scratch; valid while dens_version == version
+    float* dens = nullptr;
+    uint32_t* dens_touched = nullptr;
+    uint32_t* dens_rank = nullptr;
+    uint32_t* dens_temp = nullptr;
+    bool dens_dirty = false;
+    uint64_t dens_version = ~0ull;
+    // render export (allocated on first use): float4 positions staged on
+    // the device, copied to the host on a second stream
+    float* pos4 = nullptr;
+    hipStream_t copy_st = nullptr;
+    hipEvent_t pos4_packed = nullptr, pos4_copied = nullptr;
+    bool pos4_pending = false;
 };
 
 

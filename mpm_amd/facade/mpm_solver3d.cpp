@@ -134,6 +134,9 @@ void MPMSolverHandle::particlesCollision(float dt) { check(mpm_particles_collisi
 void MPMSolverHandle::copyPositionsToHost(float* xyz1) const {
     check(mpm_read_positions4(_h, xyz1), "copyPositionsToHost");
 }
+void MPMSolverHandle::writeToOpenVDB(std::string filePath) {
+    check(mpm_write_density(_h, filePath.c_str()), "writeToOpenVDB");
+}
 void MPMSolverHandle::synchronize() const { check(mpm_synchronize(_h), "synchronize"); }
 
 void MPMSolverHandle::readState(std::vector<MaterialPoint3D>& out, std::vector<Matrix3f>* B) const {

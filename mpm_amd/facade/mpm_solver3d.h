@@ -6,8 +6,10 @@
 //  * errors throw std::runtime_error instead of exit(EXIT_FAILURE);
 //  * the class is move-only (the reference's implicit copy double-frees);
 //  * steps are asynchronous; every read synchronises;
-//  * the OpenGL / OpenVDB methods are replaced by host-buffer equivalents
-//    (copyPositionsToHost, readParticles).
+//  * the OpenGL methods are replaced by host-buffer equivalents
+//    (copyPositionsToHost, readParticles);
+//  * writeToOpenVDB keeps its name and density raster but writes the
+//    density block file of mpm_write_density (OpenVDB is not used).
 #ifndef CHAINS_MPM_SOLVER_3D_H_
 #define CHAINS_MPM_SOLVER_3D_H_
 
@@ -47,6 +49,10 @@ public:
     // Replacement of updateGLBufferByCPU / updateGLBufferWithCUDA
     // (src/mpm_solver3d.cu:457-508): positions packed as (x, y, z, 1).
     void copyPositionsToHost(float* xyz1) const;
+    // writeToOpenVDB (src/mpm_solver3d.cu:510-572): the density raster
+    // m w / h^3 of the current positions, computed on the device and
+    // written as the 4x4x4-block file of mpm_write_density.
+    void writeToOpenVDB(std::string filePath);
     void synchronize() const;
     int particleCount() const { return _n; }
     void* handle() const { return _h; }

@@ -19,6 +19,7 @@ MPM_ERR_OOM = -3
 MPM_ERR_UNSUPPORTED = -4
 MPM_ERR_STATE = -5
 MPM_ERR_NOT_FOUND = -6
+MPM_ERR_IO = -7
 
 # enums (reference include/enums.h:6-15)
 LINEAR, QUADRATIC_BSPLINE, CUBIC_BSPLINE = 0, 1, 2
@@ -161,6 +162,14 @@ SIGNATURES = {
     "mpm_write_particles": (C.c_int, [C.c_void_p, C.POINTER(MpmParticleState)]),
     "mpm_read_grid": (C.c_int, [C.c_void_p, C.POINTER(MpmGridHost)]),
     "mpm_read_positions4": (C.c_int, [C.c_void_p, _fp]),
+    "mpm_pack_positions4": (C.c_int, [C.c_void_p, C.c_void_p]),
+    "mpm_positions4_async": (C.c_int, [C.c_void_p, _fp]),
+    "mpm_positions4_wait": (C.c_int, [C.c_void_p]),
+    "mpm_density_compute": (C.c_int, [C.c_void_p]),
+    "mpm_read_density": (C.c_int, [C.c_void_p, _fp]),
+    "mpm_read_density_blocks": (C.c_int, [C.c_void_p, C.POINTER(C.c_int32), _fp, C.c_int64,
+                                          C.POINTER(C.c_int64)]),
+    "mpm_write_density": (C.c_int, [C.c_void_p, C.c_char_p]),
     "mpm_read_sort": (C.c_int, [C.c_void_p, _up, _up, _up, C.c_int64]),
     "mpm_host_keys": (C.c_int, [C.c_void_p, _fp, C.c_int64, _up]),
     "mpm_get_stats": (C.c_int, [C.c_void_p, C.POINTER(MpmStats)]),

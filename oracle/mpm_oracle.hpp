@@ -21,6 +21,9 @@
 //   initialize      lattice closed form V0 = h^3 / ppc
 //   step            conservation, single-particle free fall, FP32 vs FP64
 //   keys / sort     std::stable_sort (integer, exact)
+//   density raster  closed forms (one particle on a node), lattice density
+//                   ppc m / h^3, sum density h^3 = sum m, an independent
+//                   numpy raster
 // The third-party SVD the reference calls (3x3_SVD_CUDA, unpinned version,
 // submodule absent) and Eigen's evaluation order are not reproducible
 // bit-for-bit; parity with them is tolerance-level only.
@@ -613,6 +616,46 @@ struct Solver {
         ++steps;
     }
 };
+
+// --------------------------------------------------------- density raster
+// writeToOpenVDB's raster, src/mpm_solver3d.cu:510-563 (host code): for each
+// particle in array order and each node of its stencil range (origin
+// ignored, as in P2G) inside [0, res), density += mass * w * cell_vol_inv
+// with cell_vol_inv = 1.0 / (h*h*h) rounded to R, w from the grid node's
+// world position.  out: res_x*res_y*res_z values in node order
+// x + rx (y + ry z), 0 where no particle reaches (the grid's background);
+// active (optional) flags the nodes the reference's accessor writes.
+template <class R>
+void density_raster(const mpm_desc& d, const float* pos, const float* mass, int64_t n, std::vector<R>& out,
+                    std::vector<uint8_t>* active) {
+    const Interpolator<R> interp(d.interp);
+    const int res[3] = {d.resolution[0], d.resolution[1], d.resolution[2]};
+    const R h = R(d.stride);
+    const R rng = interp.range;
+    const V3<R> ori = v3<R>(R(d.origin[0]), R(d.origin[1]), R(d.origin[2]));
+    const R cell_vol_inv = R(1.0 / (h * h * h));
+    const size_t nn = size_t(res[0]) * res[1] * res[2];
+    out.assign(nn, R(0));
+    if (active) active->assign(nn, 0);
+    for (int64_t p = 0; p < n; ++p) {
+        const V3<R> x = v3<R>(R(pos[3 * p]), R(pos[3 * p + 1]), R(pos[3 * p + 2]));
+        int lo[3], hi[3];
+        for (int i = 0; i < 3; ++i) {
+            lo[i] = int(std::ceil(x[i] / h - rng));
+            hi[i] = int(std::floor(x[i] / h + rng));
+        }
+        for (int gx = lo[0]; gx <= hi[0]; ++gx)
+            for (int gy = lo[1]; gy <= hi[1]; ++gy)
+                for (int gz = lo[2]; gz <= hi[2]; ++gz) {
+                    if (gx < 0 || gx >= res[0] || gy < 0 || gy >= res[1] || gz < 0 || gz >= res[2]) continue;
+                    const size_t id = size_t(gx) + size_t(res[0]) * (size_t(gy) + size_t(res[1]) * size_t(gz));
+                    const V3<R> gp = add(ori, v3<R>(h * R(gx), h * R(gy), h * R(gz)));
+                    const R w = interp.weight3D(x, gp, h);
+                    out[id] = out[id] + R(mass[p]) * w * cell_vol_inv;
+                    if (active) (*active)[id] = 1;
+                }
+    }
+}
 
 // ------------------------------------------------ particle-by-cell sort key
 // No reference counterpart (the reference never sorts).  The key of a

@@ -304,6 +304,33 @@ __attribute__((visibility("default"))) void orc_collide(const float* pos, const 
     out[0] = r[0]; out[1] = r[1]; out[2] = r[2];
 }
 
+// Density raster of n particles (writeToOpenVDB's, FP32 or FP64): dense
+// out[rx*ry*rz] in node order; active[rx*ry*rz] (may be null) flags the
+// nodes the reference writes.
+__attribute__((visibility("default"))) void orc_density(const mpm_desc* d, const float* pos, const float* mass,
+                                                        int64_t n, double* out, uint8_t* active, int precision) {
+    std::vector<uint8_t> act;
+    auto run = [&](auto tag) {
+        using R = decltype(tag);
+        std::vector<R> v;
+        density_raster<R>(*d, pos, mass, n, v, active ? &act : nullptr);
+        for (size_t i = 0; i < v.size(); ++i) out[i] = double(v[i]);
+    };
+    if (precision == 64) run(double(0)); else run(float(0));
+    if (active) std::memcpy(active, act.data(), act.size());
+}
+
+// Wall-clock seconds of one FP32 raster (the CPU baseline of the density
+// pass: the reference rasterises on the host).
+__attribute__((visibility("default"))) double orc_time_density(const mpm_desc* d, const float* pos,
+                                                               const float* mass, int64_t n) {
+    std::vector<float> v;
+    auto t0 = std::chrono::steady_clock::now();
+    density_raster<float>(*d, pos, mass, n, v, nullptr);
+    auto t1 = std::chrono::steady_clock::now();
+    return std::chrono::duration<double>(t1 - t0).count();
+}
+
 // Cell keys for n positions under a descriptor.
 __attribute__((visibility("default"))) void orc_keys(const mpm_desc* d, const float* pos, int64_t n, uint32_t* keys) {
     KeyParams k = key_params(*d);

@@ -49,6 +49,10 @@ def load():
     lib.orc_weights.argtypes = [C.c_int, _dp, C.c_int64, _dp, _dp, C.c_int]
     lib.orc_stress.argtypes = [_dp, _dp, C.c_double, C.c_double, C.c_double, C.c_int, _dp, C.c_int]
     lib.orc_collide.argtypes = [C.POINTER(C.c_float)] * 4 + [C.c_float, C.POINTER(C.c_float)]
+    lib.orc_density.argtypes = [C.POINTER(N.MpmDesc), C.POINTER(C.c_float), C.POINTER(C.c_float), C.c_int64, _dp,
+                                C.POINTER(C.c_uint8), C.c_int]
+    lib.orc_time_density.restype = C.c_double
+    lib.orc_time_density.argtypes = [C.POINTER(N.MpmDesc), C.POINTER(C.c_float), C.POINTER(C.c_float), C.c_int64]
     lib.orc_keys.argtypes = [C.POINTER(N.MpmDesc), C.POINTER(C.c_float), C.c_int64, C.POINTER(C.c_uint32)]
     lib.orc_stable_argsort.argtypes = [C.POINTER(C.c_uint32), C.c_int64, C.POINTER(C.c_uint32)]
     _lib = lib
@@ -187,3 +191,26 @@ def stable_argsort(k) -> np.ndarray:
     out = np.zeros(k.size, np.uint32)
     lib.orc_stable_argsort(N.uptr(k), k.size, N.uptr(out))
     return out
+
+
+def density(desc: SolverDesc, positions, mass, precision: int = 32, with_active: bool = False):
+    """writeToOpenVDB's density raster of the given particles: flat array in
+    the reference node order (and the written-node mask if asked)."""
+    lib = load()
+    p = np.ascontiguousarray(positions, dtype=np.float32).reshape(-1, 3)
+    m = np.ascontiguousarray(np.broadcast_to(np.asarray(mass, np.float32), (p.shape[0],)))
+    d = desc.to_c()
+    out = np.zeros(desc.nodes, np.float64)
+    act = np.zeros(desc.nodes, np.uint8) if with_active else None
+    lib.orc_density(C.byref(d), N.fptr(p), N.fptr(m), p.shape[0], _d(out),
+                    act.ctypes.data_as(C.POINTER(C.c_uint8)) if with_active else None, precision)
+    return (out, act.astype(bool)) if with_active else out
+
+
+def time_density(desc: SolverDesc, positions, mass) -> float:
+    """Seconds for one serial FP32 raster of the given particles."""
+    lib = load()
+    p = np.ascontiguousarray(positions, dtype=np.float32).reshape(-1, 3)
+    m = np.ascontiguousarray(np.broadcast_to(np.asarray(mass, np.float32), (p.shape[0],)))
+    d = desc.to_c()
+    return float(lib.orc_time_density(C.byref(d), N.fptr(p), N.fptr(m), p.shape[0]))

@@ -78,3 +78,28 @@ def test_headless_snow_cube_matches_oracle(apic, tmp_path):
     report(f"headless snow_cube {'APIC' if apic else 'PIC-FLIP'} {steps} steps", e)
     assert e["moved"] > 0.0
     assert e["position"] <= 1e-5
+
+
+def test_headless_density_frames_match_oracle(tmp_path):
+    # the offline output path: writeToOpenVDB every 2nd step into a directory
+    cfg_path = os.path.join(CFG_DIR, "snow_cube.yaml")
+    cfg = _config(cfg_path)
+    out_dir = str(tmp_path / "frames")
+    cmd = [os.path.join(LIB_DIR, "mpm_headless"), cfg_path, "--apic", "--steps", "4", "--every", "2",
+           "--density", out_dir]
+    print("running", " ".join(cmd), flush=True)
+    r = subprocess.run(cmd, cwd=CFG_DIR, capture_output=True, text=True, timeout=120)
+    print(r.stdout, r.stderr, flush=True)
+    assert r.returncode == 0
+    assert sorted(os.listdir(out_dir)) == ["00002.mpmd", "00004.mpmd"]
+    from mpm_amd.density import read_density_file
+    desc, particles = _scene(cfg, N.APIC)
+    o = pyoracle.OracleSolver(desc, particles, 32)
+    o.step(4)
+    st = o.read_particles()
+    ref = pyoracle.density(desc, st.position, st.mass, 32)
+    got = read_density_file(os.path.join(out_dir, "00004.mpmd"))
+    assert got.resolution == tuple(cfg["resolution"])
+    e = {"density": rel_err(got.dense(), ref)}
+    report("headless density frame (APIC, 4 steps)", e)
+    assert e["density"] <= 1e-5
