@@ -74,6 +74,47 @@ def cpu_baseline(scene, seconds_target=15.0):
     }
 
 
+def density_bench(solver, sc, frames=5, cpu=True):
+    """Density raster (the writeToOpenVDB replacement) after each of `frames`
+    steps: HIP-event time of the raster (sort of the current positions +
+    mass-only P2G into its own plane), wall time of the block readback, and
+    the reference's host raster (the oracle's restatement) on a sample.
+    Algorithmic bytes: x and m per particle (16 B) + 4 B per nonzero node."""
+    n = sc.particles.count
+    solver.profile(True)
+    solver.profile_reset()
+    for _ in range(frames):
+        solver.step(1)
+        solver.density_compute()
+    solver.synchronize()
+    prof = solver.profile_read()
+    solver.profile(False)
+    ms = prof["density"][1] / max(prof["density"][0], 1)
+    solver.step(1)
+    solver.synchronize()
+    q0 = time.perf_counter()
+    blk = solver.density_blocks()
+    read_ms = 1e3 * (time.perf_counter() - q0)
+    nz = int(np.count_nonzero(blk.values))
+    alg = 16 * n + 4 * nz
+    out = {"frames": frames, "ms": ms, "particles_per_s": n / (ms * 1e-3), "blocks": blk.count,
+           "nonzero_nodes": nz, "algorithmic_bytes": alg, "achieved_GBps": alg / (ms * 1e-3) / 1e9,
+           "frac": alg / (ms * 1e-3) / HBM_PEAK, "block_readback_ms": read_ms,
+           "parts_ms": {k: v[1] / max(v[0], 1) for k, v in prof.items()}}
+    if cpu:
+        from oracle import pyoracle
+
+        x = sc.particles.position
+        sel = np.nonzero((x[:, 0] < 0.3) & (x[:, 2] < 0.3))[0]
+        t = pyoracle.time_density(sc.desc, x[sel], sc.particles.mass[sel])
+        out["cpu_host_raster"] = {"particles_per_s": sel.size / t, "cores": 1, "kind": "port",
+                                  "sample": f"serial host raster of {sel.size} particles (slab sub-box x,z<0.3), "
+                                            f"{t:.2f} s"}
+    log(f"density raster: {ms:.3f} ms/frame ({out['particles_per_s']:.3e} particles/s), {blk.count} blocks, "
+        f"block readback {read_ms:.1f} ms")
+    return out
+
+
 def _pmc_pass(counter, args, outdir, tag):
     """One rocprofv3 counter pass over a short run of this script (child
     process); returns {kernel name: mean counter value per dispatch}."""
@@ -358,6 +399,7 @@ def main(argv=None):
         lat.append(1e3 * (time.perf_counter() - q0))
     lat = np.array(lat)
 
+    density = density_bench(solver, sc, cpu=not a.no_cpu_baseline) if not a.graph else None
     solver.close()
     batched = batch_bench(a.batch_scenes, a.batch_steps, 3) if a.batch_scenes > 0 else None
     served = serve_bench(a.serve_seconds) if a.serve_seconds > 0 else None
@@ -394,6 +436,7 @@ def main(argv=None):
         "cpu_baseline": cpu,
         "batched": batched,
         "served": served,
+        "density_raster": density,
     }
     print(json.dumps(out), flush=True)
     return 0

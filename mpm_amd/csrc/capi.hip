@@ -331,7 +331,9 @@ void run_fused_step(Solver& s) {
     ProfileHook h = hook(s);
     if (s.grid_dirty) run_clear(s);
     ensure_stress(s);
-    run_sort(s);
+    // a raster or sub-step since the last change of the positions has
+    // already sorted them
+    if (!s.sort_valid) run_sort(s);
     {
         Scope sc(h, "p2g");
         launch_p2g(s.S, s.P, s.I, s.interp, s.apic, 0, s.item_wgs, s.st);
@@ -607,6 +609,7 @@ MPM_API int mpm_step_graph(void* handle, int n) {
         bool dirty = s->grid_dirty;
         s->grid_dirty = true;  // force the clear node into the graph
         s->vstar_dirty = true;
+        s->sort_valid = false;  // and the sort
         run_fused_step(*s);
         s->steps -= 1;
         hipError_t e = hipStreamEndCapture(s->st, &s->graph);

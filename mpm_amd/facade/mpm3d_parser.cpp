@@ -289,6 +289,12 @@ MPM3DConfiguration parseYAML(const std::string& filePath) {
     c.stride = grid["Stride"].asFloat();
     c.friction_coeff = grid["BoundaryFrictionCoefficient"].asFloat();
     c.blend_coeff = root["Transfer"]["BlendCoefficient"].asFloat();
+    if (root["Transfer"].has("Scheme")) {
+        const std::string sc = root["Transfer"]["Scheme"].asString();
+        if (sc == "APIC") c.scheme = 1;
+        else if (sc == "PICFLIP") c.scheme = 0;
+        else throw std::runtime_error("unknown Transfer.Scheme: " + sc);
+    }
     c.interp_type = parseInterpolationType(root["Interpolation"]["Type"].asString());
     const yaml::Node& sim = root["Simulate"];
     c.delta_time = sim["deltaTime"].asFloat();
@@ -332,7 +338,7 @@ extern "C" __attribute__((visibility("default"))) int chains_parse_yaml_json(con
         o << ",\"blend_coeff\":" << c.blend_coeff << ",\"interp_type\":" << (int)c.interp_type;
         o << ",\"delta_time\":" << c.delta_time << ",\"offline\":" << (c.offline ? "true" : "false");
         o << ",\"particle_frag_shader_path\":\"" << c.particle_frag_shader_path << "\"";
-        o << ",\"gravity\":" << c.gravity << ",\"steps\":" << c.steps << "}";
+        o << ",\"gravity\":" << c.gravity << ",\"steps\":" << c.steps << ",\"scheme\":" << c.scheme << "}";
         std::string s = o.str();
         if ((int)s.size() + 1 > len) return -(int)s.size() - 1;
         std::memcpy(buf, s.c_str(), s.size() + 1);

@@ -38,6 +38,7 @@ struct MPM3DConfiguration {
     // Optional keys beyond the reference schema (absent in reference files):
     float gravity = 9.80665f;  // Simulate.Gravity, magnitude along -y
     int steps = 0;             // Simulate.Steps, for headless runs
+    int scheme = -1;           // Transfer.Scheme: 0 PICFLIP, 1 APIC, -1 absent (the demo program decides)
 };
 
 ConstitutiveModel parseConstitutiveModel(const std::string& model);

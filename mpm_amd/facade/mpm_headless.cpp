@@ -81,6 +81,7 @@ int main(int argc, const char* argv[]) {
     try {
         chains::MPM3DConfiguration config = chains::parseYAML(config_path);
         if (steps < 0) steps = config.steps > 0 ? config.steps : 100;
+        if (config.scheme == 1) apic = true;
         if (density_dir.empty() && config.offline)
             density_dir = "output/" + std::filesystem::path(argv[0]).stem().string() + "-" +
                           std::filesystem::path(config_path).stem().string();

@@ -6,6 +6,8 @@
 // directory.
 #include "mpm_solver3d_builder.h"
 
+#include <cstdint>
+#include <cstring>
 #include <fstream>
 #include <iostream>
 #include <sstream>
@@ -31,19 +33,80 @@ bool readXYZFile(const std::string& filePath, std::vector<Vector3f>& points) {
     return true;
 }
 
+bool readXYZBFile(const std::string& filePath, ParticleGroupData& out) {
+    std::ifstream in(filePath, std::ios::binary);
+    if (!in.is_open()) {
+        std::cerr << "[ERROR] cannot open file " << filePath << std::endl;
+        return false;
+    }
+    char magic[8];
+    int64_t n = 0;
+    int32_t fields = 0, zero = 0;
+    in.read(magic, 8);
+    in.read(reinterpret_cast<char*>(&n), 8);
+    in.read(reinterpret_cast<char*>(&fields), 4);
+    in.read(reinterpret_cast<char*>(&zero), 4);
+    if (!in || std::memcmp(magic, "MPMXYZB1", 8) != 0 || n < 0 || n > (int64_t(1) << 31) || (fields & ~15) != 0 ||
+        zero != 0) {
+        std::cerr << "[ERROR] " << filePath << " is not a binary particle group" << std::endl;
+        return false;
+    }
+    auto rd = [&](std::vector<float>& v, size_t k) {
+        v.resize((size_t)n * k);
+        in.read(reinterpret_cast<char*>(v.data()), (std::streamsize)(v.size() * sizeof(float)));
+        return (bool)in;
+    };
+    std::vector<float> pos, vel;
+    ParticleGroupData g;
+    g.world = (fields & 8) != 0;
+    bool ok = rd(pos, 3);
+    if (ok && (fields & 1)) ok = rd(vel, 3);
+    if (ok && (fields & 2)) ok = rd(g.young, 1);
+    if (ok && (fields & 4)) ok = rd(g.mass, 1);
+    if (ok) {
+        in.peek();
+        ok = in.eof();  // no trailing bytes
+    }
+    if (!ok) {
+        std::cerr << "[ERROR] " << filePath << ": truncated or oversized binary particle group" << std::endl;
+        return false;
+    }
+    g.position.resize((size_t)n);
+    for (int64_t i = 0; i < n; ++i) g.position[(size_t)i] = Vector3f(pos[3 * i], pos[3 * i + 1], pos[3 * i + 2]);
+    if (fields & 1) {
+        g.velocity.resize((size_t)n);
+        for (int64_t i = 0; i < n; ++i) g.velocity[(size_t)i] = Vector3f(vel[3 * i], vel[3 * i + 1], vel[3 * i + 2]);
+    }
+    out = std::move(g);
+    return true;
+}
+
+bool readParticleGroup(const std::string& filePath, ParticleGroupData& out) {
+    const std::string ext = ".xyzb";
+    if (filePath.size() >= ext.size() && filePath.compare(filePath.size() - ext.size(), ext.size(), ext) == 0)
+        return readXYZBFile(filePath, out);
+    out = ParticleGroupData();
+    return readXYZFile(filePath, out.position);
+}
+
 template <class P>
 static std::vector<P> build_points(const MPM3DConfiguration& config, int& particlesNumber) {
     std::vector<P> particles;
     const size_t groups = config.group_paths.size();
     std::cout << "[INFO] building " << groups << " group(s) of particle on host" << std::endl;
     for (size_t i = 0; i < groups; ++i) {
-        std::vector<Vector3f> coords;
-        readXYZFile(config.group_paths[i], coords);
-        for (const Vector3f& c : coords)
-            particles.emplace_back(config.stride * c, config.velocities[i], config.masses[i], config.hardenings[i],
-                                   config.youngs[i], config.poissons[i], config.compressions[i], config.stretches[i],
-                                   config.constitutive_models[i]);
-        std::cout << "[INFO] particle group " << i << " has " << coords.size() << " particles" << std::endl;
+        ParticleGroupData g;
+        // as in the reference, an unreadable group is reported and skipped
+        if (!readParticleGroup(config.group_paths[i], g)) g = ParticleGroupData();
+        const size_t n = g.position.size();
+        particles.reserve(particles.size() + n);
+        for (size_t k = 0; k < n; ++k)
+            particles.emplace_back(g.world ? g.position[k] : config.stride * g.position[k],
+                                   g.velocity.empty() ? config.velocities[i] : g.velocity[k],
+                                   g.mass.empty() ? config.masses[i] : g.mass[k], config.hardenings[i],
+                                   g.young.empty() ? config.youngs[i] : g.young[k], config.poissons[i],
+                                   config.compressions[i], config.stretches[i], config.constitutive_models[i]);
+        std::cout << "[INFO] particle group " << i << " has " << n << " particles" << std::endl;
     }
     particlesNumber = (int)particles.size();
     std::cout << "[INFO] stacking all " << particlesNumber << " particles on host" << std::endl;
@@ -65,3 +128,28 @@ APICMPMSolver3D buildAPICMPMSolver3DFromYAML(MPM3DConfiguration config, int& par
 }
 
 }  // namespace chains
+
+// Reads one particle group as the builders do (positions scaled by stride
+// unless the file holds world units): for the loader tests.  Writes up to
+// max particles; *count gets the group size and *fields the per-particle
+// fields present (1 velocity, 2 young, 4 mass).  0 on success, -1 if the
+// file cannot be read.
+extern "C" __attribute__((visibility("default"))) int chains_read_group(const char* path, float stride, float* pos,
+                                                                        float* vel, float* young, float* mass,
+                                                                        int64_t max, int64_t* count, int* fields) {
+    chains::ParticleGroupData g;
+    if (!path || !chains::readParticleGroup(path, g)) return -1;
+    const int64_t n = (int64_t)g.position.size();
+    if (count) *count = n;
+    if (fields) *fields = (g.velocity.empty() ? 0 : 1) | (g.young.empty() ? 0 : 2) | (g.mass.empty() ? 0 : 4);
+    for (int64_t i = 0; i < n && i < max; ++i) {
+        const chains::Vector3f x = g.world ? g.position[(size_t)i] : stride * g.position[(size_t)i];
+        for (int a = 0; a < 3; ++a) {
+            if (pos) pos[3 * i + a] = x(a);
+            if (vel && !g.velocity.empty()) vel[3 * i + a] = g.velocity[(size_t)i](a);
+        }
+        if (young && !g.young.empty()) young[i] = g.young[(size_t)i];
+        if (mass && !g.mass.empty()) mass[i] = g.mass[(size_t)i];
+    }
+    return 0;
+}

@@ -103,3 +103,31 @@ def test_headless_density_frames_match_oracle(tmp_path):
     e = {"density": rel_err(got.dense(), ref)}
     report("headless density frame (APIC, 4 steps)", e)
     assert e["density"] <= 1e-5
+
+
+@pytest.mark.parametrize("transfer", [N.PIC_FLIP, N.APIC])
+def test_headless_runs_generated_binary_scene(tmp_path, transfer):
+    # a generated scene written as YAML + binary group (per-particle velocity
+    # and Young's modulus); Transfer.Scheme selects the solver class
+    from mpm_amd import scenes
+    from mpm_amd.scene_io import write_scene
+    from mpm_amd.solver import Solver
+
+    sc = scenes.snow_cube(seed=41, n=20000, res=48, transfer=transfer)
+    rng = np.random.default_rng(4)
+    sc.particles.velocity[:] = rng.normal(0, 0.5, (20000, 3)).astype(np.float32)
+    sc.particles.young[:] = rng.uniform(1.4e5, 5.6e5, 20000).astype(np.float32)
+    y = write_scene(sc.desc, sc.particles, str(tmp_path), "gen", steps=3)
+    out = str(tmp_path / "pos.f32")
+    cmd = [os.path.join(LIB_DIR, "mpm_headless"), y, "--out", out]
+    print("running", " ".join(cmd), flush=True)
+    r = subprocess.run(cmd, cwd=str(tmp_path), capture_output=True, text=True, timeout=120)
+    print(r.stdout, r.stderr, flush=True)
+    assert r.returncode == 0
+    pos = np.fromfile(out, dtype=np.float32).reshape(-1, 4)
+    s = Solver(sc.desc, sc.particles)
+    s.step(3)
+    ref = s.read_positions4()
+    e = {"position": rel_err(pos, ref), "moved": float(np.abs(ref[:, :3] - sc.particles.position).max())}
+    report(f"headless generated scene {'APIC' if transfer else 'PIC-FLIP'}", e)
+    assert e["moved"] > 0.0 and e["position"] <= 1e-5
