@@ -161,8 +161,11 @@ def pmc_traffic(workload, seed_args, outdir):
     kib = 1024 * 1024  # 1 GiB each way
     sf, sw = kib / cfe[ck[0]], kib / cwr[ck[0]]
     out = {"fetch_scale": sf, "write_scale": sw}
+    # the fused step's instances: P2G mode 0 (7 planes) or 2 (folded APIC
+    # grid), G2P source 0 or 3; not initialize()'s mass / volume passes
+    fused = {"p2g": (", 0, false>", ", 2, false>"), "g2p": (", 0, false,", ", 3, false,")}
     for name, key in (("p2g", "k_p2g<"), ("g2p", "k_g2p<")):
-        ks = [k for k in fe if key in k and ", 0, false" in k and k in wr]
+        ks = [k for k in fe if key in k and any(t in k for t in fused[name]) and k in wr]
         if ks:
             k = max(ks, key=lambda q: fe[q])
             out[name] = {"kernel": k.split("(")[0], "fetch_kib": fe[k], "write_kib": wr[k],

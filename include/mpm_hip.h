@@ -196,6 +196,14 @@ MPM_API int mpm_particles_collision(void* handle, float dt);      /* :63 */
 MPM_API int mpm_advect_particles(void* handle, float dt);         /* :57 */
 MPM_API int mpm_switch_particles_collision(void* handle);         /* :40 */
 MPM_API int mpm_set_particles_collision(void* handle, int enable); /* field :23 */
+/* Grid recording for APIC.  The fused APIC step only needs v*, so by
+ * default its P2G sums momentum and dt*force into one set of planes
+ * (mass + 3 instead of mass + 6); mpm_read_grid, the grid sub-steps and
+ * mpm_grid_to_particles then return MPM_ERR_STATE until the next P2G or
+ * reset.  enable = 1 keeps momentum and force apart (the reference's grid
+ * after simulateOneStep, src/apic_mpm_solver3d.cu:99-124) at the cost of
+ * the wider P2G.  No effect on PIC-FLIP, whose G2P needs v and v*. */
+MPM_API int mpm_set_grid_record(void* handle, int enable);
 
 /* Particle-by-cell key generation + stable LSD radix sort (no reference
  * counterpart: the reference scatters in array order,

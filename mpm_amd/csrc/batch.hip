@@ -79,6 +79,7 @@ struct Batch {
     ItemsDev I{};
     Tune tune;
     bool tune_fixed = false;  // set explicitly: the tuning table is not consulted
+    bool fold = false;        // APIC with no member recording its grid: 4-plane P2G
     // graph
     hipGraph_t graph = nullptr;
     hipGraphExec_t gexec = nullptr;
@@ -162,6 +163,8 @@ SceneEntry make_entry(const Solver& s, int64_t pstart, uint32_t koff, uint32_t i
 int relayout(Batch& b) {
     drop_graph(b);
     const int ns = (int)b.members.size();
+    b.fold = b.apic != 0;
+    for (const Solver* m : b.members) b.fold = b.fold && !m->record_grid;
     b.tab_host.resize((size_t)ns);
     b.koff_host.resize((size_t)ns);
     b.bstart_host.assign((size_t)ns + 1, 0);
@@ -239,7 +242,7 @@ int sync_members(Batch& b) {
             b.tab_host[i] = e;
             upload = true;
         }
-        if (s.grid_dirty && (!s.count_on_clear || s.vstar_dirty)) run_clear(s);
+        if (s.grid_dirty && (!s.count_on_clear || s.vstar_dirty || (b.fold && s.force_dirty))) run_clear(s);
         ensure_stress(s);
         if (b.synced_version[i] != s.version) {
             launch_keys(s.S, s.P, b.interp, b.kin + b.tab_host[i].pstart, b.st, b.tab_host[i].key_offset);
@@ -257,7 +260,7 @@ int sync_members(Batch& b) {
 
 void enqueue_step(Batch& b) {
     const int ns = (int)b.members.size();
-    launch_clear_batch(b.tab, b.block_start, ns, b.total_blocks, GF_PRIMARY, b.st);
+    launch_clear_batch(b.tab, b.block_start, ns, b.total_blocks, b.fold ? GF_FOLDED : GF_PRIMARY, b.st);
     const int sel = radix_sort_pairs(b.kin, b.iota, b.ka, b.va, b.kb, b.vb, b.ntot, b.key_bits, b.hist, b.scan_temp,
                                      b.st, ProfileHook{});
     b.I.sorted_keys = sel ? b.kb : b.ka;
@@ -267,8 +270,8 @@ void enqueue_step(Batch& b) {
     const SceneDev S0{};
     const ParticlesDev P0{};
     const int wgs = effective_wgs(b);
-    launch_p2g(S0, P0, b.I, b.interp, b.apic, 0, wgs, b.st, b.tab);
-    launch_g2p(S0, P0, b.I, b.interp, b.apic, 0, 1, 0, wgs, b.st, b.kin, b.tab, b.tune.g2p_wg);
+    launch_p2g(S0, P0, b.I, b.interp, b.apic, b.fold ? 2 : 0, wgs, b.st, b.tab);
+    launch_g2p(S0, P0, b.I, b.interp, b.apic, b.fold ? 3 : 0, 1, 0, wgs, b.st, b.kin, b.tab, b.tune.g2p_wg);
 }
 
 // Host-side bookkeeping of the members after n batch steps.
@@ -280,7 +283,8 @@ void after_steps(Batch& b, int n) {
         b.synced_version[i] = s.version;  // the batch G2P wrote the keys of the new positions
         s.grid_dirty = true;
         s.count_on_clear = true;
-        s.phase = PH_RAW;
+        if (!b.fold) s.force_dirty = true;
+        s.phase = b.fold ? PH_FOLDED : PH_RAW;
         s.sort_valid = false;
         s.keys_valid = false;  // the member's own sort input is stale
     }
@@ -380,6 +384,12 @@ void batch_destroy(Batch* b) {
 // Called by mpm_destroy for a scene that is still a member.
 void batch_forget(Batch* b, Solver* s) {
     if (b && s) (void)detach(*b, *s);
+}
+
+void batch_invalidate(Batch* b) {
+    if (!b) return;
+    b->layout_valid = false;
+    drop_graph(*b);
 }
 
 int check_tune(const mpm_tune* t) {

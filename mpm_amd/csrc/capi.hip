@@ -312,9 +312,11 @@ void run_sort(Solver& s) {
 void run_clear(Solver& s) {
     ProfileHook h = hook(s);
     Scope sc(h, "grid_clear");
-    launch_clear(s.S, s.nsb, s.vstar_dirty ? GF_ALL : GF_PRIMARY, s.count_on_clear ? s.acc : nullptr, s.st);
+    launch_clear(s.S, s.nsb, s.vstar_dirty ? GF_ALL : (s.force_dirty ? GF_PRIMARY : GF_FOLDED),
+                 s.count_on_clear ? s.acc : nullptr, s.st);
     s.grid_dirty = false;
     s.vstar_dirty = false;
+    s.force_dirty = false;
     s.count_on_clear = false;
     s.phase = PH_CLEAR;
 }
@@ -334,17 +336,22 @@ void run_fused_step(Solver& s) {
     // a raster or sub-step since the last change of the positions has
     // already sorted them
     if (!s.sort_valid) run_sort(s);
+    // APIC's G2P reads v* alone, so by default its P2G sums p + dt f into
+    // one set of planes (4 instead of 7 per node)
+    const bool fold = fused_folds(s);
     {
         Scope sc(h, "p2g");
-        launch_p2g(s.S, s.P, s.I, s.interp, s.apic, 0, s.item_wgs, s.st);
+        launch_p2g(s.S, s.P, s.I, s.interp, s.apic, fold ? 2 : 0, s.item_wgs, s.st);
     }
     {
         Scope sc(h, "g2p");
-        launch_g2p(s.S, s.P, s.I, s.interp, s.apic, 0, 1, s.S.particle_collision, s.item_wgs, s.st, s.kin);
+        launch_g2p(s.S, s.P, s.I, s.interp, s.apic, fold ? 3 : 0, 1, s.S.particle_collision, s.item_wgs, s.st,
+                   s.kin);
     }
     s.grid_dirty = true;
     s.count_on_clear = true;
-    s.phase = PH_RAW;
+    if (!fold) s.force_dirty = true;
+    s.phase = fold ? PH_FOLDED : PH_RAW;
     s.sort_valid = false;
     s.keys_valid = true;  // the G2P wrote the keys of the advected positions
     s.steps += 1;
@@ -596,7 +603,8 @@ MPM_API int mpm_step_graph(void* handle, int n) {
     if (n == 0) return MPM_OK;
     // A grid left by initialize / sub-steps is cleared outside the graph
     // (without counting); the captured step always starts with a clear.
-    if (s->grid_dirty && !s->count_on_clear) run_clear(*s);
+    const bool fold = fused_folds(*s);
+    if (s->grid_dirty && (!s->count_on_clear || s->vstar_dirty || (fold && s->force_dirty))) run_clear(*s);
     ensure_stress(*s);  // outside the graph: replays rely on G2P keeping it current
     if (!s->keys_valid) {  // likewise the sort input keys
         launch_keys(s->S, s->P, s->interp, s->kin, s->st);
@@ -607,9 +615,13 @@ MPM_API int mpm_step_graph(void* handle, int n) {
         s->prof = false;
         HIP_TRY(hipStreamBeginCapture(s->st, hipStreamCaptureModeThreadLocal));
         bool dirty = s->grid_dirty;
-        s->grid_dirty = true;  // force the clear node into the graph
-        s->vstar_dirty = true;
-        s->sort_valid = false;  // and the sort
+        // force the clear node (of the planes the step writes, counting the
+        // active nodes) and the sort into the graph
+        s->grid_dirty = true;
+        s->count_on_clear = true;
+        s->vstar_dirty = false;
+        s->force_dirty = !fold;
+        s->sort_valid = false;
         run_fused_step(*s);
         s->steps -= 1;
         hipError_t e = hipStreamEndCapture(s->st, &s->graph);
@@ -624,8 +636,9 @@ MPM_API int mpm_step_graph(void* handle, int n) {
     s->version += (uint64_t)n;
     s->grid_dirty = true;
     s->count_on_clear = true;
-    s->vstar_dirty = true;
-    s->phase = PH_RAW;
+    s->vstar_dirty = false;
+    s->force_dirty = !fold;
+    s->phase = fold ? PH_FOLDED : PH_RAW;
     s->sort_valid = false;
     auto t1 = std::chrono::steady_clock::now();
     s->last_step_ms = std::chrono::duration<double, std::milli>(t1 - t0).count();
@@ -652,12 +665,34 @@ MPM_API int mpm_particles_to_grid(void* handle) {
     launch_p2g(s->S, s->P, s->I, s->interp, s->apic, 0, s->item_wgs, s->st);
     s->grid_dirty = true;
     s->count_on_clear = false;
+    s->force_dirty = true;
     s->phase = PH_SUBSTEP;
     return finish(*s);
 }
 
+// The folded grid of a fused APIC step holds no momentum or force to work on.
+static int require_unfolded(Solver& s, const char* what) {
+    if (s.phase != PH_FOLDED) return MPM_OK;
+    return fail(MPM_ERR_STATE, std::string(what) +
+                                   ": the last fused APIC step summed momentum and dt*force per node; call "
+                                   "mpm_set_grid_record(handle, 1) before stepping to keep them apart");
+}
+
+MPM_API int mpm_set_grid_record(void* handle, int enable) {
+    GET_SOLVER(s, handle);
+    if (s->record_grid == (enable != 0)) return MPM_OK;
+    s->record_grid = enable != 0;
+    // the captured step and a batch's launches depend on the mode
+    if (s->gexec) { (void)hipGraphExecDestroy(s->gexec); s->gexec = nullptr; }
+    if (s->graph) { (void)hipGraphDestroy(s->graph); s->graph = nullptr; }
+    if (s->batch) batch_invalidate(s->batch);
+    return MPM_OK;
+}
+
 MPM_API int mpm_compute_gravity_forces(void* handle) {
     GET_SOLVER(s, handle);
+    if (int rc = require_unfolded(*s, "computeGravityForces")) return rc;
+    s->force_dirty = true;
     launch_grid_op(s->S, s->nsb, 0, s->S.dt, s->st);
     s->phase = PH_SUBSTEP;
     return finish(*s);
@@ -667,6 +702,7 @@ MPM_API int mpm_compute_external_forces(void* handle) { return mpm_compute_gravi
 
 MPM_API int mpm_update_grid_velocities(void* handle, float dt) {
     GET_SOLVER(s, handle);
+    if (int rc = require_unfolded(*s, "updateGridVelocities")) return rc;
     launch_grid_op(s->S, s->nsb, 1, dt, s->st);
     s->vstar_dirty = true;
     s->phase = PH_SUBSTEP;
@@ -682,6 +718,7 @@ MPM_API int mpm_solve_linear_system(void* handle, float dt) {
 
 MPM_API int mpm_grid_collision(void* handle, float dt) {
     GET_SOLVER(s, handle);
+    if (int rc = require_unfolded(*s, "gridCollision")) return rc;
     launch_grid_op(s->S, s->nsb, 2, dt, s->st);
     s->vstar_dirty = true;
     s->phase = PH_SUBSTEP;
@@ -690,6 +727,7 @@ MPM_API int mpm_grid_collision(void* handle, float dt) {
 
 MPM_API int mpm_grid_to_particles(void* handle, float dt, float blend) {
     GET_SOLVER(s, handle);
+    if (int rc = require_unfolded(*s, "gridToParticles")) return rc;
     if (s->phase == PH_RAW) materialize(*s);
     if (!s->sort_valid) run_sort(*s);
     SceneDev S = s->S;
@@ -829,6 +867,7 @@ MPM_API int mpm_read_grid(void* handle, mpm_grid_host* out) {
     GET_SOLVER(s, handle);
     const int64_t total = (int64_t)s->S.rx * s->S.ry * s->S.rz;
     if (!out || out->nodes != total) return fail(MPM_ERR_INVALID, "grid node count mismatch");
+    if (int rc = require_unfolded(*s, "read grid")) return rc;
     if (s->phase == PH_RAW) materialize(*s);
     float *m = nullptr, *v = nullptr, *w = nullptr, *f = nullptr;
     int rc;

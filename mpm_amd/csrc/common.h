@@ -19,7 +19,8 @@ enum GridField {
     GF_FORCE = 4, // 4,5,6
     GF_VSTAR = 7, // 7,8,9
     GF_PRIMARY = 7,
-    GF_ALL = 10
+    GF_ALL = 10,
+    GF_FOLDED = 4  // fused APIC step: mass, then p + dt f in planes 1..3
 };
 
 // Material constants shared by groups of particles (hardening xi,

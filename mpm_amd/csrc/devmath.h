@@ -319,6 +319,27 @@ MPM_HD void boundary_collision(const SceneDev& S, float px, float py, float pz, 
     }
 }
 
+// The same update from a folded grid, whose planes hold the mass and
+// q = p + dt f with f the elastic force only (the fused APIC step: G2P reads
+// v* alone).  v* = q / m - dt g e_y for m > FLT_EPSILON, else 0 (the
+// reference's light nodes keep v* = 0), then the boundary collision.  In
+// exact arithmetic this is grid_node_update's v*.
+MPM_HD void grid_node_update_folded(const SceneDev& S, int i, int j, int k, float m, float qx, float qy, float qz,
+                                    float& wx, float& wy, float& wz) {
+    if (m > MPM_FLT_EPS) {
+        const float im = 1.0f / m;
+        wx = qx * im;
+        wy = fmaf(-S.dt, S.gravity, qy * im);
+        wz = qz * im;
+    } else {
+        wx = 0.0f; wy = 0.0f; wz = 0.0f;
+    }
+    float gx = (S.ox + S.h * (float)i) + S.dt * wx;
+    float gy = (S.oy + S.h * (float)j) + S.dt * wy;
+    float gz = (S.oz + S.h * (float)k) + S.dt * wz;
+    boundary_collision(S, gx, gy, gz, wx, wy, wz);
+}
+
 // ------------------------------------------------------- grid addressing
 // Storage node index (block-major 4x4x4, x fastest inside a block).
 MPM_HD int64_t node_index(const SceneDev& S, int i, int j, int k) {

@@ -386,9 +386,18 @@ __device__ __forceinline__ SegPlan seg_plan(uint64_t headmask, bool active, int 
     "v_fmac_f32_dpp %5, %5, %7 " CTRL " bank_mask:0xf bound_ctrl:1\n\t"                       \
     "v_fmac_f32_dpp %6, %6, %7 " CTRL " bank_mask:0xf bound_ctrl:1"
 
+#define MPM_FMAC_DPP4(CTRL)                                                                    \
+    "s_nop 4\n\t"                                                                              \
+    "v_fmac_f32_dpp %0, %0, %4 " CTRL " bank_mask:0xf bound_ctrl:1\n\t"                       \
+    "v_fmac_f32_dpp %1, %1, %4 " CTRL " bank_mask:0xf bound_ctrl:1\n\t"                       \
+    "v_fmac_f32_dpp %2, %2, %4 " CTRL " bank_mask:0xf bound_ctrl:1\n\t"                       \
+    "v_fmac_f32_dpp %3, %3, %4 " CTRL " bank_mask:0xf bound_ctrl:1"
+
 #define MPM_SCAN_STEP(NV, v, CTRL, flag)                                                          \
     do {                                                                                        \
-        if constexpr (NV == 7) {                                                                \
+        if constexpr (NV == 4) {                                                                \
+            asm volatile(MPM_FMAC_DPP4(CTRL) : "+v"(v[0]), "+v"(v[1]), "+v"(v[2]), "+v"(v[3]) : "v"(flag)); \
+        } else if constexpr (NV == 7) {                                                         \
             asm volatile(MPM_FMAC_DPP7(CTRL)                                                    \
                          : "+v"(v[0]), "+v"(v[1]), "+v"(v[2]), "+v"(v[3]), "+v"(v[4]), "+v"(v[5]), \
                            "+v"(v[6])                                                           \
@@ -399,9 +408,11 @@ __device__ __forceinline__ SegPlan seg_plan(uint64_t headmask, bool active, int 
         }                                                                                       \
     } while (0)
 
-template <int NV>
-__device__ __forceinline__ void seg_scan(float (&v)[NV], const SegPlan& sp) {
-    static_assert(NV == 7 || NV == 1, "scan blocks are written for 7 or 1 values");
+// Segmented scan of the first NV values of v.
+template <int NV, int N>
+__device__ __forceinline__ void seg_scan(float (&v)[N], const SegPlan& sp) {
+    static_assert(NV == 7 || NV == 4 || NV == 1, "scan blocks are written for 7, 4 or 1 values");
+    static_assert(NV <= N, "fewer values than the scan covers");
     if (sp.s1) MPM_SCAN_STEP(NV, v, "row_shr:1 row_mask:0xf", sp.f1);
     if (sp.s2) MPM_SCAN_STEP(NV, v, "row_shr:2 row_mask:0xf", sp.f2);
     if (sp.s4) MPM_SCAN_STEP(NV, v, "row_shr:4 row_mask:0xf", sp.f4);
@@ -416,7 +427,8 @@ __global__ void __launch_bounds__(64) k_p2g(SceneDev S0, ParticlesDev P0, ItemsD
     constexpr int W = Stencil<INTERP>::W;
     constexpr int T = 4 + W - 1;
     constexpr int TN = T * T * T;
-    constexpr int NF = MODE == 0 ? 7 : 1;
+    constexpr int NF = MODE == 0 ? 7 : (MODE == 1 ? 1 : 4);
+    static_assert(MODE != 2 || APIC, "the folded grid serves APIC only");
     __shared__ float tile[NF * TN];
     const uint32_t nitems = *I.count;
     const int lane = threadIdx.x;
@@ -462,10 +474,12 @@ __global__ void __launch_bounds__(64) k_p2g(SceneDev S0, ParticlesDev P0, ItemsD
                         }
                 continue;
             }
-            // cached -V0 P Fe^T (symmetric), written by the previous G2P or k_stress
+            // cached -V0 P Fe^T (symmetric), written by the previous G2P or
+            // k_stress; scaled by dt for the folded grid (MODE 2), whose
+            // planes 1..3 receive p + dt f
             float vs[3][3];
             {
-                const float act = active ? 1.0f : 0.0f;
+                const float act = active ? (MODE == 2 ? S.dt : 1.0f) : 0.0f;
                 const float t0 = act * P.tau[p], t1 = act * P.tau[N + p], t2 = act * P.tau[2 * N + p];
                 const float t3 = act * P.tau[3 * N + p], t4 = act * P.tau[4 * N + p], t5 = act * P.tau[5 * N + p];
                 vs[0][0] = t0; vs[0][1] = t1; vs[0][2] = t2;
@@ -525,14 +539,19 @@ __global__ void __launch_bounds__(64) k_p2g(SceneDev S0, ParticlesDev P0, ItemsD
 #pragma unroll
                         for (int k = 0; k < 3; ++k) {
                             const float ck = APIC ? cx[k] + (Cm[k][1] * dxn[1][b] + Cm[k][2] * dxn[2][c]) : cx[k];
-                            v[1 + k] = mip * ck;
-                            v[4 + k] = (F0[k] * yz[b][c] + F1[k] * dyz[b][c]) + F2[k] * ydz[b][c];
+                            const float fk = (F0[k] * yz[b][c] + F1[k] * dyz[b][c]) + F2[k] * ydz[b][c];
+                            if (MODE == 2) {
+                                v[1 + k] = fmaf(mip, ck, fk);
+                            } else {
+                                v[1 + k] = mip * ck;
+                                v[4 + k] = fk;
+                            }
                         }
-                        seg_scan<7>(v, sp);
+                        seg_scan<NF>(v, sp);
                         const int q = ((lz + c) * T + (ly + b)) * T + (lx + a);
                         if (writer) {
 #pragma unroll
-                            for (int f = 0; f < 7; ++f) tile[f * TN + q] += v[f];
+                            for (int f = 0; f < NF; ++f) tile[f * TN + q] += v[f];
                         }
                     }
             }
@@ -561,6 +580,13 @@ __global__ void __launch_bounds__(64) k_p2g(SceneDev S0, ParticlesDev P0, ItemsD
 template <int INTERP, int APIC>
 static void p2g_dispatch_mode(const SceneDev& S, const ParticlesDev& P, const ItemsDev& I, const SceneEntry* tab,
                               int mode, int wgs, hipStream_t st) {
+    if constexpr (APIC) {
+        if (mode == 2) {
+            if (tab) MPM_PERSISTENT((k_p2g<INTERP, APIC, 2, true>), 64, wgs, st, S, P, I, tab);
+            else MPM_PERSISTENT((k_p2g<INTERP, APIC, 2, false>), 64, wgs, st, S, P, I, tab);
+            return;
+        }
+    }
     if (tab) MPM_PERSISTENT((k_p2g<INTERP, APIC, 0, true>), 64, wgs, st, S, P, I, tab);
     else if (mode == 0) MPM_PERSISTENT((k_p2g<INTERP, APIC, 0, false>), 64, wgs, st, S, P, I, tab);
     else MPM_PERSISTENT((k_p2g<INTERP, APIC, 1, false>), 64, wgs, st, S, P, I, tab);
@@ -595,6 +621,7 @@ __global__ void __launch_bounds__(WG) __attribute__((amdgpu_waves_per_eu(G2PWave
     constexpr int T = 4 + W - 1;
     constexpr int TN = T * T * T;
     constexpr int NF = SRC == 2 ? 1 : (APIC ? 3 : 6);
+    static_assert(SRC != 3 || APIC, "the folded grid serves APIC only");
     __shared__ float tile[NF * TN];
     const uint32_t nitems = *I.count;
     for (uint32_t it = blockIdx.x; it < nitems; it += gridDim.x) {
@@ -626,6 +653,10 @@ __global__ void __launch_bounds__(WG) __attribute__((amdgpu_waves_per_eu(G2PWave
                             vals[4] = vals[1] - G[(GF_VEL + 1) * pl + ni];
                             vals[5] = vals[2] - G[(GF_VEL + 2) * pl + ni];
                         }
+                    } else if (SRC == 3) {
+                        grid_node_update_folded(S, gi, gj, gk, G[ni], G[(GF_VEL + 0) * pl + ni],
+                                                G[(GF_VEL + 1) * pl + ni], G[(GF_VEL + 2) * pl + ni], vals[0],
+                                                vals[1], vals[2]);
                     } else {
                         float vx, vy, vz, wx, wy, wz;
                         grid_node_update(S, gi, gj, gk, G[ni], G[(GF_VEL + 0) * pl + ni], G[(GF_VEL + 1) * pl + ni],
@@ -811,6 +842,13 @@ static void g2p_dispatch_wg(const SceneDev& S, const ParticlesDev& P, const Item
 template <int INTERP, int APIC>
 static void g2p_dispatch_src(const SceneDev& S, const ParticlesDev& P, const ItemsDev& I, const SceneEntry* tab,
                              int src, int advect, int pcollide, int wgs, int wg, hipStream_t st, uint32_t* nk) {
+    if constexpr (APIC) {
+        if (src == 3) {
+            if (tab) g2p_dispatch_wg<INTERP, APIC, 3, true>(S, P, I, tab, advect, pcollide, wgs, wg, st, nk);
+            else g2p_dispatch_wg<INTERP, APIC, 3, false>(S, P, I, tab, advect, pcollide, wgs, wg, st, nk);
+            return;
+        }
+    }
     if (tab) g2p_dispatch_wg<INTERP, APIC, 0, true>(S, P, I, tab, advect, pcollide, wgs, wg, st, nk);
     else if (src == 0) g2p_dispatch_wg<INTERP, APIC, 0, false>(S, P, I, tab, advect, pcollide, wgs, wg, st, nk);
     else if (src == 1)

@@ -33,7 +33,9 @@ inline int fail(int code, const std::string& msg) {
         }                                                                                             \
     } while (0)
 
-enum GridPhase { PH_CLEAR = 0, PH_MASS = 1, PH_RAW = 2, PH_SUBSTEP = 3 };
+// PH_RAW: P2G sums (mass, momentum, force) of a fused step; PH_FOLDED: the
+// fused APIC step's mass and p + dt f (momentum and force not separable).
+enum GridPhase { PH_CLEAR = 0, PH_MASS = 1, PH_RAW = 2, PH_SUBSTEP = 3, PH_FOLDED = 4 };
 
 struct Pending {
     std::string name;
@@ -65,6 +67,11 @@ struct Solver {
     unsigned long long* acc = nullptr;            // [0] active sum, [1] scratch, [2] escaped
     bool sort_valid = false, grid_dirty = false, vstar_dirty = false, count_on_clear = false;
     bool stress_valid = false;  // P.tau matches the particle state
+    // force planes (GF_FORCE..) may be non-zero: a clear must cover them
+    bool force_dirty = false;
+    // keep momentum and force apart in the fused APIC step (7-plane P2G) so
+    // that mpm_read_grid shows them after a step; off = the folded grid
+    bool record_grid = false;
     int phase = PH_CLEAR;
     int64_t steps = 0;
     int item_wgs = 0;  // 0: persistent grids sized by occupancy
@@ -121,6 +128,10 @@ void destroy(Solver* s);
 int finish(Solver& s);
 // batch.hip: removes a scene that is destroyed while a batch member.
 void batch_forget(struct Batch* b, Solver* s);
+// batch.hip: a member's settings changed; the batch recomputes its layout.
+void batch_invalidate(struct Batch* b);
+// Planes of the grid that P2G writes in the fused step.
+inline bool fused_folds(const Solver& s) { return s.apic && !s.record_grid; }
 
 }  // namespace host
 }  // namespace mpm

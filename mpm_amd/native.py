@@ -157,6 +157,7 @@ SIGNATURES = {
     "mpm_advect_particles": (C.c_int, [C.c_void_p, C.c_float]),
     "mpm_switch_particles_collision": (C.c_int, [C.c_void_p]),
     "mpm_set_particles_collision": (C.c_int, [C.c_void_p, C.c_int]),
+    "mpm_set_grid_record": (C.c_int, [C.c_void_p, C.c_int]),
     "mpm_sort": (C.c_int, [C.c_void_p]),
     "mpm_read_particles": (C.c_int, [C.c_void_p, C.POINTER(MpmParticleState)]),
     "mpm_write_particles": (C.c_int, [C.c_void_p, C.POINTER(MpmParticleState)]),

@@ -211,6 +211,11 @@ class Solver:
     def set_particles_collision(self, on: bool):
         N.check(self._lib.mpm_set_particles_collision(self._h, 1 if on else 0))
 
+    def set_grid_record(self, on: bool):
+        """APIC: keep momentum and force apart in the fused step so that
+        read_grid works after it (default: folded, faster)."""
+        N.check(self._lib.mpm_set_grid_record(self._h, 1 if on else 0))
+
     def initialize(self):
         N.check(self._lib.mpm_initialize(self._h))
 
