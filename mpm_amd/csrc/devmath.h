@@ -219,7 +219,7 @@ MPM_HD void bspline(float d, float& N, float& dN) {
     float xa = fabsf(d);
     if (INTERP == MPM_INTERP_LINEAR) {
         N = xa < 1.0f ? 1.0f - xa : 0.0f;
-        dN = (xa < 1.0f && xa > MPM_FLT_EPS) ? -d / xa : 0.0f;
+        dN = (xa < 1.0f && xa > MPM_FLT_EPS) ? -copysignf(1.0f, d) : 0.0f;  // -d/|d|
     } else if (INTERP == MPM_INTERP_QUADRATIC_BSPLINE) {
         float t = 1.5f - xa;
         float n_in = 0.75f - xa * xa;
@@ -245,24 +245,42 @@ template <> struct Stencil<MPM_INTERP_LINEAR> { static constexpr int W = 2; stat
 template <> struct Stencil<MPM_INTERP_QUADRATIC_BSPLINE> { static constexpr int W = 3; static constexpr float SHIFT = 0.5f; };
 template <> struct Stencil<MPM_INTERP_CUBIC_BSPLINE> { static constexpr int W = 4; static constexpr float SHIFT = 1.0f; };
 
-// Per-axis weights of one particle: base node b (fixed W-node stencil,
-// b = floor(fl(x/h) - shift)), N and N'/h for nodes b..b+W-1.  The
-// reference's range loop ceil(x/h - r)..floor(x/h + r)
-// (src/mpm_solver3d.cu:251-255) selects the same nodes except when
-// x/h - r is an integer, where the extra node has weight 0.
+// Base cell of the stencil along one axis (also the sort key's cell):
+// b = floor(u - shift).  Quadratic and cubic: u = fl(x * fl(1/h)), one
+// product.  Linear: u = fl(x/h), the quotient the reference's range uses,
+// and where that quotient rounded up onto an integer (x < h * floor(u),
+// decided exactly by the sign of one fused multiply-add) b is one lower:
+// the linear weight derivative jumps at the support boundary, so the two
+// nodes must be the ones with 0 <= (x - x_b)/h < 1 exactly.
+template <int INTERP>
+MPM_HD float base_cell(float x, float h, float inv_h) {
+    if (INTERP == MPM_INTERP_LINEAR) {
+        float fb = floorf(x / h);
+        if (fmaf(-h, fb, x) < 0.0f) fb -= 1.0f;
+        return fb;
+    }
+    return floorf(x * inv_h - Stencil<INTERP>::SHIFT);
+}
+
+// Per-axis weights of one particle: base node b (fixed W-node stencil, b =
+// base_cell), N and N'/h for nodes b..b+W-1 at d = (x - x_j) * fl(1/h) with
+// x_j = origin + h j, the reference's node position (gd_pos,
+// src/mpm_solver3d.cu:160).  The reference's range loop
+// ceil(x/h - r)..floor(x/h + r) (src/mpm_solver3d.cu:251-255) selects the
+// same nodes with non-zero weight or derivative, except that for quadratic
+// and cubic a node at |d| = r - delta, delta <= 2 ulp(x/h), may be in one
+// set only; N and N' vanish continuously there (N < delta^2, |N'| < delta).
 template <int INTERP>
 MPM_HD void axis_weights(float x, float origin, float h, float inv_h, int& base, float N[Stencil<INTERP>::W],
                          float dN[Stencil<INTERP>::W]) {
     constexpr int W = Stencil<INTERP>::W;
-    const float u = x / h;  // IEEE division: the same base cell as the sort key
-    float fb = floorf(u - Stencil<INTERP>::SHIFT);
+    float fb = base_cell<INTERP>(x, h, inv_h);
     fb = fminf(fmaxf(fb, -1.0e9f), 1.0e9f);
     base = (int)fb;
-    // d_j = (x - (origin + h (base + j))) / h, evaluated as u - origin/h - (base + j)
-    const float u0 = u - origin * inv_h;
 #pragma unroll
     for (int j = 0; j < W; ++j) {
-        const float d = u0 - (fb + (float)j);
+        const float xj = origin + h * (fb + (float)j);
+        const float d = (x - xj) * inv_h;
         float n, dn;
         bspline<INTERP>(d, n, dn);
         N[j] = n;
@@ -271,16 +289,22 @@ MPM_HD void axis_weights(float x, float origin, float h, float inv_h, int& base,
 }
 
 // ------------------------------------------------------ particle cell key
-// Same function as the oracle's cell_key: bit-exact by construction (one
-// IEEE division, one subtraction, floor, integer arithmetic).
+// Same function as the oracle's cell_key: bit-exact by construction (the
+// base cell of base_cell, then integer arithmetic).  width = stencil nodes
+// per axis (2 = linear).
 MPM_HD uint32_t cell_key(const SceneDev& S, float x, float y, float z, float shift, int width) {
     float p[3] = {x, y, z};
     int res[3] = {S.rx, S.ry, S.rz};
     int b[3];
 #pragma unroll
     for (int i = 0; i < 3; ++i) {
-        float u = p[i] / S.h;
-        float fb = floorf(u - shift);
+        float fb;  // base_cell
+        if (width == 2) {
+            fb = floorf(p[i] / S.h);
+            if (fmaf(-S.h, fb, p[i]) < 0.0f) fb -= 1.0f;
+        } else {
+            fb = floorf(p[i] * S.inv_h - shift);
+        }
         bool ok = (fb >= (float)(-(width - 1))) && (fb <= (float)(res[i] - 1));
         if (!ok) return S.escape_key;
         b[i] = (int)fb;

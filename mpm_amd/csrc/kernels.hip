@@ -377,7 +377,7 @@ __device__ __forceinline__ SegPlan seg_plan(uint64_t headmask, bool active, int 
 
 #define MPM_FMAC_DPP(CTRL) "v_fmac_f32_dpp %0, %0, %1 " CTRL " bank_mask:0xf bound_ctrl:1"
 #define MPM_FMAC_DPP7(CTRL)                                                                    \
-    "s_nop 4\n\t"                                                                              \
+    "s_nop 1\n\t"                                                                              \
     "v_fmac_f32_dpp %0, %0, %7 " CTRL " bank_mask:0xf bound_ctrl:1\n\t"                       \
     "v_fmac_f32_dpp %1, %1, %7 " CTRL " bank_mask:0xf bound_ctrl:1\n\t"                       \
     "v_fmac_f32_dpp %2, %2, %7 " CTRL " bank_mask:0xf bound_ctrl:1\n\t"                       \
@@ -387,7 +387,7 @@ __device__ __forceinline__ SegPlan seg_plan(uint64_t headmask, bool active, int 
     "v_fmac_f32_dpp %6, %6, %7 " CTRL " bank_mask:0xf bound_ctrl:1"
 
 #define MPM_FMAC_DPP4(CTRL)                                                                    \
-    "s_nop 4\n\t"                                                                              \
+    "s_nop 1\n\t"                                                                              \
     "v_fmac_f32_dpp %0, %0, %4 " CTRL " bank_mask:0xf bound_ctrl:1\n\t"                       \
     "v_fmac_f32_dpp %1, %1, %4 " CTRL " bank_mask:0xf bound_ctrl:1\n\t"                       \
     "v_fmac_f32_dpp %2, %2, %4 " CTRL " bank_mask:0xf bound_ctrl:1\n\t"                       \
@@ -404,7 +404,7 @@ __device__ __forceinline__ SegPlan seg_plan(uint64_t headmask, bool active, int 
                          : "v"(flag));                                                          \
         } else {                                                                                \
             for (int f_ = 0; f_ < NV; ++f_)                                                     \
-                asm volatile("s_nop 4\n\t" MPM_FMAC_DPP(CTRL) : "+v"(v[f_]) : "v"(flag));        \
+                asm volatile("s_nop 1\n\t" MPM_FMAC_DPP(CTRL) : "+v"(v[f_]) : "v"(flag));        \
         }                                                                                       \
     } while (0)
 
@@ -459,6 +459,9 @@ __global__ void __launch_bounds__(64) k_p2g(SceneDev S0, ParticlesDev P0, ItemsD
             axis_weights<INTERP>(x1, S.oy, S.h, S.inv_h, by, Ny, dNy);
             axis_weights<INTERP>(x2, S.oz, S.h, S.inv_h, bz, Nz, dNz);
             const int lx = bx - g.ox, ly = by - g.oy, lz = bz - g.oz;
+            // tile cell of the stencil's first node; node (a, b, c) is at
+            // tq[(c * T + b) * T + a] (constant offsets)
+            float* const tq = tile + ((lz * T + ly) * T + lx);
             const float m = active ? P.m[p] : 0.0f;
             if (MODE == 1) {
 #pragma unroll
@@ -469,8 +472,8 @@ __global__ void __launch_bounds__(64) k_p2g(SceneDev S0, ParticlesDev P0, ItemsD
                         for (int c = 0; c < W; ++c) {
                             float v[1] = {m * ((Nx[a] * Ny[b]) * Nz[c])};
                             seg_scan<1>(v, sp);
-                            const int q = ((lz + c) * T + (ly + b)) * T + (lx + a);
-                            if (writer) tile[q] += v[0];
+                            const int q = (c * T + b) * T + a;
+                            if (writer) tq[q] += v[0];
                         }
                 continue;
             }
@@ -548,10 +551,10 @@ __global__ void __launch_bounds__(64) k_p2g(SceneDev S0, ParticlesDev P0, ItemsD
                             }
                         }
                         seg_scan<NF>(v, sp);
-                        const int q = ((lz + c) * T + (ly + b)) * T + (lx + a);
+                        const int q = (c * T + b) * T + a;
                         if (writer) {
 #pragma unroll
-                            for (int f = 0; f < NF; ++f) tile[f * TN + q] += v[f];
+                            for (int f = 0; f < NF; ++f) tq[f * TN + q] += v[f];
                         }
                     }
             }
@@ -732,6 +735,8 @@ __global__ void __launch_bounds__(WG) __attribute__((amdgpu_waves_per_eu(G2PWave
                         dyz[b][c] = dNy[b] * Nz[c];
                         ydz[b][c] = Ny[b] * dNz[c];
                     }
+                // tile cell of the stencil's first node (constant offsets per node)
+                const float* const tq = tile + ((lz * T + ly) * T + lx);
 #pragma unroll
                 for (int a = 0; a < W; ++a) {
                     float S0[3] = {0.0f, 0.0f, 0.0f}, Sy[3] = {0.0f, 0.0f, 0.0f}, Sz[3] = {0.0f, 0.0f, 0.0f};
@@ -740,8 +745,8 @@ __global__ void __launch_bounds__(WG) __attribute__((amdgpu_waves_per_eu(G2PWave
                     for (int b = 0; b < W; ++b)
 #pragma unroll
                         for (int c = 0; c < W; ++c) {
-                            const int q = ((lz + c) * T + (ly + b)) * T + (lx + a);
-                            const float wv[3] = {tile[q], tile[TN + q], tile[2 * TN + q]};
+                            const int q = (c * T + b) * T + a;
+                            const float wv[3] = {tq[q], tq[TN + q], tq[2 * TN + q]};
 #pragma unroll
                             for (int k = 0; k < 3; ++k) {
                                 S0[k] += yz[b][c] * wv[k];
@@ -777,9 +782,9 @@ __global__ void __launch_bounds__(WG) __attribute__((amdgpu_waves_per_eu(G2PWave
                         for (int b = 0; b < W; ++b)
 #pragma unroll
                             for (int c = 0; c < W; ++c) {
-                                const int q = ((lz + c) * T + (ly + b)) * T + (lx + a);
+                                const int q = (c * T + b) * T + a;
 #pragma unroll
-                                for (int k = 0; k < 3; ++k) SF[k] += yz[b][c] * tile[(3 + k) * TN + q];
+                                for (int k = 0; k < 3; ++k) SF[k] += yz[b][c] * tq[(3 + k) * TN + q];
                             }
                         const float xa = Nx[a];
                         vf0 += xa * SF[0]; vf1 += xa * SF[1]; vf2 += xa * SF[2];

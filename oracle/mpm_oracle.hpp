@@ -665,11 +665,14 @@ void density_raster(const mpm_desc& d, const float* pos, const float* mass, int6
 //   key = block_linear(((b+4)>>2)) * 64 + in-block cell ((b+4)&3)
 // A particle whose stencil misses the grid on some axis (b outside
 // [-(S-1), res-1], or x not finite) gets the escape key, one past the largest
-// block key.  Evaluated in float with IEEE division, as on the device.
+// block key.  Evaluated in float as on the device: b = floor(x * fl(1/h) - s)
+// (one rounded product); linear: b = floor(fl(x/h)), lowered by one when
+// x < h * b exactly (the quotient rounded up onto an integer).
 struct KeyParams {
     int res[3];
     int kbn[3];
     float h;
+    float inv_h;
     float shift;
     int width;
     uint32_t escape;
@@ -681,6 +684,8 @@ inline KeyParams key_params(const mpm_desc& d) {
         k.kbn[i] = ((d.resolution[i] + 3) >> 2) + 1;
     }
     k.h = d.stride;
+    volatile float one = 1.0f;
+    k.inv_h = one / d.stride;
     k.shift = d.interp == MPM_INTERP_LINEAR ? 0.0f : d.interp == MPM_INTERP_QUADRATIC_BSPLINE ? 0.5f : 1.0f;
     k.width = d.interp == MPM_INTERP_LINEAR ? 2 : d.interp == MPM_INTERP_QUADRATIC_BSPLINE ? 3 : 4;
     k.escape = uint32_t(k.kbn[0]) * uint32_t(k.kbn[1]) * uint32_t(k.kbn[2]) * 64u;
@@ -689,9 +694,16 @@ inline KeyParams key_params(const mpm_desc& d) {
 inline uint32_t cell_key(const KeyParams& k, const float* x) {
     int b[3];
     for (int i = 0; i < 3; ++i) {
-        volatile float u = x[i] / k.h;
-        volatile float t = u - k.shift;
-        float fb = std::floor(t);
+        float fb;
+        if (k.width == 2) {
+            volatile float u = x[i] / k.h;
+            fb = std::floor(u);
+            if (std::fma(-k.h, fb, x[i]) < 0.0f) fb -= 1.0f;
+        } else {
+            volatile float u = x[i] * k.inv_h;
+            volatile float t = u - k.shift;
+            fb = std::floor(t);
+        }
         if (!(fb >= float(-(k.width - 1)) && fb <= float(k.res[i] - 1))) return k.escape;
         b[i] = int(fb);
     }

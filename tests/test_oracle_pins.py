@@ -302,8 +302,16 @@ def _keys_numpy(desc, pos):
     res = np.array(desc.resolution)
     kbn = ((res + 3) >> 2) + 1
     with np.errstate(invalid="ignore"):
-        u = pos.astype(np.float32) / np.float32(desc.stride)
-        fb = np.floor(u - np.float32(shift))
+        inv_h = np.float32(1.0) / np.float32(desc.stride)
+        p32 = pos.astype(np.float32)
+        if desc.interp == N.LINEAR:
+            fb = np.floor(p32 / np.float32(desc.stride))
+            # x < h * b exactly (float64 holds h * b exactly; the sign of
+            # the difference is exact)
+            below = p32.astype(np.float64) - np.float64(np.float32(desc.stride)) * fb.astype(np.float64) < 0
+            fb = np.where(below, fb - 1, fb)
+        else:
+            fb = np.floor(p32 * inv_h - np.float32(shift))
         ok = np.all((fb >= -(width - 1)) & (fb <= res - 1), axis=1)
     b = np.where(ok[:, None], fb, 0).astype(np.int64)
     blk = ((b + 4) >> 2)
@@ -314,11 +322,12 @@ def _keys_numpy(desc, pos):
     return np.where(ok, key, esc).astype(np.uint32)
 
 
+@pytest.mark.parametrize("stride", [1 / 32, 1 / 30])
 @pytest.mark.parametrize("interp", [N.LINEAR, N.QUADRATIC_BSPLINE, N.CUBIC_BSPLINE])
-def test_keys_and_stable_sort(interp):
+def test_keys_and_stable_sort(interp, stride):
     from mpm_amd.solver import SolverDesc
     rng = np.random.default_rng(6)
-    d = SolverDesc(resolution=(20, 33, 17), stride=1 / 32, interp=interp)
+    d = SolverDesc(resolution=(20, 33, 17), stride=stride, interp=interp)
     pos = rng.uniform(-0.2, 1.2, (5000, 3)).astype(np.float32)
     pos[:50] = (np.round(pos[:50] * 64) / 64).astype(np.float32)  # exact cell-boundary values
     pos[50, 0] = np.nan
@@ -326,3 +335,39 @@ def test_keys_and_stable_sort(interp):
     assert np.array_equal(k, _keys_numpy(d, pos))
     order = O.stable_argsort(k)
     assert np.array_equal(order, np.argsort(k, kind="stable").astype(np.uint32))
+
+
+@pytest.mark.parametrize("interp,r,shift", [(N.QUADRATIC_BSPLINE, 1.5, 0.5), (N.CUBIC_BSPLINE, 2.0, 1.0),
+                                            (N.LINEAR, 1.0, 0.0)])
+def test_fixed_stencil_matches_reference_range(interp, r, shift):
+    """The device stencil starts at floor(fl(x * fl(1/h)) - s) (linear:
+    floor(fl(x/h)), lowered by one where x < h * floor(fl(x/h))); the
+    reference loops over ceil(fl(x/h) - r)..floor(fl(x/h) + r)
+    (src/mpm_solver3d.cu:251-255).  Where the two node sets differ (x/h - r
+    within rounding of an integer) every node in one set and not the other
+    has weight < 1e-9 and |N'| < 1e-4 (evaluated at the exact d)."""
+    rng = np.random.default_rng(12)
+    h = np.float32(1 / 200)  # not a power of two: fl(1/h) is inexact
+    inv_h = np.float32(1.0) / h
+    k = rng.integers(2, 190, 200000)
+    eps = rng.uniform(-4e-5, 4e-5, k.size)
+    x = ((k + r + eps) * np.float64(h)).astype(np.float32)  # x/h - r near integers
+    lo = np.ceil(x / h - np.float32(r)).astype(np.int64)
+    hi = np.floor(x / h + np.float32(r)).astype(np.int64)
+    W = {N.LINEAR: 2, N.QUADRATIC_BSPLINE: 3, N.CUBIC_BSPLINE: 4}[interp]
+    if interp == N.LINEAR:
+        base = np.floor(x / h).astype(np.int64)
+        base -= (x.astype(np.float64) - np.float64(h) * base < 0).astype(np.int64)
+    else:
+        base = np.floor(x * inv_h - np.float32(shift)).astype(np.int64)
+    odd = np.nonzero((lo != base) | (hi != base + W - 1))[0]
+    if interp != N.LINEAR:
+        assert odd.size > 0  # the window was hit
+    nodes, xs = [], []
+    for i in odd:
+        for node in set(range(lo[i], hi[i] + 1)) ^ set(range(base[i], base[i] + W)):
+            nodes.append(node)
+            xs.append(x[i])
+    d = (np.asarray(xs, np.float64) - np.float64(h) * np.asarray(nodes)) / np.float64(h)
+    n, dn = O.weights(interp, d)
+    assert np.abs(n).max() < 1e-9 and np.abs(dn).max() < 1e-4, (np.abs(n).max(), np.abs(dn).max())
