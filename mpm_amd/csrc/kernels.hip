@@ -340,9 +340,11 @@ __device__ __forceinline__ void store_m3(float* base, int64_t N, uint32_t p, con
 // segment and 0.0 otherwise (bound_ctrl gives 0 for sources outside the
 // row; row_mask keeps the rows a broadcast does not target).  The compiler
 // does not fold a DPP move into a tied FMAC, so the step is written as
-// inline assembly; each block starts with s_nop 4, which covers the
-// VALU-write -> DPP-read (2) and VALU-exec-write -> DPP (5) wait states for
-// the values computed just before it.  Within a block the 7 values are
+// inline assembly; each block starts with s_nop 1, the 2 wait states a DPP
+// read needs after a VALU write of its source (the values are computed just
+// before the first block).  The 5 wait states a DPP needs after a VALU
+// write of EXEC do not arise: these kernels change EXEC with scalar
+// instructions only (no v_cmpx).  Within a block the 4 or 7 values are
 // independent, so successive blocks need no further waits.  Steps that no
 // lane needs are skipped (uniform branches).
 struct SegPlan {
