@@ -67,12 +67,29 @@ static unsigned resident_grid(F* kernel, int block, int requested) {
 // PIC-FLIP G2P (187-256 VGPRs) would spill 124-252 B per lane under the
 // same cap, so it keeps the compiler's allocation unless MPM_PIC_G2P_WAVES
 // says otherwise.
+#ifndef MPM_APIC_G2P_WAVES
+#define MPM_APIC_G2P_WAVES 3
+#endif
 #ifndef MPM_PIC_G2P_WAVES
 #define MPM_PIC_G2P_WAVES 1
 #endif
 template <int APIC>
 struct G2PWaves {
-    static constexpr int value = APIC ? 3 : MPM_PIC_G2P_WAVES;
+    static constexpr int value = APIC ? MPM_APIC_G2P_WAVES : MPM_PIC_G2P_WAVES;
+};
+
+// Minimum waves per SIMD for P2G.  The folded (4-value) and mass-only
+// P2G of the linear and quadratic stencils fit 128 VGPRs without spills, so
+// they run 4 waves per SIMD instead of 3 (measured: APIC P2G 0.674 ->
+// 0.616 ms on the 8.4M slab).  The 7-value P2G and the cubic stencil (64
+// nodes; 48 VGPRs of spill under the same cap) keep the compiler's
+// allocation.
+#ifndef MPM_P2G_WAVES
+#define MPM_P2G_WAVES 4
+#endif
+template <int INTERP, int MODE>
+struct P2GWaves {
+    static constexpr int value = (MODE == 0 || INTERP == MPM_INTERP_CUBIC_BSPLINE) ? 1 : MPM_P2G_WAVES;
 };
 
 // ------------------------------------------------------------------ keys
@@ -424,7 +441,7 @@ __device__ __forceinline__ void seg_scan(float (&v)[N], const SegPlan& sp) {
 }
 
 template <int INTERP, int APIC, int MODE, bool BATCH>
-__global__ void __launch_bounds__(64) k_p2g(SceneDev S0, ParticlesDev P0, ItemsDev I,
+__global__ void __launch_bounds__(64) __attribute__((amdgpu_waves_per_eu(P2GWaves<INTERP, MODE>::value))) k_p2g(SceneDev S0, ParticlesDev P0, ItemsDev I,
                                             const SceneEntry* __restrict__ tab) {
     constexpr int W = Stencil<INTERP>::W;
     constexpr int T = 4 + W - 1;
