@@ -204,6 +204,14 @@ MPM_API int mpm_set_particles_collision(void* handle, int enable); /* field :23 
  * after simulateOneStep, src/apic_mpm_solver3d.cu:99-124) at the cost of
  * the wider P2G.  No effect on PIC-FLIP, whose G2P needs v and v*. */
 MPM_API int mpm_set_grid_record(void* handle, int enable);
+/* Store reorder (no counterpart in the reference, whose particle array is
+ * never reordered): every `steps` steps (default 100; 0 = never) the
+ * particle store is permuted in place into the cell order of the current
+ * positions, so that the per-step reads and writes stay contiguous as
+ * particles move (on the 8.4M slab G2P slows from 0.88 to 1.19 ms over
+ * 1000 steps without it).  Results do not depend on the interval beyond
+ * float-atomic order; reads return the caller's particle order. */
+MPM_API int mpm_set_reorder_interval(void* handle, int steps);
 
 /* Particle-by-cell key generation + stable LSD radix sort (no reference
  * counterpart: the reference scatters in array order,
@@ -336,6 +344,9 @@ MPM_API int mpm_batch_step(void* batch, int n);
 /* Same, replaying one step captured as a hipGraph (recaptured after a
  * membership or tuning change). */
 MPM_API int mpm_batch_step_graph(void* batch, int n);
+/* The batch's store reorder interval (as mpm_set_reorder_interval, for
+ * every member; default 100, 0 = never). */
+MPM_API int mpm_batch_set_reorder_interval(void* batch, int steps);
 MPM_API int mpm_batch_synchronize(void* batch);
 /* Fixes the launch parameters; NULL returns to the tuning table. */
 MPM_API int mpm_batch_set_tune(void* batch, const mpm_tune* tune);

@@ -86,6 +86,11 @@ class Batch:
     def synchronize(self):
         N.check(self._lib.mpm_batch_synchronize(self._h))
 
+    def set_reorder_interval(self, steps: int):
+        """Permute every member's store into cell order every `steps` batch
+        steps (0 = never; default 100)."""
+        N.check(self._lib.mpm_batch_set_reorder_interval(self._h, int(steps)))
+
     def info(self) -> dict:
         s = N.MpmBatchStats()
         N.check(self._lib.mpm_batch_info(self._h, C.byref(s)))

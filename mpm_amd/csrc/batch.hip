@@ -80,6 +80,12 @@ struct Batch {
     Tune tune;
     bool tune_fixed = false;  // set explicitly: the tuning table is not consulted
     bool fold = false;        // APIC with no member recording its grid: 4-plane P2G
+    // every reorder_interval steps (0 = never) each member's store is
+    // permuted into its range of the global cell order; scratch 9 * cap words
+    int reorder_interval = 100;
+    int64_t last_reorder = 0;
+    uint32_t* reorder_scratch = nullptr;
+    int64_t reorder_cap = 0;
     // graph
     hipGraph_t graph = nullptr;
     hipGraphExec_t gexec = nullptr;
@@ -315,6 +321,33 @@ void collect_times(Batch& b) {
 }
 
 // Runs n steps eagerly (graph = false) or as replays of a captured step.
+// Sort the global keys, permute every member's store into its range of
+// the sorted order and make the sorted keys the next sort's input.
+int reorder_members(Batch& b) {
+    b.last_reorder = b.steps;
+    if (b.ntot <= 0) return MPM_OK;
+    if (b.reorder_cap < b.ntot) {
+        (void)hipStreamSynchronize(b.st);
+        bfree(b, *(void**)&b.reorder_scratch);
+        b.reorder_cap = 0;
+        int rc = balloc(b, (void**)&b.reorder_scratch, (size_t)b.cap * 9 * 4);
+        if (rc) return rc;
+        b.reorder_cap = b.cap;
+    }
+    const int sel = radix_sort_pairs(b.kin, b.iota, b.ka, b.va, b.kb, b.vb, b.ntot, b.key_bits, b.hist, b.scan_temp,
+                                     b.st, ProfileHook{});
+    const uint32_t* sorted = sel ? b.kb : b.ka;
+    const uint32_t* perm = sel ? b.vb : b.va;
+    launch_reorder_store(ParticlesDev{}, b.tab, (int)b.members.size(), perm, b.ntot, b.reorder_scratch, b.apic != 0,
+                         b.st);
+    HIP_TRY(hipMemcpyAsync(b.kin, sorted, (size_t)b.ntot * 4, hipMemcpyDeviceToDevice, b.st));
+    for (Solver* m : b.members) {
+        m->keys_valid = false;  // the member's own sort input is in the old slot order
+        m->sort_valid = false;
+    }
+    return MPM_OK;
+}
+
 int run_steps(Batch& b, int n, bool graph) {
     int rc = sync_members(b);
     if (rc) return rc;
@@ -338,6 +371,11 @@ int run_steps(Batch& b, int n, bool graph) {
             e0 = take_event(b);
             e1 = take_event(b);
             if (e0 && e1) HIP_TRY(hipEventRecord(e0, b.st));
+        }
+        if (b.reorder_interval > 0 && b.steps + i - b.last_reorder >= b.reorder_interval) {
+            int rc2 = reorder_members(b);
+            if (rc2) return rc2;
+            b.last_reorder = b.steps + i;
         }
         if (graph) HIP_TRY(hipGraphLaunch(b.gexec, b.st));
         else enqueue_step(b);
@@ -515,6 +553,13 @@ MPM_API int mpm_batch_step(void* batch, int n) {
     GET_BATCH(b, batch);
     if (n < 0) return fail(MPM_ERR_INVALID, "n < 0");
     return run_steps(*b, n, false);
+}
+
+MPM_API int mpm_batch_set_reorder_interval(void* batch, int steps) {
+    GET_BATCH(b, batch);
+    if (steps < 0) return fail(MPM_ERR_INVALID, "steps < 0");
+    b->reorder_interval = steps;
+    return MPM_OK;
 }
 
 MPM_API int mpm_batch_step_graph(void* batch, int n) {

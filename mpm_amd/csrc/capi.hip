@@ -580,11 +580,49 @@ MPM_API int mpm_create_state(const mpm_desc* desc, const mpm_particle_state* st,
 
 MPM_API void mpm_destroy(void* handle) { destroy(as_solver(handle)); }
 
+// Permute the particle store into the cell order of the current positions
+// (the slots each item reads become contiguous again); afterwards the
+// sort's permutation is the identity and its input keys are the sorted keys.
+static int reorder_store(Solver& s) {
+    s.last_reorder = s.steps;
+    if (s.N <= 0) return MPM_OK;
+    if (!s.reorder_scratch) {
+        int rc = dalloc(s, (void**)&s.reorder_scratch, (size_t)s.N * 9 * 4);
+        if (rc) return rc;
+    }
+    ProfileHook h = hook(s);
+    if (!s.sort_valid) run_sort(s);
+    Scope sc(h, "reorder");
+    launch_reorder_store(s.P, nullptr, 0, s.I.perm, s.N, s.reorder_scratch, s.apic != 0, s.st);
+    HIP_TRY(hipMemcpyAsync(s.kin, s.I.sorted_keys, (size_t)s.N * 4, hipMemcpyDeviceToDevice, s.st));
+    s.I.perm = s.iota;
+    s.keys_valid = true;
+    s.sort_valid = true;
+    return MPM_OK;
+}
+
+static bool reorder_due(const Solver& s) {
+    return s.reorder_interval > 0 && s.steps - s.last_reorder >= s.reorder_interval;
+}
+
+MPM_API int mpm_set_reorder_interval(void* handle, int steps) {
+    GET_SOLVER(s, handle);
+    if (steps < 0) return fail(MPM_ERR_INVALID, "steps < 0");
+    s->reorder_interval = steps;
+    return MPM_OK;
+}
+
 MPM_API int mpm_step(void* handle, int n) {
     GET_SOLVER(s, handle);
     if (n < 0) return fail(MPM_ERR_INVALID, "n < 0");
     auto t0 = std::chrono::steady_clock::now();
-    for (int i = 0; i < n; ++i) run_fused_step(*s);
+    for (int i = 0; i < n; ++i) {
+        if (reorder_due(*s)) {
+            int rc = reorder_store(*s);
+            if (rc) return rc;
+        }
+        run_fused_step(*s);
+    }
     int rc = finish(*s);
     auto t1 = std::chrono::steady_clock::now();
     s->last_step_ms = std::chrono::duration<double, std::milli>(t1 - t0).count();
@@ -623,7 +661,8 @@ MPM_API int mpm_step_graph(void* handle, int n) {
         s->force_dirty = !fold;
         s->sort_valid = false;
         run_fused_step(*s);
-        s->steps -= 1;
+        s->steps -= 1;  // the capture ran nothing
+        s->version -= 1;
         hipError_t e = hipStreamEndCapture(s->st, &s->graph);
         s->prof = prof;
         (void)dirty;
@@ -631,8 +670,17 @@ MPM_API int mpm_step_graph(void* handle, int n) {
         HIP_TRY(hipGraphInstantiate(&s->gexec, s->graph, nullptr, nullptr, 0));
     }
     auto t0 = std::chrono::steady_clock::now();
-    for (int i = 0; i < n; ++i) HIP_TRY(hipGraphLaunch(s->gexec, s->st));
-    s->steps += n;
+    for (int i = 0; i < n; ++i) {
+        if (reorder_due(*s)) {
+            // between replays: the eager sort of the keys the last replay's
+            // G2P wrote, then the permutation (the next replay sorts again)
+            s->sort_valid = false;
+            int rc = reorder_store(*s);
+            if (rc) return rc;
+        }
+        HIP_TRY(hipGraphLaunch(s->gexec, s->st));
+        s->steps += 1;
+    }
     s->version += (uint64_t)n;
     s->grid_dirty = true;
     s->count_on_clear = true;

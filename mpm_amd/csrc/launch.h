@@ -47,13 +47,14 @@ void launch_items(const uint32_t* sorted_keys, int64_t n, uint32_t* flags, uint3
 
 // grid_wgs: persistent workgroups; 0 = one resident round (occupancy of the
 // kernel instance x compute units).
-// mode: 0 = full P2G (mass, momentum, force), 1 = mass only (initialize).
+// mode: 0 = full P2G (mass, momentum, force), 1 = mass only (initialize,
+// density raster), 2 = folded APIC grid (mass, p + dt f).
 // With a scene table (tab != null, I.scene set) every item reads its scene
-// from the table (mode 0 only); S and P are then unused.
+// from the table (modes 0 and 2); S and P are then unused.
 void launch_p2g(const SceneDev& S, const ParticlesDev& P, const ItemsDev& I, int interp, int apic, int mode,
                 int grid_wgs, hipStream_t st, const SceneEntry* tab = nullptr);
 // src: 0 = fused grid update from P2G fields, 1 = materialised v / v*,
-//      2 = initialize() volume gather
+//      2 = initialize() volume gather, 3 = fused update from the folded grid
 // next_keys (may be null): with advect, the cell key of each particle's new
 // position is written at its storage slot (input of the next sort); in a
 // batch at the scene's global slot, plus its key offset.
@@ -86,6 +87,12 @@ void launch_unpack(const ParticlesDev& P, const HostLayoutDev& H, hipStream_t st
 void launch_positions4(const ParticlesDev& P, float* out, hipStream_t st);
 void launch_permute(const ParticlesDev& src, const ParticlesDev& dst, const uint32_t* perm, hipStream_t st);
 void launch_gather_u32(const uint32_t* src, const uint32_t* idx, uint32_t* dst, int64_t n, hipStream_t st);
+// In-place permutation of the particle store into sorted order (slot i <-
+// slot perm[i]) through scratch (9 * ntot words).  With a scene table the
+// slots are global (member s owns [pstart_s, pstart_s + n_s)) and P is
+// unused; otherwise P is the one store and ntot = P.n.
+void launch_reorder_store(const ParticlesDev& P, const SceneEntry* tab, int nscenes, const uint32_t* perm,
+                          int64_t ntot, uint32_t* scratch, bool apic, hipStream_t st);
 void launch_grid_dense(const SceneDev& S, float* mass, float* vel, float* vstar, float* force, int have_vstar,
                        hipStream_t st);
 // Density raster readout from a one-plane mass grid D (D.grid = sum m w):

@@ -109,6 +109,11 @@ struct Solver {
     hipStream_t copy_st = nullptr;
     hipEvent_t pos4_packed = nullptr, pos4_copied = nullptr;
     bool pos4_pending = false;
+    // store reorder: every reorder_interval steps (0 = never) the particle
+    // store is permuted into the current cell order; scratch 9N words
+    int reorder_interval = 100;
+    int64_t last_reorder = 0;
+    uint32_t* reorder_scratch = nullptr;
 };
 
 

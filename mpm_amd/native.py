@@ -158,6 +158,7 @@ SIGNATURES = {
     "mpm_switch_particles_collision": (C.c_int, [C.c_void_p]),
     "mpm_set_particles_collision": (C.c_int, [C.c_void_p, C.c_int]),
     "mpm_set_grid_record": (C.c_int, [C.c_void_p, C.c_int]),
+    "mpm_set_reorder_interval": (C.c_int, [C.c_void_p, C.c_int]),
     "mpm_sort": (C.c_int, [C.c_void_p]),
     "mpm_read_particles": (C.c_int, [C.c_void_p, C.POINTER(MpmParticleState)]),
     "mpm_write_particles": (C.c_int, [C.c_void_p, C.POINTER(MpmParticleState)]),
@@ -186,6 +187,7 @@ SIGNATURES = {
     "mpm_batch_step": (C.c_int, [C.c_void_p, C.c_int]),
     "mpm_batch_step_graph": (C.c_int, [C.c_void_p, C.c_int]),
     "mpm_batch_synchronize": (C.c_int, [C.c_void_p]),
+    "mpm_batch_set_reorder_interval": (C.c_int, [C.c_void_p, C.c_int]),
     "mpm_batch_set_tune": (C.c_int, [C.c_void_p, C.POINTER(MpmTune)]),
     "mpm_batch_autotune": (C.c_int, [C.c_void_p, C.c_int, C.POINTER(MpmTune)]),
     "mpm_tune_lookup": (C.c_int, [C.c_int32, C.c_int32, C.c_int64, C.POINTER(MpmTune)]),

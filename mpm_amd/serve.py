@@ -37,9 +37,10 @@ _STAGE_FLOATS_PIC = 34
 def estimate_bytes(desc: SolverDesc, n: int) -> int:
     """Peak device bytes of one scene: particle store (twice, for the build's
     physical reorder), its sort buffers and the batch's per-particle share,
-    the 10-plane block-major grid and its touched flags."""
+    the store-reorder scratch (9 words per particle, the scene's and its
+    batch's), the 10-plane block-major grid and its touched flags."""
     f = _STAGE_FLOATS_APIC if desc.transfer == N.APIC else _STAGE_FLOATS_PIC
-    per_particle = 2 * (4 * f + 6) + 8 * 4 + 9 * 4 + 4
+    per_particle = 2 * (4 * f + 6) + 8 * 4 + 9 * 4 + 4 + 2 * 9 * 4
     blocks = 1
     for r in desc.resolution:
         blocks *= (int(r) + 3) // 4

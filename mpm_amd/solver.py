@@ -216,6 +216,11 @@ class Solver:
         read_grid works after it (default: folded, faster)."""
         N.check(self._lib.mpm_set_grid_record(self._h, 1 if on else 0))
 
+    def set_reorder_interval(self, steps: int):
+        """Permute the particle store into cell order every `steps` steps
+        (0 = never; default 100)."""
+        N.check(self._lib.mpm_set_reorder_interval(self._h, int(steps)))
+
     def initialize(self):
         N.check(self._lib.mpm_initialize(self._h))
 

@@ -23,3 +23,11 @@ def grid_errors(g, r, fields=("mass", "velocity", "velocity_star", "force")) -> 
 
 def report(tag: str, errs: dict):
     print(f"[{tag}] " + ", ".join(f"{k}={v:.3e}" for k, v in errs.items()), flush=True)
+
+
+def particle_errors_apic(pa, pb, h) -> dict:
+    """x, v, Fe, Fp normwise, and B relative to max|v| * h (B ~ v h, and B is
+    ~0 for a near-uniform velocity field, where its own norm says nothing)."""
+    e = particle_errors(pa, pb, ("position", "velocity", "Fe", "Fp"))
+    e["B"] = float(np.abs(pa.B.astype(np.float64) - pb.B).max() / (np.abs(pb.velocity).max() * h + 1e-30))
+    return e
