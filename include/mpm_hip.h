@@ -137,6 +137,11 @@ typedef struct mpm_stats {
     int64_t escaped;            /* particles whose stencil misses the grid   */
     int64_t device_bytes;       /* device memory owned by the handle         */
     double last_step_ms;        /* host wall time of the last mpm_step call  */
+    /* CFL monitor (the reference steps with a fixed dt and has no check,
+     * src/mpm_solver3d.cu:101): the largest particle speed |v| now, and
+     * max|v| * dt / h.  NaN if a velocity is NaN. */
+    double max_speed;
+    double cfl;
 } mpm_stats;
 
 /* Per-kernel timing (hipEvent pairs on the handle's stream), enabled by

@@ -1182,9 +1182,12 @@ MPM_API int mpm_get_stats(void* handle, mpm_stats* out) {
     HIP_TRY(hipMemcpyAsync(&s->acc[2], &zero, 8, hipMemcpyHostToDevice, s->st));
     if (s->grid_dirty) launch_count_active(s->S, s->nsb, &s->acc[1], s->st);
     if (s->I.sorted_keys && s->N > 0) launch_count_escaped(s->I.sorted_keys, s->N, s->S.escape_key, &s->acc[2], s->st);
-    unsigned long long a[3] = {0, 0, 0};
+    // acc[3] (low word): max |v|^2 bits
+    HIP_TRY(hipMemcpyAsync(&s->acc[3], &zero, 8, hipMemcpyHostToDevice, s->st));
+    launch_max_speed2(s->P, reinterpret_cast<unsigned int*>(&s->acc[3]), s->st);
+    unsigned long long a[4] = {0, 0, 0, 0};
     uint32_t items = 0;
-    HIP_TRY(hipMemcpyAsync(a, s->acc, 24, hipMemcpyDeviceToHost, s->st));
+    HIP_TRY(hipMemcpyAsync(a, s->acc, 32, hipMemcpyDeviceToHost, s->st));
     HIP_TRY(hipMemcpyAsync(&items, s->counters, 4, hipMemcpyDeviceToHost, s->st));
     HIP_TRY(hipStreamSynchronize(s->st));
     out->particles = s->N;
@@ -1195,6 +1198,11 @@ MPM_API int mpm_get_stats(void* handle, mpm_stats* out) {
     out->escaped = (int64_t)a[2];
     out->device_bytes = s->device_bytes;
     out->last_step_ms = s->last_step_ms;
+    float v2;
+    const uint32_t bits = (uint32_t)(a[3] & 0xFFFFFFFFull);
+    std::memcpy(&v2, &bits, 4);
+    out->max_speed = std::sqrt((double)v2);
+    out->cfl = out->max_speed * (double)s->S.dt / (double)s->S.h;
     return finish(*s);
 }
 

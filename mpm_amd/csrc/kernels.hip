@@ -1392,6 +1392,31 @@ void launch_density_blocks(const SceneDev& D, const uint32_t* touched, const uin
                        rank, nsb, origin, values, max_blocks);
 }
 
+// Largest |v|^2 over the particles, as the bits of a non-negative float
+// (ordered like unsigned integers; a NaN sorts above every number).
+__global__ void __launch_bounds__(256) k_max_speed2(ParticlesDev P, unsigned int* __restrict__ out) {
+    float m = 0.0f;
+    const int64_t N = P.n;
+    for (int64_t i = (int64_t)blockIdx.x * blockDim.x + threadIdx.x; i < N; i += (int64_t)gridDim.x * blockDim.x) {
+        const float a = P.v[i], b = P.v[N + i], c = P.v[2 * N + i];
+        const float q = (a * a + b * b) + c * c;
+        m = (q > m || q != q) ? q : m;
+    }
+    unsigned int u = __float_as_uint(m);
+#pragma unroll
+    for (int off = 32; off > 0; off >>= 1) {
+        const unsigned int o = __shfl_xor(u, off, 64);
+        u = o > u ? o : u;
+    }
+    if ((threadIdx.x & 63) == 0) atomicMax(out, u);
+}
+void launch_max_speed2(const ParticlesDev& P, unsigned int* out, hipStream_t st) {
+    if (P.n <= 0) return;
+    int64_t w = (P.n + 255) / 256;
+    if (w > 2048) w = 2048;
+    hipLaunchKernelGGL(k_max_speed2, dim3((unsigned)w), dim3(256), 0, st, P, out);
+}
+
 __global__ void __launch_bounds__(256) k_count_escaped(const uint32_t* __restrict__ sk, int64_t n, uint32_t esc,
                                                        unsigned long long* out) {
     unsigned long long cnt = 0;

@@ -102,6 +102,8 @@ void launch_density_dense(const SceneDev& D, float* out, hipStream_t st);
 void launch_touched_flags(const uint32_t* touched, int64_t n, uint32_t* flags, hipStream_t st);
 void launch_density_blocks(const SceneDev& D, const uint32_t* touched, const uint32_t* rank, int64_t nsb,
                            int32_t* origin, float* values, int64_t max_blocks, hipStream_t st);
+// Largest |v|^2 of the store into *out (float bits; *out must start at 0).
+void launch_max_speed2(const ParticlesDev& P, unsigned int* out, hipStream_t st);
 void launch_count_escaped(const uint32_t* sorted_keys, int64_t n, uint32_t escape_key, unsigned long long* out,
                           hipStream_t st);
 void launch_debug_svd(const float* A, int64_t n, float* U, float* s, float* V, hipStream_t st);

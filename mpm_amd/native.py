@@ -102,6 +102,8 @@ class MpmStats(C.Structure):
         ("escaped", C.c_int64),
         ("device_bytes", C.c_int64),
         ("last_step_ms", C.c_double),
+        ("max_speed", C.c_double),
+        ("cfl", C.c_double),
     ]
 
 

@@ -78,6 +78,7 @@ class RequestRecord:
     est_bytes: int
     admitted: float = -1.0
     finished: float = -1.0
+    cfl: float = 0.0  # max|v| dt / h of the final state (CFL monitor)
     result: object = None
 
 
@@ -125,6 +126,7 @@ class ServeReport:
     queue_s_p99: float
     peak_bytes: int
     peak_scenes: int
+    max_cfl: float = 0.0  # largest final-state CFL number over the requests
     records: list = field(default_factory=list, repr=False)
 
     def summary(self) -> dict:
@@ -184,6 +186,9 @@ class Server:
         b.remove(a.solver)
         if self.keep_results:
             a.rec.result = a.solver.read_particles()
+        stats = getattr(a.solver, "stats", None)
+        if stats is not None:
+            a.rec.cfl = float(stats()["cfl"])
         a.solver.close()
         self.used -= a.rec.est_bytes
         a.rec.finished = self.clock() - t0
@@ -269,7 +274,8 @@ class Server:
             request_s_p99=_pct([r.finished - r.arrival for r in done], 99),
             queue_s_p50=_pct([r.admitted - r.arrival for r in done], 50),
             queue_s_p99=_pct([r.admitted - r.arrival for r in done], 99),
-            peak_bytes=self.peak_bytes, peak_scenes=self.peak_scenes, records=self.records)
+            peak_bytes=self.peak_bytes, peak_scenes=self.peak_scenes,
+            max_cfl=max((r.cfl for r in done), default=0.0), records=self.records)
 
     def close(self):
         for a in list(self.active.values()):

@@ -79,3 +79,25 @@ def test_particles_outside_grid_match_oracle(transfer):
     e = particle_errors(gpu.read_particles(), ref.read_particles())
     report(f"escaped quarter ({'APIC' if transfer else 'PIC-FLIP'})", e)
     assert all(v <= 1e-5 for v in e.values()), e
+
+
+def test_cfl_monitor():
+    # free fall: every particle has |v| = g n dt exactly (see above), so
+    # max_speed and cfl = max_speed dt / h have closed forms
+    d = SolverDesc(resolution=(32, 32, 32), stride=1 / 32, transfer=N.APIC, dt=1e-3, gravity=9.8,
+                   interp=N.QUADRATIC_BSPLINE)
+    p = ParticleInput(np.array([[0.5, 0.6, 0.4], [0.3, 0.7, 0.6]]), np.zeros((2, 3)), 1.0, 10.0, 1.4e5, 0.2, 0.025,
+                      0.0075, 0)
+    s = Solver(d, p)
+    st0 = s.stats()
+    assert st0["max_speed"] == 0.0 and st0["cfl"] == 0.0
+    s.step(10)
+    st = s.stats()
+    v = 9.8 * 10 * 1e-3
+    assert st["max_speed"] == pytest.approx(v, rel=2e-5)
+    assert st["cfl"] == pytest.approx(v * 1e-3 * 32, rel=2e-5)
+    # a NaN velocity is reported, not hidden
+    state = s.read_particles()
+    state.velocity[1, 0] = np.nan
+    s.write_particles(state)
+    assert np.isnan(s.stats()["max_speed"])
