@@ -240,6 +240,42 @@ MPM_HD void bspline(float d, float& N, float& dN) {
     }
 }
 
+// The same N(d), N'(d) for node j of the fixed stencil, whose offset d lies
+// in a known piece: quadratic j = 0, 1, 2 in [0.5,1.5), [-0.5,0.5),
+// [-1.5,-0.5); cubic j = 0..3 in [1,2), [0,1), [-1,0), [-2,-1).  Each piece
+// is bspline's expression for that piece, so in-range values are the same
+// bits; the piece is chosen by j instead of by comparisons (no branches).
+// An offset that rounding puts just past a knot (by ~ulp of x/h) is
+// evaluated on the neighbouring piece, which agrees there to O(delta^2) in N
+// (both kernels are C1) and O(delta) in N'.  Linear keeps bspline.
+template <int INTERP, int J>
+MPM_HD void bspline_node(float d, float& N, float& dN) {
+#pragma clang fp contract(off)
+    const float xa = fabsf(d);
+    if (INTERP == MPM_INTERP_QUADRATIC_BSPLINE) {
+        if (J == 1) {
+            N = 0.75f - xa * xa;
+            dN = -2.0f * d;
+        } else {
+            const float t = 1.5f - xa;
+            N = 0.5f * (t * t);
+            dN = d - (J == 0 ? 1.5f : -1.5f);
+        }
+    } else if (INTERP == MPM_INTERP_CUBIC_BSPLINE) {
+        if (J == 1 || J == 2) {
+            const float x2 = xa * xa;
+            N = (0.5f * x2 * xa - x2) + (2.0f / 3.0f);
+            dN = 1.5f * d * xa - 2.0f * d;
+        } else {
+            const float t = 2.0f - xa;
+            N = (t * t * t) / 6.0f;
+            dN = (2.0f * d - (J == 0 ? 2.0f : -2.0f)) - 0.5f * (d * xa);
+        }
+    } else {
+        bspline<INTERP>(d, N, dN);
+    }
+}
+
 template <int INTERP> struct Stencil;
 template <> struct Stencil<MPM_INTERP_LINEAR> { static constexpr int W = 2; static constexpr float SHIFT = 0.0f; };
 template <> struct Stencil<MPM_INTERP_QUADRATIC_BSPLINE> { static constexpr int W = 3; static constexpr float SHIFT = 0.5f; };
@@ -277,14 +313,18 @@ MPM_HD void axis_weights(float x, float origin, float h, float inv_h, int& base,
     float fb = base_cell<INTERP>(x, h, inv_h);
     fb = fminf(fmaxf(fb, -1.0e9f), 1.0e9f);
     base = (int)fb;
+    float d[W];
+#pragma unroll
+    for (int j = 0; j < W; ++j) d[j] = (x - (origin + h * (fb + (float)j))) * inv_h;
+    float n[4], dn[4];
+    bspline_node<INTERP, 0>(d[0], n[0], dn[0]);
+    bspline_node<INTERP, 1>(d[1], n[1], dn[1]);
+    if constexpr (W > 2) bspline_node<INTERP, 2>(d[2], n[2], dn[2]);
+    if constexpr (W > 3) bspline_node<INTERP, 3>(d[3], n[3], dn[3]);
 #pragma unroll
     for (int j = 0; j < W; ++j) {
-        const float xj = origin + h * (fb + (float)j);
-        const float d = (x - xj) * inv_h;
-        float n, dn;
-        bspline<INTERP>(d, n, dn);
-        N[j] = n;
-        dN[j] = dn * inv_h;
+        N[j] = n[j];
+        dN[j] = dn[j] * inv_h;
     }
 }
 
