@@ -572,8 +572,12 @@ __global__ void __launch_bounds__(64) __attribute__((amdgpu_waves_per_eu(P2GWave
                         seg_scan<NF>(v, sp);
                         const int q = (c * T + b) * T + a;
                         if (writer) {
+                            // all reads before the writes: one LDS round trip per node
+                            float o[NF];
 #pragma unroll
-                            for (int f = 0; f < NF; ++f) tq[f * TN + q] += v[f];
+                            for (int f = 0; f < NF; ++f) o[f] = tq[f * TN + q];
+#pragma unroll
+                            for (int f = 0; f < NF; ++f) tq[f * TN + q] = o[f] + v[f];
                         }
                     }
             }
