@@ -53,9 +53,12 @@ def _scene(cfg, transfer):
     return desc, particles
 
 
-@pytest.mark.parametrize("apic", [False, True])
-def test_headless_snow_cube_matches_oracle(apic, tmp_path):
-    cfg_path = os.path.join(CFG_DIR, "snow_cube.yaml")
+@pytest.mark.parametrize("config,apic", [("snow_cube.yaml", False), ("snow_cube.yaml", True),
+                                          ("snow_two_balls_sparse.yaml", False),
+                                          ("snow_two_balls_sparse.yaml", True)])
+def test_headless_reference_configs_match_oracle(config, apic, tmp_path):
+    # the reference's two configurations whose particle files ship with it
+    cfg_path = os.path.join(CFG_DIR, config)
     cfg = _config(cfg_path)
     steps = 2
     out = str(tmp_path / "pos.f32")
@@ -69,13 +72,13 @@ def test_headless_snow_cube_matches_oracle(apic, tmp_path):
     assert r.returncode == 0
     pos = np.fromfile(out, dtype=np.float32).reshape(-1, 4)
     desc, particles = _scene(cfg, N.APIC if apic else N.PIC_FLIP)
-    assert pos.shape[0] == particles.count == 11907
+    assert pos.shape[0] == particles.count == {"snow_cube.yaml": 11907, "snow_two_balls_sparse.yaml": 63008}[config]
     assert np.all(pos[:, 3] == 1.0)
     o = pyoracle.OracleSolver(desc, particles, 32)
     o.step(steps)
     ref = o.read_particles().position
     e = {"position": rel_err(pos[:, :3], ref), "moved": float(np.abs(ref - particles.position).max())}
-    report(f"headless snow_cube {'APIC' if apic else 'PIC-FLIP'} {steps} steps", e)
+    report(f"headless {config} {'APIC' if apic else 'PIC-FLIP'} {steps} steps", e)
     assert e["moved"] > 0.0
     assert e["position"] <= 1e-5
 
@@ -131,3 +134,25 @@ def test_headless_runs_generated_binary_scene(tmp_path, transfer):
     e = {"position": rel_err(pos, ref), "moved": float(np.abs(ref[:, :3] - sc.particles.position).max())}
     report(f"headless generated scene {'APIC' if transfer else 'PIC-FLIP'}", e)
     assert e["moved"] > 0.0 and e["position"] <= 1e-5
+
+
+@pytest.mark.parametrize("config", ["snow_cube.yaml", "snow_two_balls_sparse.yaml"])
+@pytest.mark.parametrize("transfer", [N.PIC_FLIP, N.APIC])
+def test_reference_configs_full_state_vs_oracle(config, transfer):
+    # the reference's configurations through the Python loader (the builder's
+    # group order, stride * xyz, per-group parameters): velocities,
+    # deformation gradients and (APIC) B after 3 steps, not only positions
+    from mpm_amd.scene_io import load_yaml
+    from mpm_amd.solver import Solver
+    from tests.helpers import particle_errors, particle_errors_apic
+
+    desc, particles = load_yaml(os.path.join(CFG_DIR, config), transfer=transfer, base_dir=CFG_DIR)
+    gpu = Solver(desc, particles)
+    ref = pyoracle.OracleSolver(desc, particles, 32)
+    gpu.step(3)
+    ref.step(3)
+    g, r = gpu.read_particles(), ref.read_particles()
+    e = particle_errors_apic(g, r, desc.stride) if transfer == N.APIC else particle_errors(g, r)
+    e["volume0"] = rel_err(g.volume0, r.volume0)
+    report(f"{config} {'APIC' if transfer else 'PIC-FLIP'} 3 steps, full state", e)
+    assert all(v <= 1e-5 for v in e.values()), e
