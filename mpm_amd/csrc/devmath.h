@@ -268,7 +268,7 @@ MPM_HD void bspline_node(float d, float& N, float& dN) {
             dN = 1.5f * d * xa - 2.0f * d;
         } else {
             const float t = 2.0f - xa;
-            N = (t * t * t) / 6.0f;
+            N = (t * t * t) * (1.0f / 6.0f);  // within an ulp of the quotient
             dN = (2.0f * d - (J == 0 ? 2.0f : -2.0f)) - 0.5f * (d * xa);
         }
     } else {

@@ -245,7 +245,9 @@ __device__ __forceinline__ void stress_from_svd(const float U[3][3], const float
                                                 const float Fp[3][3], float mu0, float lam0, float V0,
                                                 const MaterialDev& mat, float tau[6], float Fe[3][3]) {
     const float Jp = m3_det(Fp);
-    const float factor = expf(mat.hardening * (1.0f - Jp));
+    // exp to a few ulp (hardware exp2 of x log2 e); the reference's FP64
+    // exp is narrowed to float (src/material_point3d.cu:160)
+    const float factor = __expf(mat.hardening * (1.0f - Jp));
     const float mu = mu0 * factor, la = lam0 * factor;
     float s[3] = {sv[0], sv[1], sv[2]};
     float d[3], c;
