@@ -24,7 +24,7 @@ int64_t scan_temp_words(int64_t n);
 void exclusive_scan(uint32_t* data, int64_t n, uint32_t* temp, uint32_t* total, hipStream_t st);
 int64_t radix_tiles(int64_t n);
 int64_t radix_hist_words(int64_t n);
-int radix_passes(int key_bits);
+int radix_passes(int key_bits, int64_t n);
 int radix_sort_pairs(const uint32_t* kin, const uint32_t* vin, uint32_t* ka, uint32_t* va, uint32_t* kb, uint32_t* vb,
                      int64_t n, int key_bits, uint32_t* hist, uint32_t* scan_temp, hipStream_t st,
                      const ProfileHook& prof);

@@ -1,8 +1,8 @@
 // sort.hip -- stable LSD radix sort of (key, value) uint32 pairs and a
 // device-wide exclusive scan, written for wave64 (gfx950).
 //
-// One pass over a digit (8 or 9 bits: 25-bit cell keys sort in 3 passes of
-// 9+8+8 bits instead of 4 of 8) is three launches:
+// One pass over a digit (6 to 9 bits, planned by radix_passes) is three
+// launches:
 //   hist    : per-tile digit histogram, stored digit-major (digit * tiles + tile)
 //   scan    : exclusive scan of that table -> global base of (digit, tile)
 //   scatter : per-tile stable ranking (64-key chunks ranked with 8 ballots,
@@ -72,8 +72,11 @@ __global__ void __launch_bounds__(kSortWG) k_radix_scatter(const uint32_t* __res
                                                            uint32_t mask, const uint32_t* __restrict__ hist_scan,
                                                            int64_t tiles) {
     constexpr int kRadix = 1 << BITS;
-    constexpr int DPT = kRadix / kSortWG;  // digits per thread (1 or 2)
-    static_assert(DPT >= 1 && DPT * kSortWG == kRadix, "radix must be a multiple of the workgroup size");
+    // digits per thread: 2 (9 bits), 1 (8 bits), or 1 for the first kRadix
+    // threads and none for the rest (narrower digits)
+    constexpr int DPT = kRadix >= kSortWG ? kRadix / kSortWG : 1;
+    static_assert(kRadix < kSortWG || DPT * kSortWG == kRadix, "radix must be a multiple of the workgroup size");
+    const bool owns = kRadix >= kSortWG || (int)threadIdx.x < kRadix;
     __shared__ uint16_t chunk_off[kSortChunks][kRadix];  // 16 KB (8 bits) or 32 KB (9 bits)
     __shared__ uint32_t digit_base[kRadix];
     __shared__ uint32_t global_base[kRadix];
@@ -89,7 +92,7 @@ __global__ void __launch_bounds__(kSortWG) k_radix_scatter(const uint32_t* __res
     const int tile_count = (int)((n - base) < kSortTile ? (n - base) : kSortTile);
 
     uint32_t* co32 = reinterpret_cast<uint32_t*>(&chunk_off[0][0]);
-    for (int i = tid; i < kSortChunks * kRadix / 2; i += kSortWG) co32[i] = 0u;
+    for (int i = tid; i < kSortChunks * kRadix / 2; i += kSortWG) co32[i] = 0u;  // kRadix >= 2
     __syncthreads();
 
     uint32_t my_key[kChunksPerWave], my_val[kChunksPerWave], my_digit[kChunksPerWave], my_rank[kChunksPerWave];
@@ -119,6 +122,8 @@ __global__ void __launch_bounds__(kSortWG) k_radix_scatter(const uint32_t* __res
     uint32_t total = 0;
 #pragma unroll
     for (int j = 0; j < DPT; ++j) {
+        dtot[j] = 0;
+        if (!owns) continue;
         const int d = tid * DPT + j;
         uint32_t acc = 0;
         for (int c = 0; c < kSortChunks; ++c) {
@@ -141,7 +146,7 @@ __global__ void __launch_bounds__(kSortWG) k_radix_scatter(const uint32_t* __res
     __syncthreads();
     uint32_t wave_prefix = 0;
     for (int w = 0; w < wave; ++w) wave_prefix += wave_sums[w];
-    {
+    if (owns) {
         uint32_t run = wave_prefix + incl - total;
 #pragma unroll
         for (int j = 0; j < DPT; ++j) {
@@ -268,14 +273,19 @@ void exclusive_scan(uint32_t* data, int64_t n, uint32_t* temp, uint32_t* total, 
 int64_t radix_tiles(int64_t n) { return (n + kSortTile - 1) / kSortTile; }
 int64_t radix_hist_words(int64_t n) { return radix_tiles(n) * kMaxRadix; }
 
-// Digit widths: 8-bit digits unless 9-bit digits need fewer passes; the
-// bits are split as evenly as possible, widest digits first.
-int radix_passes(int key_bits) {
+// Digit widths.  Large sorts (>= 2^20 keys) use digits of at most 7 bits:
+// the scatter of a 2048-key tile then writes runs of >= 16 keys per digit,
+// and on the 8.4M-particle slab (25-bit keys) 4 passes of 7/6/6/6 bits take
+// 0.31 ms against 0.37 ms for 3 passes of 9/8/8 and 0.35 ms for 5 passes of
+// 5-6 bits.  Small sorts are launch-bound and use as few passes as 8- or
+// 9-bit digits allow.  Bits are split as evenly as possible, widest first.
+int radix_passes(int key_bits, int64_t n) {
+    if (n >= (int64_t(1) << 20)) return (key_bits + 6) / 7;
     const int p8 = (key_bits + 7) / 8, p9 = (key_bits + 8) / 9;
     return p9 < p8 ? p9 : p8;
 }
-static int digit_bits(int key_bits, int pass) {
-    const int passes = radix_passes(key_bits);
+static int digit_bits(int key_bits, int64_t n, int pass) {
+    const int passes = radix_passes(key_bits, n);
     int done = 0;
     for (int p = 0; p < pass; ++p) done += (key_bits - done + (passes - p) - 1) / (passes - p);
     return (key_bits - done + (passes - pass) - 1) / (passes - pass);
@@ -288,32 +298,36 @@ int radix_sort_pairs(const uint32_t* kin, const uint32_t* vin, uint32_t* ka, uin
                      const ProfileHook& prof) {
     if (n <= 0) return 0;
     int64_t tiles = radix_tiles(n);
-    int passes = radix_passes(key_bits);
+    int passes = radix_passes(key_bits, n);
     const uint32_t *ki = kin, *vi = vin;
     uint32_t *ko = ka, *vo = va;
     int shift = 0;
     for (int p = 0; p < passes; ++p) {
-        const int bits = digit_bits(key_bits, p);
+        const int bits = digit_bits(key_bits, n, p);
         const uint32_t mask = (1u << bits) - 1u;
-        const int radix = bits > 8 ? 512 : 256;
+        // digit widths of the plan: 9 or 8 bits, or 6-7 under a narrower plan
+        const int dw = bits > 8 ? 9 : (bits > 7 ? 8 : (bits > 6 ? 7 : 6));
+        const int radix = 1 << dw;
         prof.begin("sort_hist");
-        if (bits > 8)
-            hipLaunchKernelGGL(k_radix_hist<9>, dim3((unsigned)tiles), dim3(kSortWG), 0, st, ki, n, shift, mask, hist,
-                               tiles);
-        else
-            hipLaunchKernelGGL(k_radix_hist<8>, dim3((unsigned)tiles), dim3(kSortWG), 0, st, ki, n, shift, mask, hist,
-                               tiles);
+#define MPM_HIST(B) hipLaunchKernelGGL(k_radix_hist<B>, dim3((unsigned)tiles), dim3(kSortWG), 0, st, ki, n, shift, \
+                                       mask, hist, tiles)
+        if (dw == 9) MPM_HIST(9);
+        else if (dw == 8) MPM_HIST(8);
+        else if (dw == 7) MPM_HIST(7);
+        else MPM_HIST(6);
+#undef MPM_HIST
         prof.end("sort_hist");
         prof.begin("sort_scan");
         exclusive_scan(hist, tiles * radix, scan_temp, nullptr, st);
         prof.end("sort_scan");
         prof.begin("sort_scatter");
-        if (bits > 8)
-            hipLaunchKernelGGL(k_radix_scatter<9>, dim3((unsigned)tiles), dim3(kSortWG), 0, st, ki, vi, ko, vo, n,
-                               shift, mask, hist, tiles);
-        else
-            hipLaunchKernelGGL(k_radix_scatter<8>, dim3((unsigned)tiles), dim3(kSortWG), 0, st, ki, vi, ko, vo, n,
-                               shift, mask, hist, tiles);
+#define MPM_SCATTER(B) hipLaunchKernelGGL(k_radix_scatter<B>, dim3((unsigned)tiles), dim3(kSortWG), 0, st, ki, vi, \
+                                          ko, vo, n, shift, mask, hist, tiles)
+        if (dw == 9) MPM_SCATTER(9);
+        else if (dw == 8) MPM_SCATTER(8);
+        else if (dw == 7) MPM_SCATTER(7);
+        else MPM_SCATTER(6);
+#undef MPM_SCATTER
         prof.end("sort_scatter");
         shift += bits;
         ki = ko;

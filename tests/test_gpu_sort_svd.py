@@ -56,14 +56,14 @@ def test_sort_escapees_and_boundaries():
     assert np.isfinite(st.position[: lat.shape[0]]).all()
 
 
-@pytest.mark.parametrize("res", [128, 256])
-def test_sort_large_random(res):
-    # 128^3: 22-bit keys, three 8/7/7-bit passes; 256^3: 25-bit keys,
-    # three 9/8/8-bit passes (512-bin histograms)
-    n = 300_001
+@pytest.mark.parametrize("res,n", [(128, 300_001), (256, 300_001), (128, 1_100_003), (256, 1_100_003)])
+def test_sort_large_random(res, n):
+    # below 2^20 keys: 128^3 has 22-bit keys, three 8/7/7-bit passes; 256^3
+    # 25-bit keys, three 9/8/8-bit passes (512-bin histograms).  From 2^20
+    # keys digits are at most 7 bits: 6/6/5/5 and 7/6/6/6.
     sc = scenes.snow_cube(seed=1, n=n, res=res, lo=0.0, hi=1.0)
     gpu = Solver(sc.desc, sc.particles)
-    _check_sort(gpu, sc.desc, f"sort 300k random res={res}")
+    _check_sort(gpu, sc.desc, f"sort {n} random res={res}")
 
 
 def _svd_invariants(A, U, s, V, tol):
