@@ -11,8 +11,10 @@ Prints ONE JSON line.  ``value`` is particle-updates/s over the timed steps.
 ``roofline`` is for the dominant kernel (largest total time), with
 algorithmic bytes per launch (reads/writes of particle state and active grid
 nodes, per the byte model below) divided by its mean HIP-event duration.
-``cpu_baseline`` is the serial C++ oracle (a port of the reference step) on a
-bounded sample of the same workload.
+``cpu_baseline`` is the C++ oracle (a port of the reference step) on the
+host: on all host cores (OpenMP loops, atomic grid sums, as the reference's
+atomicAdd scatter) over the whole workload for two steps, with the serial
+oracle on a sub-box of it beside that (``serial``).
 
 Algorithmic bytes (FP32): APIC per particle-update 364 B = P2G 148 B
 (x, v, B, Fe, Fp, m, V0, mu0, lambda0) + G2P 216 B (reads x, Fe, Fp; writes
@@ -46,16 +48,53 @@ def log(msg):
     print(f"[bench] {msg}", file=sys.stderr, flush=True)
 
 
+def _cpu_model():
+    try:
+        with open("/proc/cpuinfo") as f:
+            for line in f:
+                if line.startswith("model name"):
+                    return line.split(":", 1)[1].strip()
+    except OSError:
+        pass
+    return "unknown"
+
+
 def cpu_baseline(scene, seconds_target=15.0):
+    """All-cores oracle over the whole workload (2 timed steps after 1
+    warm-up), and the serial oracle on a spatial sub-box of it (full grid)."""
+    from oracle import pyoracle
+
+    threads = int(os.environ.get("OMP_NUM_THREADS") or os.cpu_count() or 1)
+    o = pyoracle.OracleSolver(scene.desc, scene.particles, 32, threads=threads)
+    o.time_steps(1)  # warm-up step
+    t_all = o.time_steps(2)
+    o.close()
+    serial = cpu_baseline_serial(scene, seconds_target)
+    return {
+        "value": scene.particles.count * 2 / t_all,
+        "unit": "particle-updates/s",
+        "cores": threads,
+        "kind": "port",
+        "cpu": _cpu_model(),
+        "sample": f"C++ oracle (FP32 restatement of the reference step) on {threads} threads, OpenMP over "
+                  f"particles and nodes with atomic grid sums, the whole {scene.particles.count}-particle "
+                  f"workload, 2 steps after 1 warm-up, {t_all:.1f} s",
+        "serial": serial,
+    }
+
+
+def cpu_baseline_serial(scene, seconds_target=15.0):
     """Serial oracle on a spatial sub-box of the same workload (full grid)."""
-    from mpm_amd.solver import ParticleInput
     from oracle import pyoracle
 
     p = scene.particles
     x = p.position
-    # take x,z in the low corner of the slab: ~1/16 of the particles
+    # x,z in the low corner of the domain: ~1/16 of the slab's particles
+    # (scenes with few particles there run whole)
     sel = np.nonzero((x[:, 0] < 0.3) & (x[:, 2] < 0.3))[0]
-    sub = p.subset(sel)
+    whole = sel.size < min(p.count, 100_000)
+    sub = p if whole else p.subset(sel)
+    where = "the whole workload" if whole else "sub-box x,z<0.3"
     o = pyoracle.OracleSolver(scene.desc, sub, 32)
     o.time_steps(1)  # warm-up step
     steps = 0
@@ -70,7 +109,8 @@ def cpu_baseline(scene, seconds_target=15.0):
         "cores": 1,
         "kind": "port",
         "sample": f"serial C++ oracle (FP32 restatement of the reference step), {sub.count} particles "
-                  f"(slab sub-box x,z<0.3) on the full 256^3 grid, {steps} steps after 1 warm-up, {t:.1f} s",
+                  f"({where}) on the full {'x'.join(str(r) for r in scene.desc.resolution)} grid, {steps} steps after "
+                  f"1 warm-up, {t:.1f} s",
     }
 
 
@@ -409,9 +449,9 @@ def main(argv=None):
 
     cpu = None
     if not a.no_cpu_baseline:
-        log("cpu baseline (serial oracle on a bounded sample) ...")
+        log("cpu baseline (oracle on all host cores, then serial on a sub-box) ...")
         cpu = cpu_baseline(sc)
-        log(f"cpu baseline {cpu['value']:.3e} p-u/s")
+        log(f"cpu baseline {cpu['value']:.3e} p-u/s on {cpu['cores']} threads, serial {cpu['serial']['value']:.3e}")
 
     out = {
         "metric": METRIC,

@@ -24,6 +24,11 @@
 //   density raster  closed forms (one particle on a node), lattice density
 //                   ppc m / h^3, sum density h^3 = sum m, an independent
 //                   numpy raster
+// Threads: with Solver::threads > 1 (the all-cores CPU baseline only) the
+// particle and node loops run under OpenMP and the grid sums become atomic
+// adds, mirroring the reference's atomicAdd scatter; the order of those sums
+// then varies between runs, so every parity check uses threads = 1, where
+// each loop runs in array order as stated above.
 // The third-party SVD the reference calls (3x3_SVD_CUDA, unpinned version,
 // submodule absent) and Eigen's evaluation order are not reproducible
 // bit-for-bit; parity with them is tolerance-level only.
@@ -395,6 +400,17 @@ V3<R> apply_boundary_collision(const V3<R>& pos, const V3<R>& vel, const V3<R>& 
     return updated;
 }
 
+// Grid sum: an atomic add when particle loops run on several threads.
+template <class R>
+inline void grid_add(R& dst, R v, bool atomic) {
+    if (atomic) {
+#pragma omp atomic
+        dst += v;
+    } else {
+        dst += v;
+    }
+}
+
 // -------------------------------------------------------------------- grid
 template <class R>
 struct Node {  // CollocatedGridData3D, include/grid3d.h:27-47
@@ -416,6 +432,7 @@ struct Solver {
     std::vector<Particle<R>> P;
     std::vector<Node<R>> G;
     int64_t steps = 0;
+    int threads = 1;  // > 1: OpenMP loops, atomic grid sums (timing only)
 
     Solver(const mpm_desc& d, std::vector<Particle<R>> parts)
         : desc(d), apic(d.transfer == MPM_TRANSFER_APIC), interp(d.interp), P(std::move(parts)) {
@@ -459,7 +476,11 @@ struct Solver {
     // initialize(): P2GMass + G2PVolume, src/mpm_solver3d.cu:131-223;
     // APIC D/Dinv, src/apic_mpm_solver3d.cu:151-157.
     void initialize() {
-        for (auto& p : P) {
+        const int64_t np = int64_t(P.size());
+        const bool mt = threads > 1;
+#pragma omp parallel for schedule(static) if (mt) num_threads(threads)
+        for (int64_t ip = 0; ip < np; ++ip) {
+            Particle<R>& p = P[ip];
             int lo[3], hi[3];
             range(p.position, lo, hi);
             if (apic) {
@@ -474,10 +495,12 @@ struct Solver {
                     for (int z = lo[2]; z <= hi[2]; ++z) {
                         if (!inside(x, y, z)) continue;
                         R w = interp.weight3D(p.position, node_pos(x, y, z), h);
-                        G[node_id(x, y, z)].mass += p.mass * w;
+                        grid_add(G[node_id(x, y, z)].mass, p.mass * w, mt);
                     }
         }
-        for (auto& p : P) {
+#pragma omp parallel for schedule(static) if (mt) num_threads(threads)
+        for (int64_t ip = 0; ip < np; ++ip) {
+            Particle<R>& p = P[ip];
             int lo[3], hi[3];
             range(p.position, lo, hi);
             R density = 0;
@@ -495,7 +518,10 @@ struct Solver {
     }
 
     void reset_grid() {  // src/mpm_solver3d.cu:225-235, src/grid3d.cu:28-33
-        for (auto& n : G) {
+        const int64_t nn = int64_t(G.size());
+#pragma omp parallel for schedule(static) if (threads > 1) num_threads(threads)
+        for (int64_t i = 0; i < nn; ++i) {
+            Node<R>& n = G[i];
             n.mass = 0;
             n.force = vzero<R>();
             n.velocity = vzero<R>();
@@ -506,7 +532,11 @@ struct Solver {
     // particlesToGrid: PIC-FLIP src/mpm_solver3d.cu:237-294,
     // APIC src/apic_mpm_solver3d.cu:245-304.
     void particles_to_grid() {
-        for (auto& p : P) {
+        const int64_t np = int64_t(P.size());
+        const bool mt = threads > 1;
+#pragma omp parallel for schedule(static) if (mt) num_threads(threads)
+        for (int64_t ip = 0; ip < np; ++ip) {
+            Particle<R>& p = P[ip];  // the Neo-Hookean stress writes Fe back (src/material_point3d.cu:142)
             int lo[3], hi[3];
             range(p.position, lo, hi);
             M3<R> C = apic ? mm(p.B, p.Dinv) : mzero<R>();
@@ -521,28 +551,35 @@ struct Solver {
                         R m_ip = p.mass * w;
                         V3<R> f_ip = mv(vol_stress, gw);
                         Node<R>& n = G[node_id(x, y, z)];
-                        n.mass += m_ip;
+                        grid_add(n.mass, m_ip, mt);
                         if (apic) {
                             V3<R> v_ip = scale(m_ip, add(p.velocity, mv(C, sub(gp, p.position))));
-                            for (int i = 0; i < 3; ++i) n.velocity[i] += v_ip[i];
+                            for (int i = 0; i < 3; ++i) grid_add(n.velocity[i], v_ip[i], mt);
                         } else {
-                            for (int i = 0; i < 3; ++i) n.velocity[i] += p.velocity[i] * m_ip;
+                            for (int i = 0; i < 3; ++i) grid_add(n.velocity[i], p.velocity[i] * m_ip, mt);
                         }
-                        for (int i = 0; i < 3; ++i) n.force[i] += f_ip[i];
+                        for (int i = 0; i < 3; ++i) grid_add(n.force[i], f_ip[i], mt);
                     }
         }
     }
 
     // computeGravityForces, src/mpm_solver3d.cu:394-403 (double product).
     void compute_gravity_forces() {
-        for (auto& n : G)
+        const int64_t nn = int64_t(G.size());
+#pragma omp parallel for schedule(static) if (threads > 1) num_threads(threads)
+        for (int64_t i = 0; i < nn; ++i) {
+            Node<R>& n = G[i];
             if (n.mass > R(kFltEps)) n.force[1] = R(double(n.force[1]) - double(gravity) * double(n.mass));
+        }
     }
 
     // updateGridVelocities -> CollocatedGridData3D::updateVelocity,
     // src/mpm_solver3d.cu:300-310, src/grid3d.cu:36-41.
     void update_grid_velocities(R dt_) {
-        for (auto& n : G) {
+        const int64_t nn = int64_t(G.size());
+#pragma omp parallel for schedule(static) if (threads > 1) num_threads(threads)
+        for (int64_t i = 0; i < nn; ++i) {
+            Node<R>& n = G[i];
             if (n.mass > R(kFltEps)) {
                 for (int i = 0; i < 3; ++i) n.velocity[i] = n.velocity[i] / n.mass;
                 for (int i = 0; i < 3; ++i) n.velocity_star[i] = n.velocity[i] + (dt_ * n.force[i]) / n.mass;
@@ -552,7 +589,10 @@ struct Solver {
 
     // gridCollision, src/mpm_solver3d.cu:405-426.
     void grid_collision(R dt_) {
-        for (auto& n : G) {
+        const int64_t nn = int64_t(G.size());
+#pragma omp parallel for schedule(static) if (threads > 1) num_threads(threads)
+        for (int64_t i = 0; i < nn; ++i) {
+            Node<R>& n = G[i];
             V3<R> pos = add(node_pos(n.index[0], n.index[1], n.index[2]), scale(dt_, n.velocity_star));
             n.velocity_star = apply_boundary_collision(pos, n.velocity_star, origin, target, mu_b);
         }
@@ -561,7 +601,10 @@ struct Solver {
     // gridToParticles: PIC-FLIP src/mpm_solver3d.cu:316-379,
     // APIC src/apic_mpm_solver3d.cu:326-388.
     void grid_to_particles(R dt_, R blend_) {
-        for (auto& p : P) {
+        const int64_t np = int64_t(P.size());
+#pragma omp parallel for schedule(static) if (threads > 1) num_threads(threads)
+        for (int64_t ip = 0; ip < np; ++ip) {
+            Particle<R>& p = P[ip];
             int lo[3], hi[3];
             range(p.position, lo, hi);
             M3<R> grad = mzero<R>();
@@ -595,12 +638,18 @@ struct Solver {
     }
 
     void particles_collision(R dt_) {  // src/mpm_solver3d.cu:428-445
-        for (auto& p : P)
+        const int64_t np = int64_t(P.size());
+#pragma omp parallel for schedule(static) if (threads > 1) num_threads(threads)
+        for (int64_t ip = 0; ip < np; ++ip) {
+            Particle<R>& p = P[ip];
             p.velocity = apply_boundary_collision(add(p.position, scale(dt_, p.velocity)), p.velocity, origin, target, mu_b);
+        }
     }
 
     void advect_particles(R dt_) {  // src/mpm_solver3d.cu:381-391, src/material_point3d.cu:48-50
-        for (auto& p : P) p.position = add(p.position, scale(dt_, p.velocity));
+        const int64_t np = int64_t(P.size());
+#pragma omp parallel for schedule(static) if (threads > 1) num_threads(threads)
+        for (int64_t ip = 0; ip < np; ++ip) P[ip].position = add(P[ip].position, scale(dt_, P[ip].velocity));
     }
 
     // simulateOneStep, src/mpm_solver3d.cu:100-125 / src/apic_mpm_solver3d.cu:99-124.

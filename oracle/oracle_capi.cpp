@@ -150,20 +150,29 @@ int dispatch(void* h, F&& f) {
 
 extern "C" {
 
-__attribute__((visibility("default"))) void* orc_create(const mpm_desc* d, const mpm_particle_input* in,
-                                                        int precision) {
-    if (!check_desc(d) || !in) return nullptr;
+// threads > 1 runs initialize() and every later step under OpenMP with atomic
+// grid sums (the all-cores CPU baseline; sums in run-dependent order).
+__attribute__((visibility("default"))) void* orc_create_threads(const mpm_desc* d, const mpm_particle_input* in,
+                                                                int precision, int threads) {
+    if (!check_desc(d) || !in || threads < 1) return nullptr;
     Handle* H = new (std::nothrow) Handle();
     if (!H) return nullptr;
     H->precision = precision;
     if (precision == 64) {
         H->s64.reset(new Solver<double>(*d, from_input<double>(in)));
+        H->s64->threads = threads;
         H->s64->initialize();
     } else {
         H->s32.reset(new Solver<float>(*d, from_input<float>(in)));
+        H->s32->threads = threads;
         H->s32->initialize();
     }
     return H;
+}
+
+__attribute__((visibility("default"))) void* orc_create(const mpm_desc* d, const mpm_particle_input* in,
+                                                        int precision) {
+    return orc_create_threads(d, in, precision, 1);
 }
 
 __attribute__((visibility("default"))) void* orc_create_state(const mpm_desc* d, const mpm_particle_state* st,

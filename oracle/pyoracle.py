@@ -34,6 +34,8 @@ def load():
     lib = C.CDLL(LIB)
     lib.orc_create.restype = C.c_void_p
     lib.orc_create.argtypes = [C.POINTER(N.MpmDesc), C.POINTER(N.MpmParticleInput), C.c_int]
+    lib.orc_create_threads.restype = C.c_void_p
+    lib.orc_create_threads.argtypes = [C.POINTER(N.MpmDesc), C.POINTER(N.MpmParticleInput), C.c_int, C.c_int]
     lib.orc_create_state.restype = C.c_void_p
     lib.orc_create_state.argtypes = [C.POINTER(N.MpmDesc), C.POINTER(N.MpmParticleState), C.c_int, C.c_int]
     lib.orc_destroy.argtypes = [C.c_void_p]
@@ -68,19 +70,25 @@ SUBSTEPS = {"reset": 0, "p2g": 1, "gravity": 2, "update": 3, "collision": 4, "g2
 
 
 class OracleSolver:
-    """Serial CPU restatement of the reference step (precision 32 or 64)."""
+    """Serial CPU restatement of the reference step (precision 32 or 64).
 
-    def __init__(self, desc: SolverDesc, particles, precision: int = 32, run_initialize: bool = True):
+    threads > 1 (inputs only; the all-cores timing) runs the loops under
+    OpenMP with atomic grid sums, whose order varies from run to run."""
+
+    def __init__(self, desc: SolverDesc, particles, precision: int = 32, run_initialize: bool = True,
+                 threads: int = 1):
         self._lib = load()
         self.desc = desc
         self._cdesc = desc.to_c()
+        if threads != 1 and isinstance(particles, ParticleState):
+            raise ValueError("threaded oracle takes solver inputs only")
         if isinstance(particles, ParticleState):
             st = particles.copy()
             c = st.to_c()
             h = self._lib.orc_create_state(C.byref(self._cdesc), C.byref(c), 1 if run_initialize else 0, precision)
         else:
             c = particles.to_c()
-            h = self._lib.orc_create(C.byref(self._cdesc), C.byref(c), precision)
+            h = self._lib.orc_create_threads(C.byref(self._cdesc), C.byref(c), precision, int(threads))
         if not h:
             raise ValueError("oracle rejected the scene")
         self._h = C.c_void_p(h)

@@ -371,3 +371,25 @@ def test_fixed_stencil_matches_reference_range(interp, r, shift):
     d = (np.asarray(xs, np.float64) - np.float64(h) * np.asarray(nodes)) / np.float64(h)
     n, dn = O.weights(interp, d)
     assert np.abs(n).max() < 1e-9 and np.abs(dn).max() < 1e-4, (np.abs(n).max(), np.abs(dn).max())
+
+
+@pytest.mark.parametrize("transfer", [N.PIC_FLIP, N.APIC])
+def test_threaded_oracle_matches_serial(transfer):
+    """The all-cores timing mode (OpenMP loops, atomic grid sums) computes the
+    same step as the serial oracle up to the order of the grid sums; with one
+    thread it is the serial oracle bit for bit."""
+    from mpm_amd import scenes
+    from tests.helpers import particle_errors
+    sc = scenes.snow_cube(seed=3, n=6000, res=32, transfer=transfer, velocity=(1.0, -2.0, 0.5), dt=2e-4)
+    serial = O.OracleSolver(sc.desc, sc.particles, 32)
+    one = O.OracleSolver(sc.desc, sc.particles, 32, threads=1)
+    many = O.OracleSolver(sc.desc, sc.particles, 32, threads=4)
+    for o in (serial, one, many):
+        o.step(5)
+    a, b, c = serial.read_particles(), one.read_particles(), many.read_particles()
+    for f in ("position", "velocity", "Fe", "Fp", "B"):
+        assert np.array_equal(getattr(a, f), getattr(b, f)), f
+    e = particle_errors(c, a)
+    assert all(v <= 1e-5 for v in e.values()), e
+    with pytest.raises(ValueError):
+        O.OracleSolver(sc.desc, serial.read_particles(), 32, threads=2)
