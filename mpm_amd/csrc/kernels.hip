@@ -638,6 +638,61 @@ void launch_p2g(const SceneDev& S, const ParticlesDev& P, const ItemsDev& I, int
 }
 
 // -------------------------------------------------------------------- G2P
+// Folded-grid tile of one item, in two halves.  fetch_folded_tile issues
+// the lane's tile-node loads (mass and p + dt f) as global_load_lds_dword:
+// the data goes straight from HBM into the wave's staging area in LDS (word
+// (r * 4 + f) * 64 + lane of its region) without passing through VGPRs.
+// commit_folded_tile waits for them (vmcnt(0): the compiler does not track
+// these LDS writes) and turns them into v* (the fused grid update) in the
+// tile.  The G2P loop fetches the next item's tile before the current item's
+// particles, so the loads are in flight during that work.  Nodes outside the
+// grid issue no load; commit writes 0 for them without reading the stage.
+typedef __attribute__((address_space(3))) void lds_void;
+template <int T, int TR, int WG>
+__device__ __forceinline__ void fetch_folded_tile(const SceneDev& S, const ItemGeom& g, float* stage) {
+    constexpr int TN = T * T * T;
+    float* wst = stage + (threadIdx.x >> 6) * (TR * 4 * 64);
+#pragma unroll 1  // unrolled, the address arithmetic of all rounds stays live (spills at 64 threads)
+    for (int r = 0; r < TR; ++r) {
+        const int q = (int)threadIdx.x + r * WG;
+        const int lx = q % T, ly = (q / T) % T, lz = q / (T * T);
+        const int gi = g.ox + lx, gj = g.oy + ly, gk = g.oz + lz;
+        if (q < TN && !g.escape && in_grid(S, gi, gj, gk)) {
+            const int64_t ni = node_index(S, gi, gj, gk);
+            const int64_t pl = S.plane;
+            __builtin_amdgcn_global_load_lds((const void*)(S.grid + ni), (lds_void*)(wst + (r * 4 + 0) * 64), 4, 0, 0);
+            __builtin_amdgcn_global_load_lds((const void*)(S.grid + (GF_VEL + 0) * pl + ni),
+                                             (lds_void*)(wst + (r * 4 + 1) * 64), 4, 0, 0);
+            __builtin_amdgcn_global_load_lds((const void*)(S.grid + (GF_VEL + 1) * pl + ni),
+                                             (lds_void*)(wst + (r * 4 + 2) * 64), 4, 0, 0);
+            __builtin_amdgcn_global_load_lds((const void*)(S.grid + (GF_VEL + 2) * pl + ni),
+                                             (lds_void*)(wst + (r * 4 + 3) * 64), 4, 0, 0);
+        }
+    }
+}
+template <int T, int TR, int WG>
+__device__ __forceinline__ void commit_folded_tile(const SceneDev& S, const ItemGeom& g, const float* stage,
+                                                   float* tile) {
+    constexpr int TN = T * T * T;
+    asm volatile("s_waitcnt vmcnt(0)" ::: "memory");
+    const float* wst = stage + (threadIdx.x >> 6) * (TR * 4 * 64) + (threadIdx.x & 63);
+#pragma unroll 1  // as in fetch_folded_tile (the cubic stencil spills at 64 threads otherwise)
+    for (int r = 0; r < TR; ++r) {
+        const int q = (int)threadIdx.x + r * WG;
+        if (q >= TN) break;
+        const int lx = q % T, ly = (q / T) % T, lz = q / (T * T);
+        const int gi = g.ox + lx, gj = g.oy + ly, gk = g.oz + lz;
+        float v0 = 0.0f, v1 = 0.0f, v2 = 0.0f;
+        if (in_grid(S, gi, gj, gk))
+            grid_node_update_folded(S, gi, gj, gk, wst[(r * 4 + 0) * 64], wst[(r * 4 + 1) * 64],
+                                    wst[(r * 4 + 2) * 64], wst[(r * 4 + 3) * 64], v0, v1, v2);
+        tile[q] = v0;
+        tile[TN + q] = v1;
+        tile[2 * TN + q] = v2;
+    }
+}
+
+// -------------------------------------------------------------------- G2P
 // gridToParticles: src/mpm_solver3d.cu:316-379 (PIC-FLIP),
 // src/apic_mpm_solver3d.cu:326-388 (APIC), with the optional
 // particlesCollision (:428-445) and advectParticles (:381-391) fused in.
@@ -650,16 +705,29 @@ __global__ void __launch_bounds__(WG) __attribute__((amdgpu_waves_per_eu(G2PWave
     constexpr int TN = T * T * T;
     constexpr int NF = SRC == 2 ? 1 : (APIC ? 3 : 6);
     static_assert(SRC != 3 || APIC, "the folded grid serves APIC only");
+    // one scene, folded grid: the next item's tile loads overlap this item
+    constexpr bool PREFETCH = SRC == 3 && !BATCH;
+    constexpr int TR = PREFETCH ? (TN + WG - 1) / WG : 1;
     __shared__ float tile[NF * TN];
+    __shared__ float stage[PREFETCH ? TR * 4 * WG : 1];
     const uint32_t nitems = *I.count;
+    ItemGeom gnext{};
+    if constexpr (PREFETCH) {
+        if (blockIdx.x < nitems) {
+            gnext = item_geom(S0, I, blockIdx.x, nitems, 0);
+            fetch_folded_tile<T, TR, WG>(S0, gnext, stage);
+        }
+    }
     for (uint32_t it = blockIdx.x; it < nitems; it += gridDim.x) {
         SceneDev S;
         ParticlesDev P;
         uint32_t pbase, koff;
         item_scene<BATCH>(S0, P0, I, tab, it, S, P, pbase, koff);
         const int64_t N = P.n;
-        ItemGeom g = item_geom(S, I, it, nitems, koff);
-        if (!g.escape) {
+        ItemGeom g = PREFETCH ? gnext : item_geom(S, I, it, nitems, koff);
+        if (PREFETCH) {
+            if (!g.escape) commit_folded_tile<T, TR, WG>(S, g, stage, tile);
+        } else if (!g.escape) {
             for (int q = threadIdx.x; q < TN; q += WG) {
                 const int lx = q % T, ly = (q / T) % T, lz = q / (T * T);
                 const int gi = g.ox + lx, gj = g.oy + ly, gk = g.oz + lz;
@@ -700,6 +768,13 @@ __global__ void __launch_bounds__(WG) __attribute__((amdgpu_waves_per_eu(G2PWave
             }
         }
         __syncthreads();
+        if constexpr (PREFETCH) {
+            const uint32_t itn = it + gridDim.x;
+            if (itn < nitems) {
+                gnext = item_geom(S, I, itn, nitems, koff);
+                fetch_folded_tile<T, TR, WG>(S, gnext, stage);
+            }
+        }
         for (uint32_t i = g.s + threadIdx.x; i < g.e; i += WG) {
             const uint32_t p = I.perm[i] - pbase;
             const float x0 = P.x[p], x1 = P.x[N + p], x2 = P.x[2 * N + p];
