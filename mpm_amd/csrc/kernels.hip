@@ -708,8 +708,14 @@ __global__ void __launch_bounds__(WG) __attribute__((amdgpu_waves_per_eu(G2PWave
     // one scene, folded grid: the next item's tile loads overlap this item
     constexpr bool PREFETCH = SRC == 3 && !BATCH;
     constexpr int TR = PREFETCH ? (TN + WG - 1) / WG : 1;
+    // and each particle's Fe, Fp, mu0, lam0, V0 go straight into LDS when
+    // the particle is picked, to be read after the gather (not for cubic:
+    // its larger tile would cut the workgroups per CU)
+    constexpr bool PSTAGE = PREFETCH && INTERP != MPM_INTERP_CUBIC_BSPLINE;
+    constexpr int PW = 21;  // staged words per particle
     __shared__ float tile[NF * TN];
     __shared__ float stage[PREFETCH ? TR * 4 * WG : 1];
+    __shared__ float pstage[PSTAGE ? PW * WG : 1];
     const uint32_t nitems = *I.count;
     ItemGeom gnext{};
     if constexpr (PREFETCH) {
@@ -777,6 +783,19 @@ __global__ void __launch_bounds__(WG) __attribute__((amdgpu_waves_per_eu(G2PWave
         }
         for (uint32_t i = g.s + threadIdx.x; i < g.e; i += WG) {
             const uint32_t p = I.perm[i] - pbase;
+            float* const pst = pstage + (threadIdx.x >> 6) * (PW * 64);  // the wave's region
+            if constexpr (PSTAGE) {
+#pragma unroll 1
+                for (int f = 0; f < 9; ++f) {
+                    __builtin_amdgcn_global_load_lds((const void*)(P.Fe + (int64_t)f * N + p), (lds_void*)(pst + f * 64),
+                                                     4, 0, 0);
+                    __builtin_amdgcn_global_load_lds((const void*)(P.Fp + (int64_t)f * N + p),
+                                                     (lds_void*)(pst + (9 + f) * 64), 4, 0, 0);
+                }
+                __builtin_amdgcn_global_load_lds((const void*)(P.mu0 + p), (lds_void*)(pst + 18 * 64), 4, 0, 0);
+                __builtin_amdgcn_global_load_lds((const void*)(P.lam0 + p), (lds_void*)(pst + 19 * 64), 4, 0, 0);
+                __builtin_amdgcn_global_load_lds((const void*)(P.V0 + p), (lds_void*)(pst + 20 * 64), 4, 0, 0);
+            }
             const float x0 = P.x[p], x1 = P.x[N + p], x2 = P.x[2 * N + p];
             int bx = 0, by = 0, bz = 0;
             float Nx[W], Ny[W], Nz[W], dNx[W], dNy[W], dNz[W];
@@ -900,6 +919,21 @@ __global__ void __launch_bounds__(WG) __attribute__((amdgpu_waves_per_eu(G2PWave
                 n1 = S.blend * vf1 + ob * vpic1;
                 n2 = S.blend * vf2 + ob * vpic2;
             }
+            float Fe[3][3], Fp[3][3], mu0, lam0, V0;
+            if constexpr (PSTAGE) {
+                // staged words have landed (waited for before this
+                // particle's stores are issued, so the wait covers loads only)
+                asm volatile("s_waitcnt vmcnt(0)" ::: "memory");
+                const float* q = pst + (threadIdx.x & 63);
+#pragma unroll
+                for (int f = 0; f < 9; ++f) {
+                    Fe[f / 3][f % 3] = q[f * 64];
+                    Fp[f / 3][f % 3] = q[(9 + f) * 64];
+                }
+                mu0 = q[18 * 64];
+                lam0 = q[19 * 64];
+                V0 = q[20 * 64];
+            }
             // Velocity, B, position and next sort key first: they depend on
             // the gather only, and storing them before the F update frees
             // their registers for the SVD.
@@ -917,14 +951,18 @@ __global__ void __launch_bounds__(WG) __attribute__((amdgpu_waves_per_eu(G2PWave
                 if (next_keys)
                     next_keys[pbase + p] = koff + cell_key(S, y0, y1, y2, Stencil<INTERP>::SHIFT, Stencil<INTERP>::W);
             }
-            float Fe[3][3], Fp[3][3];
-            load_m3(P.Fe, N, p, Fe);
-            load_m3(P.Fp, N, p, Fp);
+            if constexpr (!PSTAGE) {
+                load_m3(P.Fe, N, p, Fe);
+                load_m3(P.Fp, N, p, Fp);
+                mu0 = P.mu0[p];
+                lam0 = P.lam0[p];
+                V0 = P.V0[p];
+            }
             const MaterialDev mat = P.mats[P.mat[p]];
             float U[3][3], sv[3], V[3][3], tau[6];
             update_deformation(Fe, Fp, grad, S.dt, mat, U, sv, V);
             // stress for the next P2G from the same decomposition
-            stress_from_svd(U, sv, V, Fp, P.mu0[p], P.lam0[p], P.V0[p], mat, tau, Fe);
+            stress_from_svd(U, sv, V, Fp, mu0, lam0, V0, mat, tau, Fe);
             store_m3(P.Fe, N, p, Fe);
             store_m3(P.Fp, N, p, Fp);
 #pragma unroll
