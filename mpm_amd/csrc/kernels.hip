@@ -705,8 +705,8 @@ __global__ void __launch_bounds__(WG) __attribute__((amdgpu_waves_per_eu(G2PWave
     constexpr int TN = T * T * T;
     constexpr int NF = SRC == 2 ? 1 : (APIC ? 3 : 6);
     static_assert(SRC != 3 || APIC, "the folded grid serves APIC only");
-    // one scene, folded grid: the next item's tile loads overlap this item
-    constexpr bool PREFETCH = SRC == 3 && !BATCH;
+    // folded grid: the next item's tile loads overlap this item
+    constexpr bool PREFETCH = SRC == 3;
     constexpr int TR = PREFETCH ? (TN + WG - 1) / WG : 1;
     // and each particle's Fe, Fp, mu0, lam0, V0 go straight into LDS when
     // the particle is picked, to be read after the gather (not for cubic:
@@ -720,8 +720,12 @@ __global__ void __launch_bounds__(WG) __attribute__((amdgpu_waves_per_eu(G2PWave
     ItemGeom gnext{};
     if constexpr (PREFETCH) {
         if (blockIdx.x < nitems) {
-            gnext = item_geom(S0, I, blockIdx.x, nitems, 0);
-            fetch_folded_tile<T, TR, WG>(S0, gnext, stage);
+            SceneDev Sn;
+            ParticlesDev Pn;
+            uint32_t pbn, kon;
+            item_scene<BATCH>(S0, P0, I, tab, blockIdx.x, Sn, Pn, pbn, kon);
+            gnext = item_geom(Sn, I, blockIdx.x, nitems, kon);
+            fetch_folded_tile<T, TR, WG>(Sn, gnext, stage);
         }
     }
     for (uint32_t it = blockIdx.x; it < nitems; it += gridDim.x) {
@@ -777,8 +781,12 @@ __global__ void __launch_bounds__(WG) __attribute__((amdgpu_waves_per_eu(G2PWave
         if constexpr (PREFETCH) {
             const uint32_t itn = it + gridDim.x;
             if (itn < nitems) {
-                gnext = item_geom(S, I, itn, nitems, koff);
-                fetch_folded_tile<T, TR, WG>(S, gnext, stage);
+                SceneDev Sn;  // the next item's scene (a batch's items span scenes)
+                ParticlesDev Pn;
+                uint32_t pbn, kon;
+                item_scene<BATCH>(S0, P0, I, tab, itn, Sn, Pn, pbn, kon);
+                gnext = item_geom(Sn, I, itn, nitems, kon);
+                fetch_folded_tile<T, TR, WG>(Sn, gnext, stage);
             }
         }
         for (uint32_t i = g.s + threadIdx.x; i < g.e; i += WG) {
