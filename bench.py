@@ -389,9 +389,11 @@ def main(argv=None):
     solver.synchronize()
     log(f"warmup {a.warmup} steps done")
 
-    # timed region: K steps; per-kernel HIP events on the solver stream
+    # timed region: K steps; HIP events on the solver stream around the item
+    # kernels (p2g, g2p) only: events around every launch group would add
+    # ~0.1 ms of stream gaps per step
     if not a.graph:
-        solver.profile(True)
+        solver.profile(2)
         solver.profile_reset()
     s0 = solver.stats()
     solver.synchronize()
@@ -414,11 +416,24 @@ def main(argv=None):
     step_bytes = n * (bp["p2g"] + bp["g2p"]) + act * (GRID_BYTES["p2g"] + GRID_BYTES["g2p"])
     step_frac = step_bytes / (elapsed / a.steps) / HBM_PEAK
 
-    roof = None
+    # per-kernel breakdown of the whole step: a separate pass with events
+    # around every launch group (p2g and g2p keep their timed-region figures)
     kernels = {}
     if prof:
+        solver.profile(1)
+        solver.profile_reset()
+        nb = min(50, a.steps)
+        for _ in range(nb):
+            run(1)
+        solver.synchronize()
+        for k, (launches, tot) in solver.profile_read().items():
+            kernels[k] = {"launches": launches, "mean_ms": tot / max(launches, 1), "pass": f"breakdown ({nb} steps)"}
+        solver.profile(False)
         for k, (launches, tot) in prof.items():
-            kernels[k] = {"launches": launches, "mean_ms": tot / max(launches, 1)}
+            kernels[k] = {"launches": launches, "mean_ms": tot / max(launches, 1), "pass": "timed"}
+
+    roof = None
+    if prof:
         dom = max(prof.items(), key=lambda kv: kv[1][1])[0]
         mean_s = kernels[dom]["mean_ms"] * 1e-3
         if dom in bp:

@@ -88,7 +88,7 @@ class Batch:
 
     def set_reorder_interval(self, steps: int):
         """Permute every member's store into cell order every `steps` batch
-        steps (0 = never; default 100)."""
+        steps (0 = never; default 16)."""
         N.check(self._lib.mpm_batch_set_reorder_interval(self._h, int(steps)))
 
     def info(self) -> dict:

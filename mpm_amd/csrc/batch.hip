@@ -82,7 +82,7 @@ struct Batch {
     bool fold = false;        // APIC with no member recording its grid: 4-plane P2G
     // every reorder_interval steps (0 = never) each member's store is
     // permuted into its range of the global cell order; scratch 9 * cap words
-    int reorder_interval = 100;
+    int reorder_interval = 16;
     int64_t last_reorder = 0;
     uint32_t* reorder_scratch = nullptr;
     int64_t reorder_cap = 0;
@@ -338,8 +338,9 @@ int reorder_members(Batch& b) {
                                      b.st, ProfileHook{});
     const uint32_t* sorted = sel ? b.kb : b.ka;
     const uint32_t* perm = sel ? b.vb : b.va;
+    // flags (ntot words, free between steps) holds each slot's scene
     launch_reorder_store(ParticlesDev{}, b.tab, (int)b.members.size(), perm, b.ntot, b.reorder_scratch, b.apic != 0,
-                         b.st);
+                         b.st, b.flags);
     HIP_TRY(hipMemcpyAsync(b.kin, sorted, (size_t)b.ntot * 4, hipMemcpyDeviceToDevice, b.st));
     for (Solver* m : b.members) {
         m->keys_valid = false;  // the member's own sort input is in the old slot order

@@ -61,6 +61,9 @@ hipEvent_t take_event(Solver& s) {
 }
 void prof_begin(void* ctx, const char* name) {
     Solver& s = *static_cast<Solver*>(ctx);
+    if (s.prof == 2 && std::strcmp(name, "p2g") != 0 && std::strcmp(name, "g2p") != 0 &&
+        std::strcmp(name, "reorder") != 0)
+        return;
     Pending p;
     p.name = name;
     p.a = take_event(s);
@@ -1208,7 +1211,8 @@ MPM_API int mpm_get_stats(void* handle, mpm_stats* out) {
 
 MPM_API int mpm_profile_enable(void* handle, int enable) {
     GET_SOLVER(s, handle);
-    s->prof = enable != 0;
+    if (enable < 0 || enable > 2) return fail(MPM_ERR_INVALID, "profile level must be 0, 1 or 2");
+    s->prof = enable;
     return MPM_OK;
 }
 

@@ -1382,10 +1382,19 @@ __device__ __forceinline__ int slot_scene(const SceneEntry* __restrict__ tab, in
     return lo;
 }
 
+// Scene of every global slot of a batch (one binary search per slot, once
+// per reorder instead of once per field and pass).
+__global__ void __launch_bounds__(256) k_slot_scenes(const SceneEntry* __restrict__ tab, int nscenes, int64_t ntot,
+                                                     uint32_t* __restrict__ slot_sc) {
+    const int64_t g = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+    if (g < ntot) slot_sc[g] = (uint32_t)slot_scene(tab, nscenes, g);
+}
+
 template <bool BATCH, bool BACK>
 __global__ void __launch_bounds__(256) k_reorder_field(ParticlesDev P0, const SceneEntry* __restrict__ tab,
-                                                       int nscenes, const uint32_t* __restrict__ perm, int64_t ntot,
-                                                       int f, uint32_t* __restrict__ scratch) {
+                                                       const uint32_t* __restrict__ slot_sc,
+                                                       const uint32_t* __restrict__ perm, int64_t ntot, int f,
+                                                       uint32_t* __restrict__ scratch) {
     const int64_t i = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
     if (i >= ntot) return;
     // gather: scratch[i] <- slot perm[i]; back: slot i <- scratch[i]
@@ -1393,7 +1402,7 @@ __global__ void __launch_bounds__(256) k_reorder_field(ParticlesDev P0, const Sc
     ParticlesDev P = P0;
     int64_t ps = 0;
     if (BATCH) {
-        const int sc = slot_scene(tab, nscenes, g);
+        const uint32_t sc = slot_sc[g];
         P = tab[sc].P;
         ps = tab[sc].pstart;
     }
@@ -1413,20 +1422,21 @@ __global__ void __launch_bounds__(256) k_reorder_field(ParticlesDev P0, const Sc
 }
 
 void launch_reorder_store(const ParticlesDev& P, const SceneEntry* tab, int nscenes, const uint32_t* perm,
-                          int64_t ntot, uint32_t* scratch, bool apic, hipStream_t st) {
+                          int64_t ntot, uint32_t* scratch, bool apic, hipStream_t st, uint32_t* slot_sc) {
     if (ntot <= 0) return;
     const dim3 grid((unsigned)((ntot + 255) / 256)), block(256);
+    if (tab) hipLaunchKernelGGL(k_slot_scenes, grid, block, 0, st, tab, nscenes, ntot, slot_sc);
     for (int f = 0; f < 12; ++f) {
         if (f == 2 && !apic) continue;
         if (tab) {
-            hipLaunchKernelGGL((k_reorder_field<true, false>), grid, block, 0, st, P, tab, nscenes, perm, ntot, f,
+            hipLaunchKernelGGL((k_reorder_field<true, false>), grid, block, 0, st, P, tab, slot_sc, perm, ntot, f,
                                scratch);
-            hipLaunchKernelGGL((k_reorder_field<true, true>), grid, block, 0, st, P, tab, nscenes, perm, ntot, f,
+            hipLaunchKernelGGL((k_reorder_field<true, true>), grid, block, 0, st, P, tab, slot_sc, perm, ntot, f,
                                scratch);
         } else {
-            hipLaunchKernelGGL((k_reorder_field<false, false>), grid, block, 0, st, P, tab, nscenes, perm, ntot, f,
+            hipLaunchKernelGGL((k_reorder_field<false, false>), grid, block, 0, st, P, tab, nullptr, perm, ntot, f,
                                scratch);
-            hipLaunchKernelGGL((k_reorder_field<false, true>), grid, block, 0, st, P, tab, nscenes, perm, ntot, f,
+            hipLaunchKernelGGL((k_reorder_field<false, true>), grid, block, 0, st, P, tab, nullptr, perm, ntot, f,
                                scratch);
         }
     }

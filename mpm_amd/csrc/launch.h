@@ -89,10 +89,11 @@ void launch_permute(const ParticlesDev& src, const ParticlesDev& dst, const uint
 void launch_gather_u32(const uint32_t* src, const uint32_t* idx, uint32_t* dst, int64_t n, hipStream_t st);
 // In-place permutation of the particle store into sorted order (slot i <-
 // slot perm[i]) through scratch (9 * ntot words).  With a scene table the
-// slots are global (member s owns [pstart_s, pstart_s + n_s)) and P is
-// unused; otherwise P is the one store and ntot = P.n.
+// slots are global (member s owns [pstart_s, pstart_s + n_s)), P is unused
+// and slot_sc (ntot words) receives each slot's scene; otherwise P is the
+// one store, ntot = P.n and slot_sc is unused.
 void launch_reorder_store(const ParticlesDev& P, const SceneEntry* tab, int nscenes, const uint32_t* perm,
-                          int64_t ntot, uint32_t* scratch, bool apic, hipStream_t st);
+                          int64_t ntot, uint32_t* scratch, bool apic, hipStream_t st, uint32_t* slot_sc = nullptr);
 void launch_grid_dense(const SceneDev& S, float* mass, float* vel, float* vstar, float* force, int have_vstar,
                        hipStream_t st);
 // Density raster readout from a one-plane mass grid D (D.grid = sum m w):

@@ -79,7 +79,7 @@ struct Solver {
     std::vector<void*> allocs;
     int64_t device_bytes = 0;
     // profiling
-    bool prof = false;
+    int prof = 0;  // 0 off, 1 every kernel, 2 the item kernels (p2g, g2p) and the store reorder only
     std::vector<Pending> pending;
     std::vector<hipEvent_t> event_pool;
     std::map<std::string, std::pair<int64_t, double>> prof_tot;

@@ -354,8 +354,10 @@ class Solver:
         N.check(self._lib.mpm_get_stats(self._h, C.byref(s)))
         return {f[0]: getattr(s, f[0]) for f in N.MpmStats._fields_}
 
-    def profile(self, enable: bool = True):
-        N.check(self._lib.mpm_profile_enable(self._h, 1 if enable else 0))
+    def profile(self, enable=True):
+        """enable: False/0 off, True/1 every kernel group, 2 only p2g, g2p and
+        the store reorder (fewer events in the stream)."""
+        N.check(self._lib.mpm_profile_enable(self._h, int(enable)))
 
     def profile_reset(self):
         N.check(self._lib.mpm_profile_reset(self._h))

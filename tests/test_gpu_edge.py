@@ -101,3 +101,29 @@ def test_cfl_monitor():
     state.velocity[1, 0] = np.nan
     s.write_particles(state)
     assert np.isnan(s.stats()["max_speed"])
+
+
+def test_profile_levels():
+    # level 1 times every launch group, level 2 only the item kernels and the
+    # store reorder; other levels are rejected
+    sc = scenes.snow_cube(seed=9, n=5000, res=32, transfer=N.APIC)
+    s = Solver(sc.desc, sc.particles)
+    s.profile(1)
+    s.profile_reset()
+    s.step(3)
+    full = s.profile_read()
+    assert {"p2g", "g2p", "sort_hist", "sort_scatter", "items", "grid_clear"} <= set(full), full
+    assert full["g2p"][0] == 3 and full["p2g"][0] == 3
+    s.profile(2)
+    s.profile_reset()
+    s.set_reorder_interval(2)
+    s.step(4)
+    items = s.profile_read()
+    assert set(items) == {"p2g", "g2p", "reorder"}, items
+    assert items["g2p"][0] == 4 and items["g2p"][1] > 0.0
+    with pytest.raises(Exception):
+        s.profile(3)
+    s.profile(False)
+    s.profile_reset()
+    s.step(1)
+    assert s.profile_read() == {}
