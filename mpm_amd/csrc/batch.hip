@@ -611,9 +611,16 @@ MPM_API int mpm_batch_autotune(void* batch, int trial_steps, mpm_tune* out) {
     HIP_TRY(hipEventCreate(&e1));
     const bool fixed = b->tune_fixed;
     b->tune_fixed = true;
+    // every trial starts from a store in cell order (the order decays within
+    // tens of steps, which would favour the first trials), with no reorder
+    // inside the timed steps
+    const int interval = b->reorder_interval;
+    b->reorder_interval = 0;
     auto measure = [&](const Tune& t, double& ms) -> int {
         b->tune = t;
-        int r = run_steps(*b, 1, false);  // warm-up (code objects, caches)
+        int r = reorder_members(*b);
+        if (r) return r;
+        r = run_steps(*b, 1, false);  // warm-up (code objects, caches)
         if (r) return r;
         HIP_TRY(hipEventRecord(e0, b->st));
         r = run_steps(*b, trial_steps, false);
@@ -643,6 +650,7 @@ MPM_API int mpm_batch_autotune(void* batch, int trial_steps, mpm_tune* out) {
     (void)hipEventDestroy(e0);
     (void)hipEventDestroy(e1);
     b->tune_fixed = fixed;
+    b->reorder_interval = interval;
     if (rc) return rc;
     b->tune = best;
     {
