@@ -62,11 +62,11 @@ static unsigned resident_grid(F* kernel, int block, int requested) {
     hipLaunchKernelGGL(K, dim3(resident_grid(K, BLOCK, REQ)), dim3(BLOCK), 0, ST, __VA_ARGS__)
 
 // Minimum waves per SIMD asked of the register allocator for G2P.  APIC
-// G2P needs 151-171 VGPRs: a cap of 168 (3 waves per SIMD with 512
+// G2P needs 146-171 VGPRs: a cap of 168 (3 waves per SIMD with 512
 // registers per lane) costs no spills, while 171 would drop it to 2 waves.
-// PIC-FLIP G2P (187-256 VGPRs) would spill 124-252 B per lane under the
-// same cap, so it keeps the compiler's allocation unless MPM_PIC_G2P_WAVES
-// says otherwise.
+// PIC-FLIP G2P keeps the compiler's allocation unless MPM_PIC_G2P_WAVES
+// says otherwise (linear 127-128 VGPRs, quadratic 133-136, cubic 166-172:
+// 3 or more waves per SIMD without a cap).
 #ifndef MPM_APIC_G2P_WAVES
 #define MPM_APIC_G2P_WAVES 3
 #endif
