@@ -350,7 +350,7 @@ def main(argv=None):
         return 0
 
     if a.gpus != 1:
-        print(f"bench.py runs on one GPU only (asked for --gpus {a.gpus}); multi-GPU is not part of this build",
+        print(f"bench.py runs on one GPU only (asked for --gpus {a.gpus}); the solver shards nothing across GPUs",
               file=sys.stderr)
         return 2
 
