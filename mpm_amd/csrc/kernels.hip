@@ -18,6 +18,7 @@
 #include <mutex>
 #include <utility>
 
+#include "blockscan.h"
 #include "devmath.h"
 #include "launch.h"
 
@@ -134,35 +135,63 @@ void launch_iota(uint32_t* p, int64_t n, hipStream_t st) {
 __device__ __forceinline__ bool item_head(const uint32_t* sk, int64_t i, uint32_t item_max) {
     return i == 0 || (sk[i] >> 6) != (sk[i - 1] >> 6) || ((uint64_t)i % item_max) == 0;
 }
-__global__ void k_item_flags(const uint32_t* __restrict__ sk, int64_t n, uint32_t item_max,
-                             uint32_t* __restrict__ flags) {
-    int64_t i = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
-    if (i < n) flags[i] = item_head(sk, i, item_max) ? 1u : 0u;
-}
-// Writes the start position of each item, its scene-local key block and,
-// for a batch, its scene: the last scene whose key offset is <= the item's
-// first key.  items[n + k] holds the block of item k.
-__global__ void k_item_compact(const uint32_t* __restrict__ sk, int64_t n, uint32_t item_max,
-                               const uint32_t* __restrict__ pos, uint32_t* __restrict__ items,
-                               const uint32_t* __restrict__ key_offsets, int nscenes,
-                               uint32_t* __restrict__ item_scene) {
-    int64_t i = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
-    if (i >= n || !item_head(sk, i, item_max)) return;
-    const uint32_t k = pos[i];
-    const uint32_t key = sk[i];
-    items[k] = (uint32_t)i;
-    uint32_t koff = 0;
-    if (item_scene) {
-        int lo = 0, hi = nscenes - 1;
-        while (lo < hi) {
-            const int mid = (lo + hi + 1) >> 1;
-            if (key_offsets[mid] <= key) lo = mid;
-            else hi = mid - 1;
-        }
-        item_scene[k] = (uint32_t)lo;
-        koff = key_offsets[lo];
+// Work items in two passes over the sorted keys, with the head flags
+// recomputed in each pass instead of stored: per-tile head counts, a scan
+// of the tile counts, then every head written at its rank.  items[k] is the
+// start of item k, items[n + k] its scene-local key block and, for a batch,
+// item_scene[k] its scene (the last scene whose key offset is <= the item's
+// first key).
+__global__ void __launch_bounds__(kScanWG) k_item_count(const uint32_t* __restrict__ sk, int64_t n,
+                                                        uint32_t item_max, uint32_t* __restrict__ partial) {
+    __shared__ uint32_t ws[kScanWG / 64];
+    const int64_t base = (int64_t)blockIdx.x * kScanTile + (int64_t)threadIdx.x * kScanIPT;
+    uint32_t c = 0;
+#pragma unroll
+    for (int j = 0; j < kScanIPT; ++j) {
+        const int64_t i = base + j;
+        if (i < n && item_head(sk, i, item_max)) ++c;
     }
-    items[n + k] = (key - koff) >> 6;
+    uint32_t tot;
+    block_exclusive_scan(c, ws, tot);
+    if (threadIdx.x == 0) partial[blockIdx.x] = tot;
+}
+__global__ void __launch_bounds__(kScanWG) k_item_emit(const uint32_t* __restrict__ sk, int64_t n, uint32_t item_max,
+                                                       const uint32_t* __restrict__ tile_start,
+                                                       uint32_t* __restrict__ items,
+                                                       const uint32_t* __restrict__ key_offsets, int nscenes,
+                                                       uint32_t* __restrict__ item_scene) {
+    __shared__ uint32_t ws[kScanWG / 64];
+    const int64_t base = (int64_t)blockIdx.x * kScanTile + (int64_t)threadIdx.x * kScanIPT;
+    bool head[kScanIPT];
+    uint32_t c = 0;
+#pragma unroll
+    for (int j = 0; j < kScanIPT; ++j) {
+        const int64_t i = base + j;
+        head[j] = i < n && item_head(sk, i, item_max);
+        c += head[j] ? 1u : 0u;
+    }
+    uint32_t tot;
+    uint32_t k = tile_start[blockIdx.x] + block_exclusive_scan(c, ws, tot);
+#pragma unroll
+    for (int j = 0; j < kScanIPT; ++j) {
+        if (!head[j]) continue;
+        const int64_t i = base + j;
+        const uint32_t key = sk[i];
+        items[k] = (uint32_t)i;
+        uint32_t koff = 0;
+        if (item_scene) {
+            int lo = 0, hi = nscenes - 1;
+            while (lo < hi) {
+                const int mid = (lo + hi + 1) >> 1;
+                if (key_offsets[mid] <= key) lo = mid;
+                else hi = mid - 1;
+            }
+            item_scene[k] = (uint32_t)lo;
+            koff = key_offsets[lo];
+        }
+        items[n + k] = (key - koff) >> 6;
+        ++k;
+    }
 }
 
 void launch_items(const uint32_t* sorted_keys, int64_t n, uint32_t* flags, uint32_t* scan_temp, uint32_t* items,
@@ -170,11 +199,12 @@ void launch_items(const uint32_t* sorted_keys, int64_t n, uint32_t* flags, uint3
                   int nscenes, uint32_t* item_scene) {
     if (n <= 0) return;
     if (item_max == 0) item_max = kItemMax;
-    dim3 g((unsigned)((n + 255) / 256)), b(256);
-    hipLaunchKernelGGL(k_item_flags, g, b, 0, st, sorted_keys, n, item_max, flags);
-    exclusive_scan(flags, n, scan_temp, item_count, st);
-    hipLaunchKernelGGL(k_item_compact, g, b, 0, st, sorted_keys, n, item_max, flags, items, key_offsets, nscenes,
-                       item_scene);
+    const int64_t tiles = (n + kScanTile - 1) / kScanTile;
+    // flags[0..tiles) holds the tile head counts, then their exclusive scan
+    hipLaunchKernelGGL(k_item_count, dim3((unsigned)tiles), dim3(kScanWG), 0, st, sorted_keys, n, item_max, flags);
+    exclusive_scan(flags, tiles, scan_temp, item_count, st);
+    hipLaunchKernelGGL(k_item_emit, dim3((unsigned)tiles), dim3(kScanWG), 0, st, sorted_keys, n, item_max, flags,
+                       items, key_offsets, nscenes, item_scene);
 }
 
 // ------------------------------------------------------- item decoding
