@@ -195,12 +195,25 @@ __global__ void __launch_bounds__(kScanWG) k_scan_reduce(const uint32_t* __restr
     if (threadIdx.x == 0) partial[blockIdx.x] = tot;
 }
 
-// Exclusive scan of each tile, offset by the (already scanned) tile prefix.
-// in and out may alias.  If total != null, thread 0 of the last tile writes
-// the grand total.
+// Exclusive scan of each tile, offset by the tile prefix: partial_scan[b]
+// (already scanned), or with RAW the sum of the unscanned tile totals
+// partial[0..b), which the workgroup adds up itself (few tiles: this saves
+// the launches of scanning them).  in and out may alias.  If total !=
+// null, the last thread of the last tile writes the grand total.
+template <bool RAW>
 __global__ void __launch_bounds__(kScanWG) k_scan_down(const uint32_t* in, int64_t n, const uint32_t* partial_scan,
                                                        uint32_t* out, uint32_t* total) {
     __shared__ uint32_t ws[kScanWG / 64];
+    uint32_t tile_prefix = 0;
+    if (partial_scan) {
+        if (RAW) {
+            uint32_t part = 0;
+            for (uint32_t t = threadIdx.x; t < blockIdx.x; t += kScanWG) part += partial_scan[t];
+            block_exclusive_scan(part, ws, tile_prefix);
+        } else {
+            tile_prefix = partial_scan[blockIdx.x];
+        }
+    }
     int64_t base = (int64_t)blockIdx.x * kScanTile;
     uint32_t v[kScanIPT];
     uint32_t s = 0;
@@ -212,7 +225,7 @@ __global__ void __launch_bounds__(kScanWG) k_scan_down(const uint32_t* in, int64
     }
     uint32_t tot;
     uint32_t pre = block_exclusive_scan(s, ws, tot);
-    uint32_t off = (partial_scan ? partial_scan[blockIdx.x] : 0u) + pre;
+    uint32_t off = tile_prefix + pre;
 #pragma unroll
     for (int i = 0; i < kScanIPT; ++i) {
         int64_t idx = base + (int64_t)threadIdx.x * kScanIPT + i;
@@ -237,15 +250,21 @@ int64_t scan_temp_words(int64_t n) {
 void exclusive_scan(uint32_t* data, int64_t n, uint32_t* temp, uint32_t* total, hipStream_t st) {
     if (n <= 0) return;
     if (n <= kScanTile) {
-        hipLaunchKernelGGL(k_scan_down, dim3(1), dim3(kScanWG), 0, st, data, n, (const uint32_t*)nullptr, data, total);
+        hipLaunchKernelGGL(k_scan_down<false>, dim3(1), dim3(kScanWG), 0, st, data, n, (const uint32_t*)nullptr, data,
+                           total);
         return;
     }
     int64_t tiles = (n + kScanTile - 1) / kScanTile;
     uint32_t* partial = temp;
     hipLaunchKernelGGL(k_scan_reduce, dim3((unsigned)tiles), dim3(kScanWG), 0, st, data, n, partial);
+    if (tiles <= 2048) {  // each tile sums at most 2047 tile totals itself
+        hipLaunchKernelGGL(k_scan_down<true>, dim3((unsigned)tiles), dim3(kScanWG), 0, st, data, n,
+                           (const uint32_t*)partial, data, total);
+        return;
+    }
     exclusive_scan(partial, tiles, temp + tiles, nullptr, st);
-    hipLaunchKernelGGL(k_scan_down, dim3((unsigned)tiles), dim3(kScanWG), 0, st, data, n, (const uint32_t*)partial,
-                       data, total);
+    hipLaunchKernelGGL(k_scan_down<false>, dim3((unsigned)tiles), dim3(kScanWG), 0, st, data, n,
+                       (const uint32_t*)partial, data, total);
 }
 
 int64_t radix_tiles(int64_t n) { return (n + kSortTile - 1) / kSortTile; }
