@@ -23,7 +23,7 @@ from .solver import Solver
 @dataclass
 class Tune:
     item_max: int = 512
-    g2p_threads: int = 64
+    g2p_threads: int = 128
     workgroups: int = 0
     bucket: int = 0
     step_ms: float = 0.0

@@ -25,6 +25,12 @@
 
 using namespace mpm;
 
+// Threads per workgroup of the fused G2P of one scene: 128 (two waves per
+// item) runs the linear and quadratic APIC G2P at 4 waves per SIMD.
+#ifndef MPM_SOLVER_G2P_WG
+#define MPM_SOLVER_G2P_WG 128
+#endif
+
 namespace mpm {
 namespace host {
 
@@ -349,7 +355,7 @@ void run_fused_step(Solver& s) {
     {
         Scope sc(h, "g2p");
         launch_g2p(s.S, s.P, s.I, s.interp, s.apic, fold ? 3 : 0, 1, s.S.particle_collision, s.item_wgs, s.st,
-                   s.kin);
+                   s.kin, nullptr, MPM_SOLVER_G2P_WG);
     }
     s.grid_dirty = true;
     s.count_on_clear = true;

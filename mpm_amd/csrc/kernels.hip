@@ -63,20 +63,27 @@ static unsigned resident_grid(F* kernel, int block, int requested) {
     hipLaunchKernelGGL(K, dim3(resident_grid(K, BLOCK, REQ)), dim3(BLOCK), 0, ST, __VA_ARGS__)
 
 // Minimum waves per SIMD asked of the register allocator for G2P.  APIC
-// G2P needs 146-171 VGPRs: a cap of 168 (3 waves per SIMD with 512
-// registers per lane) costs no spills, while 171 would drop it to 2 waves.
-// PIC-FLIP G2P keeps the compiler's allocation unless MPM_PIC_G2P_WAVES
-// says otherwise (linear 127-128 VGPRs, quadratic 133-136, cubic 166-172:
-// 3 or more waves per SIMD without a cap).
+// G2P with 128-thread workgroups fits 128 VGPRs for the linear and
+// quadratic stencils (4 waves per SIMD, no spill; 64 and 256 threads would
+// spill 8-10 registers under that cap), and the other APIC variants fit the
+// 168 of 3 waves.  Measured on the 8.4M-particle slab: G2P 0.833 ms (64
+// threads, 3 waves) -> 0.821 ms (128 threads, 4 waves).  PIC-FLIP G2P keeps
+// the compiler's allocation unless MPM_PIC_G2P_WAVES says otherwise (linear
+// 127-128 VGPRs, quadratic 133-134, cubic 156-179).
 #ifndef MPM_APIC_G2P_WAVES
 #define MPM_APIC_G2P_WAVES 3
+#endif
+#ifndef MPM_APIC_G2P_WAVES_WG128
+#define MPM_APIC_G2P_WAVES_WG128 4
 #endif
 #ifndef MPM_PIC_G2P_WAVES
 #define MPM_PIC_G2P_WAVES 1
 #endif
-template <int APIC>
+template <int INTERP, int APIC, int WG>
 struct G2PWaves {
-    static constexpr int value = APIC ? MPM_APIC_G2P_WAVES : MPM_PIC_G2P_WAVES;
+    static constexpr int value =
+        APIC ? ((WG == 128 && INTERP != MPM_INTERP_CUBIC_BSPLINE) ? MPM_APIC_G2P_WAVES_WG128 : MPM_APIC_G2P_WAVES)
+             : MPM_PIC_G2P_WAVES;
 };
 
 // Minimum waves per SIMD for P2G.  The folded (4-value) and mass-only
@@ -728,7 +735,7 @@ __device__ __forceinline__ void commit_folded_tile(const SceneDev& S, const Item
 // particlesCollision (:428-445) and advectParticles (:381-391) fused in.
 // SRC 2 is initialize()'s G2PVolume (src/mpm_solver3d.cu:173-214).
 template <int INTERP, int APIC, int SRC, bool BATCH, int WG>
-__global__ void __launch_bounds__(WG) __attribute__((amdgpu_waves_per_eu(G2PWaves<APIC>::value))) k_g2p(SceneDev S0, ParticlesDev P0, ItemsDev I, const SceneEntry* __restrict__ tab,
+__global__ void __launch_bounds__(WG) __attribute__((amdgpu_waves_per_eu(G2PWaves<INTERP, APIC, WG>::value))) k_g2p(SceneDev S0, ParticlesDev P0, ItemsDev I, const SceneEntry* __restrict__ tab,
                                             int advect, int pcollide, uint32_t* __restrict__ next_keys) {
     constexpr int W = Stencil<INTERP>::W;
     constexpr int T = 4 + W - 1;
@@ -880,52 +887,61 @@ __global__ void __launch_bounds__(WG) __attribute__((amdgpu_waves_per_eu(G2PWave
                         dxn[2][j] = (S.oz + S.h * (float)(bz + j)) - x2;
                     }
                 }
-                // Tensor-product factorisation: w = Nx[a] (Ny[b] Nz[c]) and
-                // grad w = (dNx[a] yz, Nx[a] dNy[b] Nz[c], Nx[a] Ny[b] dNz[c]) / h,
-                // so the (b, c) sums are formed first for each a.
-                float yz[W][W], dyz[W][W], ydz[W][W];
+                // Nested tensor-product sums, z innermost: with w = Nx Ny Nz,
+                // grad w = (dNx Ny Nz, Nx dNy Nz, Nx Ny dNz) / h and the APIC
+                // moment w (x_i - x_p), every sum over the stencil is
+                //   sum_a X[a] sum_b Y[b] sum_c Z[c] v(a, b, c)
+                // for one of a few per-axis weight vectors; the z sums (3
+                // weights x 3 components per node) feed the y sums (5 per
+                // (a, b)), which feed the x sums (7 per a).
+                float zdx[W], ydx[W];
 #pragma unroll
-                for (int b = 0; b < W; ++b)
-#pragma unroll
-                    for (int c = 0; c < W; ++c) {
-                        yz[b][c] = Ny[b] * Nz[c];
-                        dyz[b][c] = dNy[b] * Nz[c];
-                        ydz[b][c] = Ny[b] * dNz[c];
-                    }
+                for (int j = 0; j < W; ++j) {
+                    zdx[j] = APIC ? Nz[j] * dxn[2][j] : 0.0f;
+                    ydx[j] = APIC ? Ny[j] * dxn[1][j] : 0.0f;
+                }
                 // tile cell of the stencil's first node (constant offsets per node)
                 const float* const tq = tile + ((lz * T + ly) * T + lx);
 #pragma unroll
                 for (int a = 0; a < W; ++a) {
-                    float S0[3] = {0.0f, 0.0f, 0.0f}, Sy[3] = {0.0f, 0.0f, 0.0f}, Sz[3] = {0.0f, 0.0f, 0.0f};
-                    float Sdy[3] = {0.0f, 0.0f, 0.0f}, Sdz[3] = {0.0f, 0.0f, 0.0f};
+                    float Y0[3] = {0.0f, 0.0f, 0.0f}, Yd[3] = {0.0f, 0.0f, 0.0f}, Yz[3] = {0.0f, 0.0f, 0.0f};
+                    float Yx[3] = {0.0f, 0.0f, 0.0f}, Yzx[3] = {0.0f, 0.0f, 0.0f};
 #pragma unroll
-                    for (int b = 0; b < W; ++b)
+                    for (int b = 0; b < W; ++b) {
+                        float Z0[3] = {0.0f, 0.0f, 0.0f}, Zd[3] = {0.0f, 0.0f, 0.0f}, Zx[3] = {0.0f, 0.0f, 0.0f};
 #pragma unroll
                         for (int c = 0; c < W; ++c) {
                             const int q = (c * T + b) * T + a;
                             const float wv[3] = {tq[q], tq[TN + q], tq[2 * TN + q]};
 #pragma unroll
                             for (int k = 0; k < 3; ++k) {
-                                S0[k] += yz[b][c] * wv[k];
-                                Sy[k] += dyz[b][c] * wv[k];
-                                Sz[k] += ydz[b][c] * wv[k];
-                                if (APIC) {
-                                    Sdy[k] += (yz[b][c] * dxn[1][b]) * wv[k];
-                                    Sdz[k] += (yz[b][c] * dxn[2][c]) * wv[k];
-                                }
+                                Z0[k] += Nz[c] * wv[k];
+                                Zd[k] += dNz[c] * wv[k];
+                                if (APIC) Zx[k] += zdx[c] * wv[k];
                             }
                         }
+#pragma unroll
+                        for (int k = 0; k < 3; ++k) {
+                            Y0[k] += Ny[b] * Z0[k];   // sum Ny Nz v
+                            Yd[k] += dNy[b] * Z0[k];  // sum dNy Nz v
+                            Yz[k] += Ny[b] * Zd[k];   // sum Ny dNz v
+                            if (APIC) {
+                                Yx[k] += ydx[b] * Z0[k];  // sum Ny dy Nz v
+                                Yzx[k] += Ny[b] * Zx[k];  // sum Ny Nz dz v
+                            }
+                        }
+                    }
                     const float xa = Nx[a], dxa = dNx[a];
-                    vpic0 += xa * S0[0]; vpic1 += xa * S0[1]; vpic2 += xa * S0[2];
+                    vpic0 += xa * Y0[0]; vpic1 += xa * Y0[1]; vpic2 += xa * Y0[2];
 #pragma unroll
                     for (int k = 0; k < 3; ++k) {
-                        grad[k][0] += dxa * S0[k];
-                        grad[k][1] += xa * Sy[k];
-                        grad[k][2] += xa * Sz[k];
+                        grad[k][0] += dxa * Y0[k];
+                        grad[k][1] += xa * Yd[k];
+                        grad[k][2] += xa * Yz[k];
                         if (APIC) {
-                            Bm[k][0] += (xa * dxn[0][a]) * S0[k];
-                            Bm[k][1] += xa * Sdy[k];
-                            Bm[k][2] += xa * Sdz[k];
+                            Bm[k][0] += (xa * dxn[0][a]) * Y0[k];
+                            Bm[k][1] += xa * Yx[k];
+                            Bm[k][2] += xa * Yzx[k];
                         }
                     }
                 }
@@ -934,17 +950,21 @@ __global__ void __launch_bounds__(WG) __attribute__((amdgpu_waves_per_eu(G2PWave
                     // stencil so the main gather holds only the v* sums
 #pragma unroll
                     for (int a = 0; a < W; ++a) {
-                        float SF[3] = {0.0f, 0.0f, 0.0f};
+                        float YF[3] = {0.0f, 0.0f, 0.0f};
 #pragma unroll
-                        for (int b = 0; b < W; ++b)
+                        for (int b = 0; b < W; ++b) {
+                            float ZF[3] = {0.0f, 0.0f, 0.0f};
 #pragma unroll
                             for (int c = 0; c < W; ++c) {
                                 const int q = (c * T + b) * T + a;
 #pragma unroll
-                                for (int k = 0; k < 3; ++k) SF[k] += yz[b][c] * tq[(3 + k) * TN + q];
+                                for (int k = 0; k < 3; ++k) ZF[k] += Nz[c] * tq[(3 + k) * TN + q];
                             }
+#pragma unroll
+                            for (int k = 0; k < 3; ++k) YF[k] += Ny[b] * ZF[k];
+                        }
                         const float xa = Nx[a];
-                        vf0 += xa * SF[0]; vf1 += xa * SF[1]; vf2 += xa * SF[2];
+                        vf0 += xa * YF[0]; vf1 += xa * YF[1]; vf2 += xa * YF[2];
                     }
                 }
             }
