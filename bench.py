@@ -14,7 +14,9 @@ nodes, per the byte model below) divided by its mean HIP-event duration.
 ``cpu_baseline`` is the C++ oracle (a port of the reference step) on the
 host: on all host cores (OpenMP loops, atomic grid sums, as the reference's
 atomicAdd scatter) over the whole workload for two steps, with the serial
-oracle on a sub-box of it beside that (``serial``).
+oracle on a sub-box of it beside that (``serial``).  ``small_scene`` times the reference's
+CPU-runnable configuration (16,384-particle snow cube, PIC-FLIP) eagerly and
+as a replayed hipGraph, beside the serial oracle on the same scene.
 
 Algorithmic bytes (FP32): APIC per particle-update 364 B = P2G 148 B
 (x, v, B, Fe, Fp, m, V0, mu0, lambda0) + G2P 216 B (reads x, Fe, Fp; writes
@@ -296,6 +298,40 @@ def batch_bench(count, steps, warmup, seed=3):
     }
 
 
+def small_scene_bench(steps=200, cpu=True):
+    """The reference's CPU-runnable configuration (snow cube, 16,384
+    particles, 64^3, PIC-FLIP quadratic): launch-bound on the GPU, so it is
+    timed eagerly and as a replayed hipGraph, beside the serial oracle."""
+    from mpm_amd import scenes
+    from mpm_amd.solver import Solver
+
+    sc = scenes.workload("snow_cube")
+    n = sc.particles.count
+    out = {"workload": f"snow_cube: {n} particles, 64^3 grid, PIC-FLIP quadratic, dt 1e-4", "steps": steps}
+    for mode in ("eager", "graph"):
+        s = Solver(sc.desc, sc.particles)
+        run = s.step if mode == "eager" else s.step_graph
+        run(5)
+        s.synchronize()
+        t0 = time.perf_counter()
+        run(steps)
+        s.synchronize()
+        el = time.perf_counter() - t0
+        s.close()
+        out[mode] = {"ms_per_step": 1e3 * el / steps, "value": n * steps / el}
+    if cpu:
+        from oracle import pyoracle
+
+        o = pyoracle.OracleSolver(sc.desc, sc.particles, 32)
+        o.time_steps(1)
+        t = o.time_steps(10)
+        o.close()
+        out["cpu_serial"] = {"ms_per_step": 1e3 * t / 10, "value": n * 10 / t, "cores": 1, "kind": "port"}
+    log(f"small scene: eager {out['eager']['ms_per_step']:.3f} ms/step, graph {out['graph']['ms_per_step']:.3f} ms/step"
+        + (f", serial oracle {out['cpu_serial']['ms_per_step']:.1f} ms/step" if cpu else ""))
+    return out
+
+
 def serve_bench(window_s, seed=4, rate=20.0):
     """Poisson request stream (seeded) over a window of window_s seconds,
     served to completion under a 200 GB HBM budget."""
@@ -326,6 +362,7 @@ def main(argv=None):
     ap.add_argument("--workload", default="apic_slab", choices=["apic_slab", "two_balls", "snow_cube"])
     ap.add_argument("--graph", action="store_true", help="replay the step as a captured hipGraph")
     ap.add_argument("--no-cpu-baseline", action="store_true")
+    ap.add_argument("--no-small", action="store_true", help="skip the 16k-particle snow cube timing")
     ap.add_argument("--seed", type=int, default=None)
     ap.add_argument("--batch-scenes", type=int, default=256, help="scenes of the batched run (0: skip)")
     ap.add_argument("--batch-steps", type=int, default=20)
@@ -461,6 +498,7 @@ def main(argv=None):
     solver.close()
     batched = batch_bench(a.batch_scenes, a.batch_steps, 3) if a.batch_scenes > 0 else None
     served = serve_bench(a.serve_seconds) if a.serve_seconds > 0 else None
+    small = small_scene_bench(cpu=not a.no_cpu_baseline) if a.workload == "apic_slab" and not a.no_small else None
 
     cpu = None
     if not a.no_cpu_baseline:
@@ -495,6 +533,7 @@ def main(argv=None):
         "batched": batched,
         "served": served,
         "density_raster": density,
+        "small_scene": small,
     }
     print(json.dumps(out), flush=True)
     return 0
