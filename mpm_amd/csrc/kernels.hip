@@ -1452,7 +1452,11 @@ __global__ void __launch_bounds__(256) k_reorder_field(ParticlesDev P0, const Sc
     ParticlesDev P = P0;
     int64_t ps = 0;
     if (BATCH) {
-        const uint32_t sc = slot_sc[g];
+        // slot g = perm[i] lies in the same scene as position i: scenes hold
+        // ascending, disjoint key ranges in slot order, so the sorted order
+        // keeps each scene's particles in its own slot range; reading the
+        // scene at i keeps that load coalesced and off the perm[i] chain
+        const uint32_t sc = slot_sc[i];
         P = tab[sc].P;
         ps = tab[sc].pstart;
     }
