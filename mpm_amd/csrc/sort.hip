@@ -65,7 +65,7 @@ __global__ void __launch_bounds__(kSortWG) k_radix_hist(const uint32_t* __restri
     for (int d = tid; d < kRadix; d += kSortWG) hist[(int64_t)d * tiles + tile] = h[d];
 }
 
-template <int BITS>
+template <int BITS, bool RAW>
 __global__ void __launch_bounds__(kSortWG) k_radix_scatter(const uint32_t* __restrict__ keys_in,
                                                            const uint32_t* __restrict__ vals_in,
                                                            uint32_t* __restrict__ keys_out,
@@ -134,7 +134,38 @@ __global__ void __launch_bounds__(kSortWG) k_radix_scatter(const uint32_t* __res
         }
         dtot[j] = acc;
         total += acc;
-        global_base[d] = hist_scan[(int64_t)d * tiles + tile];
+        if (!RAW) global_base[d] = hist_scan[(int64_t)d * tiles + tile];
+    }
+    if (RAW) {
+        // few tiles: the digit table is not scanned beforehand; each
+        // workgroup forms its bases from the raw counts, as the sum over
+        // smaller digits of their totals plus this digit's earlier tiles
+        uint32_t gtot[DPT];
+        uint32_t gsum = 0;
+#pragma unroll
+        for (int j = 0; j < DPT; ++j) {
+            gtot[j] = 0;
+            if (!owns) continue;
+            const int d = tid * DPT + j;
+            uint32_t before = 0, all = 0;
+            for (int64_t t = 0; t < tiles; ++t) {
+                const uint32_t c = hist_scan[(int64_t)d * tiles + t];
+                before += t < tile ? c : 0u;
+                all += c;
+            }
+            global_base[d] = before;
+            gtot[j] = all;
+            gsum += all;
+        }
+        uint32_t gall;
+        uint32_t run = block_exclusive_scan(gsum, wave_sums, gall);
+        if (owns) {
+#pragma unroll
+            for (int j = 0; j < DPT; ++j) {
+                global_base[tid * DPT + j] += run;
+                run += gtot[j];
+            }
+        }
     }
     // Block exclusive scan of the per-thread digit totals.
     uint32_t incl = total;
@@ -268,6 +299,8 @@ void exclusive_scan(uint32_t* data, int64_t n, uint32_t* temp, uint32_t* total, 
 }
 
 int64_t radix_tiles(int64_t n) { return (n + kSortTile - 1) / kSortTile; }
+// Sorts of at most this many tiles skip the separate digit-table scan.
+constexpr int64_t kRawTiles = 32;
 int64_t radix_hist_words(int64_t n) { return radix_tiles(n) * kMaxRadix; }
 
 // Digit widths.  Large sorts (>= 2^20 keys) use digits of at most 7 bits:
@@ -314,16 +347,26 @@ int radix_sort_pairs(const uint32_t* kin, const uint32_t* vin, uint32_t* ka, uin
         else MPM_HIST(6);
 #undef MPM_HIST
         prof.end("sort_hist");
+        // up to kRawTiles tiles the scatter reads the raw digit table itself
+        // (one launch fewer per pass for small sorts)
+        const bool raw = tiles <= kRawTiles;
         prof.begin("sort_scan");
-        exclusive_scan(hist, tiles * radix, scan_temp, nullptr, st);
+        if (!raw) exclusive_scan(hist, tiles * radix, scan_temp, nullptr, st);
         prof.end("sort_scan");
         prof.begin("sort_scatter");
-#define MPM_SCATTER(B) hipLaunchKernelGGL(k_radix_scatter<B>, dim3((unsigned)tiles), dim3(kSortWG), 0, st, ki, vi, \
-                                          ko, vo, n, shift, mask, hist, tiles)
-        if (dw == 9) MPM_SCATTER(9);
-        else if (dw == 8) MPM_SCATTER(8);
-        else if (dw == 7) MPM_SCATTER(7);
-        else MPM_SCATTER(6);
+#define MPM_SCATTER(B, R) hipLaunchKernelGGL((k_radix_scatter<B, R>), dim3((unsigned)tiles), dim3(kSortWG), 0, st, ki, \
+                                             vi, ko, vo, n, shift, mask, hist, tiles)
+        if (raw) {
+            if (dw == 9) MPM_SCATTER(9, true);
+            else if (dw == 8) MPM_SCATTER(8, true);
+            else if (dw == 7) MPM_SCATTER(7, true);
+            else MPM_SCATTER(6, true);
+        } else {
+            if (dw == 9) MPM_SCATTER(9, false);
+            else if (dw == 8) MPM_SCATTER(8, false);
+            else if (dw == 7) MPM_SCATTER(7, false);
+            else MPM_SCATTER(6, false);
+        }
 #undef MPM_SCATTER
         prof.end("sort_scatter");
         shift += bits;
