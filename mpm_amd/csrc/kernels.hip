@@ -162,12 +162,25 @@ __global__ void __launch_bounds__(kScanWG) k_item_count(const uint32_t* __restri
     block_exclusive_scan(c, ws, tot);
     if (threadIdx.x == 0) partial[blockIdx.x] = tot;
 }
+// tile_start: the scanned tile counts, or with RAW the raw counts, which
+// each workgroup sums over the tiles before it (and the last one writes the
+// item count): few tiles, one launch fewer.
+template <bool RAW>
 __global__ void __launch_bounds__(kScanWG) k_item_emit(const uint32_t* __restrict__ sk, int64_t n, uint32_t item_max,
                                                        const uint32_t* __restrict__ tile_start,
                                                        uint32_t* __restrict__ items,
                                                        const uint32_t* __restrict__ key_offsets, int nscenes,
-                                                       uint32_t* __restrict__ item_scene) {
+                                                       uint32_t* __restrict__ item_scene,
+                                                       uint32_t* __restrict__ item_count) {
     __shared__ uint32_t ws[kScanWG / 64];
+    uint32_t first = 0;
+    if (RAW) {
+        uint32_t part = 0;
+        for (uint32_t t = threadIdx.x; t < blockIdx.x; t += kScanWG) part += tile_start[t];
+        block_exclusive_scan(part, ws, first);
+    } else {
+        first = tile_start[blockIdx.x];
+    }
     const int64_t base = (int64_t)blockIdx.x * kScanTile + (int64_t)threadIdx.x * kScanIPT;
     bool head[kScanIPT];
     uint32_t c = 0;
@@ -178,7 +191,8 @@ __global__ void __launch_bounds__(kScanWG) k_item_emit(const uint32_t* __restric
         c += head[j] ? 1u : 0u;
     }
     uint32_t tot;
-    uint32_t k = tile_start[blockIdx.x] + block_exclusive_scan(c, ws, tot);
+    uint32_t k = first + block_exclusive_scan(c, ws, tot);
+    if (RAW && blockIdx.x == gridDim.x - 1 && threadIdx.x == 0) *item_count = first + tot;
 #pragma unroll
     for (int j = 0; j < kScanIPT; ++j) {
         if (!head[j]) continue;
@@ -207,11 +221,17 @@ void launch_items(const uint32_t* sorted_keys, int64_t n, uint32_t* flags, uint3
     if (n <= 0) return;
     if (item_max == 0) item_max = kItemMax;
     const int64_t tiles = (n + kScanTile - 1) / kScanTile;
-    // flags[0..tiles) holds the tile head counts, then their exclusive scan
+    // flags[0..tiles) holds the tile head counts (then, for many tiles,
+    // their exclusive scan)
     hipLaunchKernelGGL(k_item_count, dim3((unsigned)tiles), dim3(kScanWG), 0, st, sorted_keys, n, item_max, flags);
+    if (tiles <= 2048) {
+        hipLaunchKernelGGL(k_item_emit<true>, dim3((unsigned)tiles), dim3(kScanWG), 0, st, sorted_keys, n, item_max,
+                           flags, items, key_offsets, nscenes, item_scene, item_count);
+        return;
+    }
     exclusive_scan(flags, tiles, scan_temp, item_count, st);
-    hipLaunchKernelGGL(k_item_emit, dim3((unsigned)tiles), dim3(kScanWG), 0, st, sorted_keys, n, item_max, flags,
-                       items, key_offsets, nscenes, item_scene);
+    hipLaunchKernelGGL(k_item_emit<false>, dim3((unsigned)tiles), dim3(kScanWG), 0, st, sorted_keys, n, item_max,
+                       flags, items, key_offsets, nscenes, item_scene, item_count);
 }
 
 // ------------------------------------------------------- item decoding
