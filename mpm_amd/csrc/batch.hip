@@ -603,7 +603,7 @@ MPM_API int mpm_batch_autotune(void* batch, int trial_steps, mpm_tune* out) {
     // G2P workgroup size, then persistent workgroups (one resident round,
     // or a fixed 8 or 16 per CU).  Every trial step is a real step of every
     // scene (the parameters change the work split, not the arithmetic).
-    const int item_opts[] = {256, 512, 1024};
+    const int item_opts[] = {128, 256, 512, 1024};
     const int wg_opts[] = {64, 128, 256};
     const int wgs_opts[] = {0, b->cus * 8, b->cus * 16};
     hipEvent_t e0 = nullptr, e1 = nullptr;
@@ -636,7 +636,7 @@ MPM_API int mpm_batch_autotune(void* batch, int trial_steps, mpm_tune* out) {
     double best_ms = 0.0;
     rc = measure(best, best_ms);
     for (int stage = 0; stage < 3 && !rc; ++stage) {
-        for (int k = 0; k < 3 && !rc; ++k) {
+        for (int k = 0; k < (stage == 0 ? 4 : 3) && !rc; ++k) {
             Tune t = best;
             if (stage == 0) t.item_max = item_opts[k];
             if (stage == 1) t.g2p_wg = wg_opts[k];
