@@ -191,7 +191,7 @@ def pmc_traffic(workload, seed_args, outdir):
     dispatch) of a 3-step run, each scaled by what the same counter reads
     for a plain 1 GiB device copy."""
     run = ["--workload", workload, "--steps", "3", "--warmup", "1", "--no-cpu-baseline", "--batch-scenes", "0",
-           "--no-traffic"] + seed_args
+           "--serve-seconds", "0", "--no-small", "--no-traffic"] + seed_args
     copy = ["--calibrate-copy", "1024"]
     fe, wr = _pmc_pass("FETCH_SIZE", run, outdir, "step"), _pmc_pass("WRITE_SIZE", run, outdir, "step")
     cfe, cwr = _pmc_pass("FETCH_SIZE", copy, outdir, "copy"), _pmc_pass("WRITE_SIZE", copy, outdir, "copy")
