@@ -68,8 +68,10 @@ static unsigned resident_grid(F* kernel, int block, int requested) {
 // spill 8-10 registers under that cap), and the other APIC variants fit the
 // 168 of 3 waves.  Measured on the 8.4M-particle slab: G2P 0.833 ms (64
 // threads, 3 waves) -> 0.821 ms (128 threads, 4 waves).  PIC-FLIP G2P keeps
-// the compiler's allocation unless MPM_PIC_G2P_WAVES says otherwise (linear
-// 127-128 VGPRs, quadratic 133-134, cubic 156-179).
+// the compiler's allocation for the cubic stencil and 256 threads
+// (MPM_PIC_G2P_WAVES; cubic 156-179 VGPRs) and is capped at 4 waves for the
+// linear and quadratic stencils at 64 and 128 threads, which fit 128 VGPRs
+// without spilling.
 #ifndef MPM_APIC_G2P_WAVES
 #define MPM_APIC_G2P_WAVES 3
 #endif
@@ -79,11 +81,14 @@ static unsigned resident_grid(F* kernel, int block, int requested) {
 #ifndef MPM_PIC_G2P_WAVES
 #define MPM_PIC_G2P_WAVES 1
 #endif
+#ifndef MPM_PIC_G2P_WAVES_LQ
+#define MPM_PIC_G2P_WAVES_LQ 4
+#endif
 template <int INTERP, int APIC, int WG>
 struct G2PWaves {
     static constexpr int value =
         APIC ? ((WG == 128 && INTERP != MPM_INTERP_CUBIC_BSPLINE) ? MPM_APIC_G2P_WAVES_WG128 : MPM_APIC_G2P_WAVES)
-             : MPM_PIC_G2P_WAVES;
+             : ((WG <= 128 && INTERP != MPM_INTERP_CUBIC_BSPLINE) ? MPM_PIC_G2P_WAVES_LQ : MPM_PIC_G2P_WAVES);
 };
 
 // Minimum waves per SIMD for P2G.  The folded (4-value) and mass-only
