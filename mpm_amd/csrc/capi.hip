@@ -65,11 +65,14 @@ hipEvent_t take_event(Solver& s) {
     if (hipEventCreate(&e) != hipSuccess) return nullptr;
     return e;
 }
+// Level 2 times the item kernels and the store reorder only.
+bool prof_wants(const Solver& s, const char* name) {
+    return s.prof != 2 || std::strcmp(name, "p2g") == 0 || std::strcmp(name, "g2p") == 0 ||
+           std::strcmp(name, "reorder") == 0;
+}
 void prof_begin(void* ctx, const char* name) {
     Solver& s = *static_cast<Solver*>(ctx);
-    if (s.prof == 2 && std::strcmp(name, "p2g") != 0 && std::strcmp(name, "g2p") != 0 &&
-        std::strcmp(name, "reorder") != 0)
-        return;
+    if (!prof_wants(s, name)) return;
     Pending p;
     p.name = name;
     p.a = take_event(s);
@@ -80,6 +83,7 @@ void prof_begin(void* ctx, const char* name) {
 }
 void prof_end(void* ctx, const char* name) {
     Solver& s = *static_cast<Solver*>(ctx);
+    if (!prof_wants(s, name)) return;  // never close an older pair of a skipped name
     for (size_t i = s.pending.size(); i-- > 0;)
         if (s.pending[i].name == name) {
             (void)hipEventRecord(s.pending[i].b, s.st);

@@ -127,3 +127,14 @@ def test_profile_levels():
     s.profile_reset()
     s.step(1)
     assert s.profile_read() == {}
+    # switching levels without a reset: the skipped kernels' end events must
+    # not re-close the pairs the previous level opened
+    s.profile(1)
+    s.step(1)
+    s.profile(2)
+    s.step(3)
+    mixed = s.profile_read()
+    s.profile(False)
+    one = full["sort_hist"][1] / full["sort_hist"][0]
+    assert mixed["g2p"][0] == 4 and mixed["sort_hist"][0] >= 1  # the level-1 step's passes only
+    assert mixed["sort_hist"][1] / mixed["sort_hist"][0] < 20 * one + 0.05, (mixed["sort_hist"], one)
