@@ -83,6 +83,15 @@ def test_slab_apic_quadratic_random_velocity():
     assert all(v <= TOL1 for v in errs.values()), errs
 
 
+def test_slab_large_items():
+    # 256k particles: one scene's work items hold up to 512 particles
+    # (several P2G chunks and G2P rounds per item); the smaller parity scenes
+    # run 64-particle items
+    sc = scenes.apic_slab(seed=6, nx=80, ny=40, nz=80, res=64)
+    errs = _one_step(sc, "slab_apic_quad_256k")
+    assert all(v <= TOL1 for v in errs.values()), errs
+
+
 def test_linear_pic_flip():
     sc = scenes.snow_cube(seed=5, n=3000, res=32, interp=N.LINEAR, velocity=(0.5, 0.2, -0.3))
     errs = _one_step(sc, "linear_pic")
