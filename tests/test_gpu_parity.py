@@ -6,6 +6,8 @@ Tolerances (FP32 throughout):
   * 100 steps: particle mass exact, max position difference <= 1e-3 * dx;
     the FP32-oracle vs FP64-oracle error is printed beside it.
 """
+import dataclasses
+
 import numpy as np
 import pytest
 
@@ -83,12 +85,14 @@ def test_slab_apic_quadratic_random_velocity():
     assert all(v <= TOL1 for v in errs.values()), errs
 
 
-def test_slab_large_items():
+@pytest.mark.parametrize("transfer", [N.APIC, N.PIC_FLIP])
+def test_slab_large_items(transfer):
     # 256k particles: one scene's work items hold up to 512 particles
     # (several P2G chunks and G2P rounds per item); the smaller parity scenes
     # run 64-particle items
     sc = scenes.apic_slab(seed=6, nx=80, ny=40, nz=80, res=64)
-    errs = _one_step(sc, "slab_apic_quad_256k")
+    sc = dataclasses.replace(sc, desc=dataclasses.replace(sc.desc, transfer=transfer))
+    errs = _one_step(sc, f"slab_256k_{'apic' if transfer == N.APIC else 'picflip'}")
     assert all(v <= TOL1 for v in errs.values()), errs
 
 
