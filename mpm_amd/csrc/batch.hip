@@ -264,13 +264,22 @@ int sync_members(Batch& b) {
     return MPM_OK;
 }
 
-void enqueue_step(Batch& b) {
+// reorder: permute every member's store by this step's own sort (eager
+// steps; reorder_scratch allocated), after which the step runs on the
+// identity permutation.
+void enqueue_step(Batch& b, bool reorder = false) {
     const int ns = (int)b.members.size();
     launch_clear_batch(b.tab, b.block_start, ns, b.total_blocks, b.fold ? GF_FOLDED : GF_PRIMARY, b.st);
     const int sel = radix_sort_pairs(b.kin, b.iota, b.ka, b.va, b.kb, b.vb, b.ntot, b.key_bits, b.hist, b.scan_temp,
                                      b.st, ProfileHook{});
     b.I.sorted_keys = sel ? b.kb : b.ka;
     b.I.perm = sel ? b.vb : b.va;
+    if (reorder) {
+        // flags (ntot words) holds each slot's scene until launch_items reuses it
+        launch_reorder_store(ParticlesDev{}, b.tab, ns, b.I.perm, b.ntot, b.reorder_scratch, b.apic != 0, b.st,
+                             b.flags);
+        b.I.perm = b.iota;  // slot i now holds the i-th particle in cell order
+    }
     launch_items(b.I.sorted_keys, b.ntot, b.flags, b.scan_temp, b.items, b.counters, b.st, (uint32_t)b.tune.item_max,
                  b.key_offsets, ns, b.item_scene);
     const SceneDev S0{};
@@ -323,17 +332,22 @@ void collect_times(Batch& b) {
 // Runs n steps eagerly (graph = false) or as replays of a captured step.
 // Sort the global keys, permute every member's store into its range of
 // the sorted order and make the sorted keys the next sort's input.
+int ensure_reorder_scratch(Batch& b) {
+    if (b.reorder_cap >= b.ntot) return MPM_OK;
+    (void)hipStreamSynchronize(b.st);
+    bfree(b, *(void**)&b.reorder_scratch);
+    b.reorder_cap = 0;
+    int rc = balloc(b, (void**)&b.reorder_scratch, (size_t)b.cap * 9 * 4);
+    if (rc) return rc;
+    b.reorder_cap = b.cap;
+    return MPM_OK;
+}
+
 int reorder_members(Batch& b) {
     b.last_reorder = b.steps;
     if (b.ntot <= 0) return MPM_OK;
-    if (b.reorder_cap < b.ntot) {
-        (void)hipStreamSynchronize(b.st);
-        bfree(b, *(void**)&b.reorder_scratch);
-        b.reorder_cap = 0;
-        int rc = balloc(b, (void**)&b.reorder_scratch, (size_t)b.cap * 9 * 4);
-        if (rc) return rc;
-        b.reorder_cap = b.cap;
-    }
+    int rc0 = ensure_reorder_scratch(b);
+    if (rc0) return rc0;
     const int sel = radix_sort_pairs(b.kin, b.iota, b.ka, b.va, b.kb, b.vb, b.ntot, b.key_bits, b.hist, b.scan_temp,
                                      b.st, ProfileHook{});
     const uint32_t* sorted = sel ? b.kb : b.ka;
@@ -373,13 +387,16 @@ int run_steps(Batch& b, int n, bool graph) {
             e1 = take_event(b);
             if (e0 && e1) HIP_TRY(hipEventRecord(e0, b.st));
         }
-        if (b.reorder_interval > 0 && b.steps + i - b.last_reorder >= b.reorder_interval) {
-            int rc2 = reorder_members(b);
+        const bool due = b.reorder_interval > 0 && b.steps + i - b.last_reorder >= b.reorder_interval;
+        if (due) {
+            // a replayed graph sorts by itself, so the reorder runs its own
+            // sort between replays; an eager step reorders on its own sort
+            int rc2 = graph ? reorder_members(b) : ensure_reorder_scratch(b);
             if (rc2) return rc2;
             b.last_reorder = b.steps + i;
         }
         if (graph) HIP_TRY(hipGraphLaunch(b.gexec, b.st));
-        else enqueue_step(b);
+        else enqueue_step(b, due);
         if (e0 && e1) {
             HIP_TRY(hipEventRecord(e1, b.st));
             b.pending.push_back({e0, e1});
