@@ -225,10 +225,10 @@ void launch_items(const uint32_t* sorted_keys, int64_t n, uint32_t* flags, uint3
                   int nscenes, uint32_t* item_scene) {
     if (n <= 0) return;
     if (item_max == 0) {
-        // a multiple of 64 up to kItemMax that still gives about two items
+        // a multiple of 64 up to kItemMax that still gives about four items
         // per compute unit: a small scene's few blocks are split so that
         // its P2G and G2P are not one item's serial chunks long
-        const int64_t per = (n + 2 * 64 * (int64_t)device_cus() - 1) / (2 * 64 * (int64_t)device_cus());
+        const int64_t per = (n + 4 * 64 * (int64_t)device_cus() - 1) / (4 * 64 * (int64_t)device_cus());
         item_max = (uint32_t)(64 * (per < 1 ? 1 : (per > kItemMax / 64 ? kItemMax / 64 : per)));
     }
     const int64_t tiles = (n + kScanTile - 1) / kScanTile;

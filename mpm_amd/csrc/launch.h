@@ -39,7 +39,7 @@ void launch_keys(const SceneDev& S, const ParticlesDev& P, int interp, uint32_t*
 void launch_iota(uint32_t* p, int64_t n, hipStream_t st);
 // Work items over sorted keys: items[0..n) receives the item starts and
 // items[n..2n) their key blocks (so items holds 2n words).  item_max = 0
-// picks a multiple of 64 up to kItemMax from n (about two items per compute
+// picks a multiple of 64 up to kItemMax from n (about four items per compute
 // unit for small n).  For a batch, key_offsets[nscenes] (ascending) gives
 // each item's scene in item_scene.
 void launch_items(const uint32_t* sorted_keys, int64_t n, uint32_t* flags, uint32_t* scan_temp, uint32_t* items,

@@ -87,12 +87,12 @@ def test_slab_apic_quadratic_random_velocity():
 
 @pytest.mark.parametrize("transfer", [N.APIC, N.PIC_FLIP])
 def test_slab_large_items(transfer):
-    # 256k particles: one scene's work items hold up to 512 particles
-    # (several P2G chunks and G2P rounds per item); the smaller parity scenes
-    # run 64-particle items
-    sc = scenes.apic_slab(seed=6, nx=80, ny=40, nz=80, res=64)
+    # 524k particles: one scene's work items hold up to 512 particles
+    # (8 P2G chunks and 4 G2P rounds per full item); the smaller parity
+    # scenes run 64-particle items
+    sc = scenes.apic_slab(seed=6, nx=128, ny=64, nz=64, res=64)
     sc = dataclasses.replace(sc, desc=dataclasses.replace(sc.desc, transfer=transfer))
-    errs = _one_step(sc, f"slab_256k_{'apic' if transfer == N.APIC else 'picflip'}")
+    errs = _one_step(sc, f"slab_524k_{'apic' if transfer == N.APIC else 'picflip'}")
     assert all(v <= TOL1 for v in errs.values()), errs
 
 
