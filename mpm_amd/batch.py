@@ -22,7 +22,7 @@ from .solver import Solver
 
 @dataclass
 class Tune:
-    item_max: int = 512
+    item_max: int = 256
     g2p_threads: int = 128
     workgroups: int = 0
     bucket: int = 0

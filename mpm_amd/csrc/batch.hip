@@ -34,7 +34,7 @@ namespace mpm {
 namespace host {
 
 struct Tune {
-    int item_max = kItemMax;
+    int item_max = 256;  // what the autotuner picks for the 256-scene uneven batch
     int g2p_wg = 128;  // as the single-scene G2P (4 waves per SIMD for linear/quadratic APIC)
     int wgs = 0;  // 0: one resident round per kernel (occupancy x CUs)
     bool operator==(const Tune& o) const { return item_max == o.item_max && g2p_wg == o.g2p_wg && wgs == o.wgs; }
