@@ -142,6 +142,10 @@ typedef struct mpm_stats {
      * max|v| * dt / h.  NaN if a velocity is NaN. */
     double max_speed;
     double cfl;
+    /* step graphs captured by mpm_step_graph since creation (a capture is
+     * dropped when the particle collision switch, the grid record mode or
+     * the material table changes) */
+    int64_t graph_captures;
 } mpm_stats;
 
 /* Per-kernel timing (hipEvent pairs on the handle's stream), enabled by
@@ -372,6 +376,22 @@ MPM_API int mpm_batch_timing(void* batch, int enable);
 /* Copies up to max recorded step times (ms) and clears the record;
  * with ms == NULL only *count is set. */
 MPM_API int mpm_batch_read_times(void* batch, float* ms, int max, int* count);
+
+/* Health of each member after its steps, in membership order (one pass
+ * over all members' particles): the largest particle speed |v| and
+ * |v| * dt / h (NaN when a velocity is NaN), the particles with a non-finite
+ * position or velocity, and the particles outside their grid (stencil off
+ * the grid: the escape key).  n must equal the member count.  No
+ * counterpart in the reference, which neither batches scenes nor checks
+ * its state (src/mpm_solver3d.cu:100-125); a request server uses it to
+ * evict a failing scene and keep the others running. */
+typedef struct mpm_scene_health {
+    double max_speed;
+    double cfl;
+    int64_t nonfinite;
+    int64_t escaped;
+} mpm_scene_health;
+MPM_API int mpm_batch_health(void* batch, mpm_scene_health* out, int n);
 
 /* ------------------------------------------------------------- test hooks */
 /* Device 3x3 SVD (the solver's own) on n row-major matrices:

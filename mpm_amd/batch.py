@@ -110,6 +110,17 @@ class Batch:
         N.check(self._lib.mpm_batch_autotune(self._h, int(trial_steps), C.byref(out)))
         return Tune.from_c(out)
 
+    def health(self) -> list[dict]:
+        """Per member (membership order): max_speed, cfl, nonfinite and
+        escaped particle counts of the current state (one device pass)."""
+        n = len(self.members)
+        arr = (N.MpmSceneHealth * max(n, 1))()
+        N.check(self._lib.mpm_batch_health(self._h, arr, n))
+        return [{f[0]: getattr(arr[i], f[0]) for f in N.MpmSceneHealth._fields_} for i in range(n)]
+
+    def device_bytes(self) -> int:
+        return int(self.info()["device_bytes"])
+
     def timing(self, enable: bool = True):
         N.check(self._lib.mpm_batch_timing(self._h, 1 if enable else 0))
 

@@ -19,6 +19,7 @@
 
 #include <algorithm>
 #include <chrono>
+#include <cmath>
 #include <cstring>
 #include <map>
 #include <mutex>
@@ -86,6 +87,9 @@ struct Batch {
     int64_t last_reorder = 0;
     uint32_t* reorder_scratch = nullptr;
     int64_t reorder_cap = 0;
+    // per-member health words (3 per member), allocated on first use
+    uint32_t* health = nullptr;
+    int health_cap = 0;
     // graph
     hipGraph_t graph = nullptr;
     hipGraphExec_t gexec = nullptr;
@@ -693,6 +697,38 @@ MPM_API int mpm_batch_read_times(void* batch, float* ms, int max, int* count) {
     if (!ms) return MPM_OK;  // count only
     for (int i = 0; i < std::min(k, max); ++i) ms[i] = b->step_ms[(size_t)i];
     b->step_ms.clear();
+    return MPM_OK;
+}
+
+MPM_API int mpm_batch_health(void* batch, mpm_scene_health* out, int n) {
+    GET_BATCH(b, batch);
+    if (!out && n > 0) return fail(MPM_ERR_INVALID, "out is NULL");
+    int rc = sync_members(*b);  // keys of members changed outside the batch
+    if (rc) return rc;
+    const int ns = (int)b->members.size();
+    if (n != ns) return fail(MPM_ERR_INVALID, "n must equal the number of members");
+    if (ns == 0) return MPM_OK;
+    if (b->health_cap < ns) {
+        HIP_TRY(hipStreamSynchronize(b->st));
+        bfree(*b, *(void**)&b->health);
+        b->health_cap = 0;
+        if ((rc = balloc(*b, (void**)&b->health, (size_t)b->tab_cap * 3 * 4))) return rc;
+        b->health_cap = b->tab_cap;
+    }
+    launch_fill_u32(b->health, 0u, (int64_t)ns * 3, b->st);
+    launch_batch_health(b->tab, b->key_offsets, ns, b->ntot, b->kin, b->health, b->st);
+    std::vector<uint32_t> h((size_t)ns * 3);
+    HIP_TRY(hipMemcpyAsync(h.data(), b->health, h.size() * 4, hipMemcpyDeviceToHost, b->st));
+    HIP_TRY(hipStreamSynchronize(b->st));
+    for (int i = 0; i < ns; ++i) {
+        const Solver& s = *b->members[(size_t)i];
+        float v2;
+        std::memcpy(&v2, &h[(size_t)i * 3], 4);
+        out[i].max_speed = std::sqrt((double)v2);
+        out[i].cfl = out[i].max_speed * (double)s.S.dt / (double)s.S.h;
+        out[i].nonfinite = h[(size_t)i * 3 + 1];
+        out[i].escaped = h[(size_t)i * 3 + 2];
+    }
     return MPM_OK;
 }
 

@@ -229,7 +229,15 @@ int dedupe_materials(Solver& s, int64_t n, const float* xi, const float* thc, co
     return MPM_OK;
 }
 
+// The captured step holds the particle descriptor (with the material table's
+// address) by value: a new table invalidates it.
+static void drop_graph(Solver& s) {
+    if (s.gexec) { (void)hipGraphExecDestroy(s.gexec); s.gexec = nullptr; }
+    if (s.graph) { (void)hipGraphDestroy(s.graph); s.graph = nullptr; }
+}
+
 int upload_materials(Solver& s) {
+    drop_graph(s);
     if (s.mats_dev) dfree(s, s.mats_dev);
     s.mats_dev = nullptr;
     size_t bytes = std::max<size_t>(1, s.mats.size()) * sizeof(MaterialDev);
@@ -681,6 +689,7 @@ MPM_API int mpm_step_graph(void* handle, int n) {
         (void)dirty;
         if (e != hipSuccess) return fail(MPM_ERR_HIP, std::string("graph capture: ") + hipGetErrorString(e));
         HIP_TRY(hipGraphInstantiate(&s->gexec, s->graph, nullptr, nullptr, 0));
+        s->graph_captures += 1;
     }
     auto t0 = std::chrono::steady_clock::now();
     for (int i = 0; i < n; ++i) {
@@ -1216,6 +1225,7 @@ MPM_API int mpm_get_stats(void* handle, mpm_stats* out) {
     std::memcpy(&v2, &bits, 4);
     out->max_speed = std::sqrt((double)v2);
     out->cfl = out->max_speed * (double)s->S.dt / (double)s->S.h;
+    out->graph_captures = s->graph_captures;
     return finish(*s);
 }
 

@@ -1655,6 +1655,73 @@ void launch_count_escaped(const uint32_t* sorted_keys, int64_t n, uint32_t escap
     hipLaunchKernelGGL(k_count_escaped, dim3((unsigned)w), dim3(256), 0, st, sorted_keys, n, escape_key, out);
 }
 
+// Health of every scene of a batch after its steps, one pass over the
+// global slots: per scene the largest |v|^2 (float bits; a NaN wins), the
+// particles with a non-finite position or velocity component, and the
+// particles whose next key is the scene's escape key (stencil outside the
+// grid).  keys: the global sort input the last G2P wrote (key offset +
+// cell key per slot), so a slot's scene is the last whose key offset is <=
+// its key.  out: 3 words per scene, zeroed by the caller.
+__global__ void __launch_bounds__(256) k_batch_health(const SceneEntry* __restrict__ tab,
+                                                      const uint32_t* __restrict__ key_offsets, int ns, int64_t ntot,
+                                                      const uint32_t* __restrict__ keys, uint32_t* __restrict__ out) {
+    const int lane = threadIdx.x & 63;
+    for (int64_t i0 = (int64_t)blockIdx.x * blockDim.x; i0 < ntot; i0 += (int64_t)gridDim.x * blockDim.x) {
+        const int64_t i = i0 + threadIdx.x;
+        const bool act = i < ntot;
+        int sc = 0;
+        uint32_t q = 0, bad = 0, esc = 0;
+        if (act) {
+            const uint32_t key = keys[i];
+            int lo = 0, hi = ns - 1;
+            while (lo < hi) {
+                const int mid = (lo + hi + 1) >> 1;
+                if (key_offsets[mid] <= key) lo = mid;
+                else hi = mid - 1;
+            }
+            sc = lo;
+            const int64_t p = i - tab[sc].pstart;
+            const int64_t N = tab[sc].P.n;
+            const float* x = tab[sc].P.x;
+            const float* v = tab[sc].P.v;
+            const float a = v[p], b = v[N + p], c = v[2 * N + p];
+            const float x0 = x[p], x1 = x[N + p], x2 = x[2 * N + p];
+            const float sq = (a * a + b * b) + c * c;
+            q = __float_as_uint(sq) & 0x7FFFFFFFu;  // |v|^2 >= 0; a NaN keeps its payload above +inf
+            bad = (isfinite(a) && isfinite(b) && isfinite(c) && isfinite(x0) && isfinite(x1) && isfinite(x2)) ? 0u : 1u;
+            esc = key - key_offsets[sc] == tab[sc].S.escape_key ? 1u : 0u;
+        }
+        // a wave of one scene reduces first; scene boundaries add per lane
+        const int s0 = __shfl(sc, 0, 64);
+        if (__ballot(act && sc != s0) == 0ull) {
+            uint32_t m = q, nb = bad, ne = esc;
+#pragma unroll
+            for (int off = 32; off > 0; off >>= 1) {
+                const uint32_t o = __shfl_xor(m, off, 64);
+                m = o > m ? o : m;
+                nb += __shfl_xor(nb, off, 64);
+                ne += __shfl_xor(ne, off, 64);
+            }
+            if (lane == 0) {
+                atomicMax(&out[3 * s0], m);
+                if (nb) atomicAdd(&out[3 * s0 + 1], nb);
+                if (ne) atomicAdd(&out[3 * s0 + 2], ne);
+            }
+        } else if (act) {
+            atomicMax(&out[3 * sc], q);
+            if (bad) atomicAdd(&out[3 * sc + 1], bad);
+            if (esc) atomicAdd(&out[3 * sc + 2], esc);
+        }
+    }
+}
+void launch_batch_health(const SceneEntry* tab, const uint32_t* key_offsets, int ns, int64_t ntot,
+                         const uint32_t* keys, uint32_t* out, hipStream_t st) {
+    if (ntot <= 0 || ns <= 0) return;
+    int64_t w = (ntot + 255) / 256;
+    if (w > 2048) w = 2048;
+    hipLaunchKernelGGL(k_batch_health, dim3((unsigned)w), dim3(256), 0, st, tab, key_offsets, ns, ntot, keys, out);
+}
+
 __global__ void k_debug_svd(const float* A, int64_t n, float* U, float* s, float* V) {
     int64_t i = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
     if (i >= n) return;

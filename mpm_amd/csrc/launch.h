@@ -106,6 +106,10 @@ void launch_density_blocks(const SceneDev& D, const uint32_t* touched, const uin
                            int32_t* origin, float* values, int64_t max_blocks, hipStream_t st);
 // Largest |v|^2 of the store into *out (float bits; *out must start at 0).
 void launch_max_speed2(const ParticlesDev& P, unsigned int* out, hipStream_t st);
+// Per-scene health of a batch (3 words per scene, zeroed by the caller):
+// largest |v|^2 as float bits, non-finite particles, escaped particles.
+void launch_batch_health(const SceneEntry* tab, const uint32_t* key_offsets, int ns, int64_t ntot,
+                         const uint32_t* keys, uint32_t* out, hipStream_t st);
 void launch_count_escaped(const uint32_t* sorted_keys, int64_t n, uint32_t escape_key, unsigned long long* out,
                           hipStream_t st);
 void launch_debug_svd(const float* A, int64_t n, float* U, float* s, float* V, hipStream_t st);

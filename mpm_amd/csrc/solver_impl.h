@@ -87,6 +87,7 @@ struct Solver {
     // graph
     hipGraph_t graph = nullptr;
     hipGraphExec_t gexec = nullptr;
+    int64_t graph_captures = 0;
     // batch membership: while in a batch the solver works on the batch's
     // stream (own_st keeps its own)
     struct Batch* batch = nullptr;

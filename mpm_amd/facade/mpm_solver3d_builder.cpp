@@ -6,31 +6,94 @@
 // directory.
 #include "mpm_solver3d_builder.h"
 
+#include <cerrno>
+#include <cmath>
 #include <cstdint>
+#include <cstdlib>
 #include <cstring>
 #include <fstream>
 #include <iostream>
-#include <sstream>
+#include <iterator>
 
 namespace chains {
 
-bool readXYZFile(const std::string& filePath, std::vector<Vector3f>& points) {
-    std::ifstream in(filePath);
-    if (!in.is_open()) {
-        std::cerr << "[ERROR] cannot open file " << filePath << std::endl;
-        return false;
+namespace {
+
+bool is_blank(char c) { return c == ' ' || c == '\t' || c == '\r' || c == '\v' || c == '\f'; }
+bool is_digit(char c) { return c >= '0' && c <= '9'; }
+
+// One decimal number at s (blanks skipped first, never past the end of the
+// line): [+-] digits [. digits] [e [+-] digits], or [+-] . digits ...  This
+// is the form a stream extraction of a float reads (no inf, nan or hex).
+// Returns the position after it, or nullptr when no number starts there.
+const char* scan_decimal(const char* s, const char* eol, float& out) {
+    while (s < eol && is_blank(*s)) ++s;
+    const char* p = s;
+    if (p < eol && (*p == '+' || *p == '-')) ++p;
+    const char* mant = p;
+    while (p < eol && is_digit(*p)) ++p;
+    bool digits = p > mant;
+    if (p < eol && *p == '.') {
+        const char* frac = ++p;
+        while (p < eol && is_digit(*p)) ++p;
+        digits = digits || p > frac;
     }
-    std::string line;
-    while (std::getline(in, line)) {
-        std::stringstream ss(line);
+    if (!digits) return nullptr;
+    if (p < eol && (*p == 'e' || *p == 'E')) {
+        const char* q = p + 1;
+        if (q < eol && (*q == '+' || *q == '-')) ++q;
+        const char* ex = q;
+        while (q < eol && is_digit(*q)) ++q;
+        if (q > ex) p = q;
+    }
+    char buf[128];
+    const size_t len = (size_t)(p - s);
+    if (len >= sizeof(buf)) return nullptr;
+    std::memcpy(buf, s, len);
+    buf[len] = '\0';
+    errno = 0;
+    const float v = std::strtof(buf, nullptr);
+    if (errno == ERANGE && std::isinf(v)) return nullptr;  // out of float range
+    out = v;
+    return p;
+}
+
+}  // namespace
+
+TextGroupStatus readTextGroup(const std::string& filePath, std::vector<Vector3f>& points, std::string* badLine) {
+    std::ifstream in(filePath, std::ios::binary);
+    if (!in.is_open()) return TextGroupStatus::kUnreadable;
+    const std::string text((std::istreambuf_iterator<char>(in)), std::istreambuf_iterator<char>());
+    const char* s = text.data();
+    const char* const end = s + text.size();
+    while (s < end) {
+        const char* eol = static_cast<const char*>(std::memchr(s, '\n', (size_t)(end - s)));
+        if (!eol) eol = end;
         Vector3f p;
-        if (!(ss >> p.x() >> p.y() >> p.z())) {
-            std::cerr << "[ERROR] cannot parse line: " << line << " when trying to parse file " << filePath << std::endl;
-            return false;
+        const char* c = s;
+        for (int a = 0; a < 3 && c; ++a) c = scan_decimal(c, eol, p(a));
+        if (!c) {
+            if (badLine) badLine->assign(s, eol);
+            return TextGroupStatus::kStopped;
         }
-        points.push_back(p);
+        points.push_back(p);  // text after the third number is ignored
+        s = eol + 1;
     }
-    return true;
+    return TextGroupStatus::kComplete;
+}
+
+bool readXYZFile(const std::string& filePath, std::vector<Vector3f>& points) {
+    std::string bad;
+    switch (readTextGroup(filePath, points, &bad)) {
+        case TextGroupStatus::kComplete: return true;
+        case TextGroupStatus::kUnreadable:
+            std::cerr << "[ERROR] cannot open file " << filePath << std::endl;
+            return false;
+        default:
+            std::cerr << "[ERROR] cannot parse line: " << bad << " when trying to parse file " << filePath
+                      << std::endl;
+            return false;
+    }
 }
 
 bool readXYZBFile(const std::string& filePath, ParticleGroupData& out) {
@@ -86,7 +149,7 @@ bool readParticleGroup(const std::string& filePath, ParticleGroupData& out) {
     if (filePath.size() >= ext.size() && filePath.compare(filePath.size() - ext.size(), ext.size(), ext) == 0)
         return readXYZBFile(filePath, out);
     out = ParticleGroupData();
-    return readXYZFile(filePath, out.position);
+    return readXYZFile(filePath, out.position);  // on false, out keeps the lines read before the stop
 }
 
 template <class P>
@@ -95,9 +158,11 @@ static std::vector<P> build_points(const MPM3DConfiguration& config, int& partic
     const size_t groups = config.group_paths.size();
     std::cout << "[INFO] building " << groups << " group(s) of particle on host" << std::endl;
     for (size_t i = 0; i < groups; ++i) {
+        // As in the reference (whose builder ignores readXYZFile's result),
+        // an unreadable file gives an empty group and a text file that
+        // stops parsing gives the points read before the bad line.
         ParticleGroupData g;
-        // as in the reference, an unreadable group is reported and skipped
-        if (!readParticleGroup(config.group_paths[i], g)) g = ParticleGroupData();
+        (void)readParticleGroup(config.group_paths[i], g);
         const size_t n = g.position.size();
         particles.reserve(particles.size() + n);
         for (size_t k = 0; k < n; ++k)
@@ -132,13 +197,20 @@ APICMPMSolver3D buildAPICMPMSolver3DFromYAML(MPM3DConfiguration config, int& par
 // Reads one particle group as the builders do (positions scaled by stride
 // unless the file holds world units): for the loader tests.  Writes up to
 // max particles; *count gets the group size and *fields the per-particle
-// fields present (1 velocity, 2 young, 4 mass).  0 on success, -1 if the
-// file cannot be read.
+// fields present (1 velocity, 2 young, 4 mass).  0 when the whole file was
+// read, 1 when a text file stopped at a line that does not parse (the group
+// is the lines before it, as the builders keep them), -1 if the file cannot
+// be read (an empty group).
 extern "C" __attribute__((visibility("default"))) int chains_read_group(const char* path, float stride, float* pos,
                                                                         float* vel, float* young, float* mass,
                                                                         int64_t max, int64_t* count, int* fields) {
     chains::ParticleGroupData g;
-    if (!path || !chains::readParticleGroup(path, g)) return -1;
+    if (!path) return -1;
+    const bool ok = chains::readParticleGroup(path, g);
+    if (!ok && g.position.empty()) {
+        if (count) *count = 0;
+        return -1;
+    }
     const int64_t n = (int64_t)g.position.size();
     if (count) *count = n;
     if (fields) *fields = (g.velocity.empty() ? 0 : 1) | (g.young.empty() ? 0 : 2) | (g.mass.empty() ? 0 : 4);
@@ -151,5 +223,5 @@ extern "C" __attribute__((visibility("default"))) int chains_read_group(const ch
         if (young && !g.young.empty()) young[i] = g.young[(size_t)i];
         if (mass && !g.mass.empty()) mass[i] = g.mass[(size_t)i];
     }
-    return 0;
+    return ok ? 0 : 1;
 }

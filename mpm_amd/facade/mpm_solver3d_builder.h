@@ -13,9 +13,17 @@
 
 namespace chains {
 
-// Reference src/mpm_solver3d_builder.cu:10-30: whitespace-separated x y z
-// per line; false (and a message on stderr) if the file cannot be opened or
-// a line does not parse.
+// Text particle group: one point "x y z" per line (blanks between and
+// before the numbers, anything after the third ignored).  Reading stops at
+// the first line without three numbers -- a blank line included -- and the
+// points read before it stay in `points` (badLine gets that line).
+enum class TextGroupStatus { kComplete, kStopped, kUnreadable };
+TextGroupStatus readTextGroup(const std::string& filePath, std::vector<Vector3f>& points,
+                              std::string* badLine = nullptr);
+
+// Reference src/mpm_solver3d_builder.cu:10-30: appends the file's points;
+// false (and the reference's message on stderr) if the file cannot be
+// opened or reading stopped at a bad line, whose prefix is kept.
 bool readXYZFile(const std::string& filePath, std::vector<Vector3f>& points);
 
 // Binary particle group (no reference counterpart: the text reader above is

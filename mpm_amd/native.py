@@ -104,6 +104,7 @@ class MpmStats(C.Structure):
         ("last_step_ms", C.c_double),
         ("max_speed", C.c_double),
         ("cfl", C.c_double),
+        ("graph_captures", C.c_int64),
     ]
 
 
@@ -133,6 +134,10 @@ class MpmBatchStats(C.Structure):
         ("last_step_ms", C.c_double),
         ("tune", MpmTune),
     ]
+
+
+class MpmSceneHealth(C.Structure):
+    _fields_ = [("max_speed", C.c_double), ("cfl", C.c_double), ("nonfinite", C.c_int64), ("escaped", C.c_int64)]
 
 
 # Every exported entry point and its signature (also used by the CPU test
@@ -195,6 +200,7 @@ SIGNATURES = {
     "mpm_tune_lookup": (C.c_int, [C.c_int32, C.c_int32, C.c_int64, C.POINTER(MpmTune)]),
     "mpm_batch_timing": (C.c_int, [C.c_void_p, C.c_int]),
     "mpm_batch_read_times": (C.c_int, [C.c_void_p, _fp, C.c_int, C.POINTER(C.c_int)]),
+    "mpm_batch_health": (C.c_int, [C.c_void_p, C.POINTER(MpmSceneHealth), C.c_int]),
 }
 
 _lib = None

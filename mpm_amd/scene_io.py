@@ -25,6 +25,7 @@ from __future__ import annotations
 import argparse
 import json
 import os
+import re
 import sys
 
 import numpy as np
@@ -89,13 +90,48 @@ def read_xyzb(path: str) -> dict:
     return out
 
 
+# A decimal number as a stream extraction of a float reads it (blanks before
+# it; no inf, nan or hex): the C++ builder's scanner uses the same form.
+_DECIMAL = re.compile(r"[ \t\r\v\f]*([+-]?(?:[0-9]+\.?[0-9]*|\.[0-9]+)(?:[eE][+-]?[0-9]+)?)")
+
+
+def read_text_group(path: str):
+    """(points float32 [n, 3], complete) of a text group: one "x y z" per
+    line, anything after the third number ignored.  Reading stops at the
+    first line without three numbers (a blank line included); the points
+    before it are kept and complete is False (the reference's builder keeps
+    them too, src/mpm_solver3d_builder.cu:10-30, :40)."""
+    with open(path, "rb") as f:
+        text = f.read().decode("latin-1")
+    lines = text.split("\n")
+    if text.endswith("\n") or not text:
+        lines = lines[:-1]
+    pts = []
+    for line in lines:
+        pos, xyz = 0, []
+        for _ in range(3):
+            m = _DECIMAL.match(line, pos)
+            if not m:
+                break
+            v = np.float32(m.group(1))
+            if np.isinf(v):
+                break  # out of float range: the extraction fails
+            xyz.append(v)
+            pos = m.end()
+        if len(xyz) < 3:
+            return np.array(pts, np.float32).reshape(-1, 3), False
+        pts.append(xyz)
+    return np.array(pts, np.float32).reshape(-1, 3), True
+
+
 def read_group(path: str, stride: float) -> dict:
     """A group as the builders read it: positions in world units (stride *
-    coordinates for `.xyz` and cell-unit `.xyzb` files, in float32)."""
+    coordinates for `.xyz` and cell-unit `.xyzb` files, in float32).  A text
+    group that stops at a bad line holds the points before it."""
     if path.endswith(".xyzb"):
         g = read_xyzb(path)
     else:
-        g = {"position": np.loadtxt(path, dtype=np.float32, ndmin=2).reshape(-1, 3), "world": False}
+        g = {"position": read_text_group(path)[0], "world": False}
     if not g.pop("world"):
         g["position"] = (np.float32(stride) * g["position"].astype(np.float32)).astype(np.float32)
     return g

@@ -11,6 +11,19 @@ retires the members that are done: their state is read back, their memory
 freed and their budget returned.  Scenes stay on the device from admission
 to completion; requests that arrive while others run join at the next tick.
 
+Failures stay with the request that caused them.  After every tick the
+batch's health pass (one launch) flags members with a non-finite position
+or velocity, particles outside their grid, or a CFL number above the limit;
+such a member is evicted with an error record and the others continue.  A
+library error in a batch's step evicts that batch's members and the other
+batches continue; a failed build rejects its request.  Whatever ends the
+loop, every scene and batch is freed.
+
+Admission counts device memory as measured where it exists: the handles'
+own bytes after their build, the batches' buffers (which grow by 1.5x and
+are freed when a batch empties), and the estimate of scenes built but not
+yet joined.
+
 Reported: per-step latency (device time of each batch step, hipEvents, and
 host wall time per step of a tick), per-request latency (finish - arrival)
 and queueing delay as p50/p99, and aggregate particle-updates/s.
@@ -34,18 +47,25 @@ _STAGE_FLOATS_APIC = 43  # x3 v3 B9 Fe9 Fp9 m V0 mu0 lam0 tau6
 _STAGE_FLOATS_PIC = 34
 
 
+# Per-particle words of a batch's buffers: sort keys and values in and out
+# (6), flags and item scenes (2), item records (2), store-reorder scratch (9).
+_BATCH_WORDS = 19
+_BATCH_GROWTH = 1.5  # the batch's buffers grow by this factor and never shrink while it has members
+
+
 def estimate_bytes(desc: SolverDesc, n: int) -> int:
     """Peak device bytes of one scene: particle store (twice, for the build's
-    physical reorder), its sort buffers and the batch's per-particle share,
-    the store-reorder scratch (9 words per particle, the scene's and its
-    batch's), the 10-plane block-major grid and its touched flags."""
+    physical reorder), its sort buffers, its store-reorder scratch (9 words
+    per particle), the 10-plane block-major grid with its touched flags, and
+    its share of the batch's buffers with their growth slack."""
     f = _STAGE_FLOATS_APIC if desc.transfer == N.APIC else _STAGE_FLOATS_PIC
-    per_particle = 2 * (4 * f + 6) + 8 * 4 + 9 * 4 + 4 + 2 * 9 * 4
+    own = 2 * (4 * f + 6) + 8 * 4 + 9 * 4 + 4
+    share = _BATCH_GROWTH * 4 * _BATCH_WORDS
     blocks = 1
     for r in desc.resolution:
         blocks *= (int(r) + 3) // 4
     grid = blocks * 64 * 10 * 4 + blocks * 4
-    return int(n) * per_particle + grid + (1 << 20)
+    return int(int(n) * (own + share) + grid + (1 << 20))
 
 
 def key_span(desc: SolverDesc) -> int:
@@ -80,6 +100,8 @@ class RequestRecord:
     finished: float = -1.0
     cfl: float = 0.0  # max|v| dt / h of the final state (CFL monitor)
     result: object = None
+    error: str = ""  # why the request was evicted or rejected ("" when it completed)
+    steps_done: int = 0
 
 
 @dataclass
@@ -89,6 +111,7 @@ class _Active:
     rec: RequestRecord
     key: tuple
     span: int = 0
+    bytes: int = 0  # the handle's own device bytes, measured after its build
 
 
 class DeviceBackend:
@@ -127,6 +150,7 @@ class ServeReport:
     peak_bytes: int
     peak_scenes: int
     max_cfl: float = 0.0  # largest final-state CFL number over the requests
+    failed: int = 0  # requests evicted or rejected with an error
     records: list = field(default_factory=list, repr=False)
 
     def summary(self) -> dict:
@@ -135,7 +159,8 @@ class ServeReport:
 
 class Server:
     def __init__(self, budget_bytes: float = 200e9, tick_steps: int = 16, max_scenes: int | None = None,
-                 backend=None, clock=time.perf_counter, keep_results: bool = False, sleep=time.sleep):
+                 backend=None, clock=time.perf_counter, keep_results: bool = False, sleep=time.sleep,
+                 cfl_limit: float = 1.0, evict_escaped: bool = True):
         self.budget = int(budget_bytes)
         self.tick_steps = int(tick_steps)
         self.max_scenes = max_scenes
@@ -143,56 +168,141 @@ class Server:
         self.clock = clock
         self.sleep = sleep
         self.keep_results = keep_results
-        self.used = 0
+        self.cfl_limit = float(cfl_limit)
+        self.evict_escaped = bool(evict_escaped)
         self.batches: dict[tuple, object] = {}
+        self.batch_bytes: dict[tuple, int] = {}
         self.active: dict[int, _Active] = {}
+        self.staged: list[_Active] = []
         self.records: list[RequestRecord] = []
         self.step_ms: list[float] = []
         self.tick_step_ms: list[float] = []
         self.peak_bytes = 0
         self.peak_scenes = 0
 
+    # -- device memory
+    @property
+    def used(self) -> int:
+        """Device bytes committed: measured bytes of the joined handles and
+        of the batches, estimates of the scenes built but not yet joined."""
+        return (sum(a.bytes for a in self.active.values()) + sum(self.batch_bytes.values()) +
+                sum(a.rec.est_bytes for a in self.staged))
+
+    def _note_peak(self):
+        self.peak_bytes = max(self.peak_bytes, self.used)
+
+    def _measure_batch(self, key):
+        b = self.batches.get(key)
+        fn = getattr(b, "device_bytes", None)
+        if b is not None and fn is not None:
+            self.batch_bytes[key] = int(fn())
+
     # -- admission
-    def _fits(self, req: ServeRequest, rec: RequestRecord, staged: list) -> bool:
-        if self.max_scenes is not None and len(self.active) + len(staged) >= self.max_scenes:
+    def _fits(self, req: ServeRequest, rec: RequestRecord) -> bool:
+        if self.max_scenes is not None and len(self.active) + len(self.staged) >= self.max_scenes:
             return False
         key = (req.desc.interp, req.desc.transfer)
-        keys = sum(a.span for a in list(self.active.values()) + staged if a.key == key)
+        keys = sum(a.span for a in list(self.active.values()) + self.staged if a.key == key)
         if keys + key_span(req.desc) > (1 << 32):
             return False
         return self.used + rec.est_bytes <= self.budget
 
-    def _build(self, req: ServeRequest, rec: RequestRecord, t0: float) -> _Active:
+    def _build(self, req: ServeRequest, rec: RequestRecord, t0: float):
         """Creates the scene on the device (its own stream): safe while
-        batch steps are in flight."""
-        s = self.backend.create_solver(req.desc, req.particles)
-        self.used += rec.est_bytes
+        batch steps are in flight.  A library error rejects the request."""
+        try:
+            s = self.backend.create_solver(req.desc, req.particles)
+        except N.MpmError as e:
+            rec.error = f"build failed: {e}"
+            rec.admitted = rec.finished = self.clock() - t0
+            return
         rec.admitted = self.clock() - t0
-        self.peak_bytes = max(self.peak_bytes, self.used)
-        return _Active(s, int(req.steps), rec, (req.desc.interp, req.desc.transfer), key_span(req.desc))
+        a = _Active(s, int(req.steps), rec, (req.desc.interp, req.desc.transfer), key_span(req.desc))
+        stats = getattr(s, "stats", None)
+        # measured where the backend reports it; else the estimate stands
+        a.bytes = int(stats()["device_bytes"]) if stats is not None else rec.est_bytes
+        self.staged.append(a)
+        self._note_peak()
 
-    def _join(self, a: _Active):
+    def _join(self, a: _Active, t0: float):
         """Adds a built scene to its batch (between ticks)."""
         b = self.batches.get(a.key)
-        if b is None:
-            b = self.backend.create_batch(*a.key)
-            self.batches[a.key] = b
-        b.add(a.solver)
+        try:
+            if b is None:
+                b = self.backend.create_batch(*a.key)
+                self.batches[a.key] = b
+                self.batch_bytes[a.key] = 0
+            b.add(a.solver)
+        except N.MpmError as e:
+            self._drop(a, t0, f"join failed: {e}", joined=False)
+            return
         self.active[a.rec.rid] = a
+        self._measure_batch(a.key)
         self.peak_scenes = max(self.peak_scenes, len(self.active))
+        self._note_peak()
 
-    def _retire(self, a: _Active, t0: float):
-        b = self.batches[a.key]
-        b.remove(a.solver)
-        if self.keep_results:
-            a.rec.result = a.solver.read_particles()
-        stats = getattr(a.solver, "stats", None)
-        if stats is not None:
-            a.rec.cfl = float(stats()["cfl"])
-        a.solver.close()
-        self.used -= a.rec.est_bytes
+    def _drop(self, a: _Active, t0: float, error: str = "", joined: bool = True):
+        """Takes a scene out (finished or failed): result and final CFL read
+        if it finished, handle freed, budget returned."""
+        b = self.batches.get(a.key)
+        if joined and b is not None:
+            try:
+                b.remove(a.solver)
+            except N.MpmError as e:
+                error = error or f"remove failed: {e}"
+        if not error:
+            try:
+                if self.keep_results:
+                    a.rec.result = a.solver.read_particles()
+                stats = getattr(a.solver, "stats", None)
+                if stats is not None:
+                    a.rec.cfl = float(stats()["cfl"])
+            except N.MpmError as e:
+                error = f"readback failed: {e}"
+        a.rec.error = error
+        try:
+            a.solver.close()
+        except N.MpmError:
+            pass
         a.rec.finished = self.clock() - t0
-        del self.active[a.rec.rid]
+        self.active.pop(a.rec.rid, None)
+        if b is not None and not any(x.key == a.key for x in self.active.values()):
+            self._close_batch(a.key)  # an empty batch gives its buffers back
+        else:
+            self._measure_batch(a.key)
+
+    def _close_batch(self, key):
+        b = self.batches.pop(key, None)
+        self.batch_bytes.pop(key, None)
+        if b is not None:
+            try:
+                b.close()
+            except N.MpmError:
+                pass
+
+    def _health(self, b, members, t0):
+        """Evicts members whose state is non-finite, left the grid or runs
+        above the CFL limit."""
+        fn = getattr(b, "health", None)
+        if fn is None:
+            return
+        order = {id(s): k for k, s in enumerate(getattr(b, "members", []))}
+        h = fn()
+        for a in members:
+            k = order.get(id(a.solver))
+            if k is None:
+                continue
+            r = h[k]
+            why = ""
+            if r["nonfinite"] > 0:
+                why = f"{r['nonfinite']} particles with a non-finite position or velocity"
+            elif self.evict_escaped and r["escaped"] > 0:
+                why = f"{r['escaped']} particles outside the grid"
+            elif not (r["cfl"] <= self.cfl_limit):
+                why = f"CFL number {r['cfl']:.3g} above the limit {self.cfl_limit:g}"
+            if why:
+                a.rec.cfl = float(r["cfl"])
+                self._drop(a, t0, f"evicted after {a.rec.steps_done} steps: {why}")
 
     # -- main loop
     def run(self, requests) -> ServeReport:
@@ -202,9 +312,15 @@ class Server:
                 raise ValueError(f"request {r.rid} alone exceeds the HBM budget")
             if key_span(r.desc) > (1 << 32):
                 raise ValueError(f"request {r.rid}: grid too large for 32-bit cell keys")
+        try:
+            return self._run(reqs)
+        except BaseException:
+            self.close()  # nothing stays allocated when the loop ends by an exception
+            raise
+
+    def _run(self, reqs) -> ServeReport:
         pending = deque(reqs)
         queue: deque = deque()
-        recs = {}
         t0 = self.clock()
 
         def arrivals():
@@ -213,57 +329,72 @@ class Server:
                 r = pending.popleft()
                 rec = RequestRecord(r.rid, r.particles.count, int(r.steps), r.desc.interp, r.desc.transfer, r.arrival,
                                     estimate_bytes(r.desc, r.particles.count))
-                recs[r.rid] = rec
                 self.records.append(rec)
                 queue.append((r, rec))
 
-        staged: list[_Active] = []
-
         def admissions():
-            while queue and self._fits(queue[0][0], queue[0][1], staged):
+            while queue and self._fits(queue[0][0], queue[0][1]):
                 r, rec = queue.popleft()
-                staged.append(self._build(r, rec, t0))
+                self._build(r, rec, t0)
 
         def joins():
+            staged, self.staged = self.staged, []
             for a in staged:
-                self._join(a)
-            staged.clear()
+                self._join(a, t0)
 
-        while pending or queue or staged or self.active:
+        while pending or queue or self.staged or self.active:
             arrivals()
             admissions()
             joins()
             if not self.active:
-                if pending:
+                if pending and not queue and not self.staged:
                     self.sleep(max(0.0, min(0.05, pending[0].arrival - (self.clock() - t0))))
                 continue
             # one tick: enqueue k steps on every batch, admit meanwhile, then retire
             ticks = []
             tick_start = self.clock()
-            for key, b in self.batches.items():
+            for key, b in list(self.batches.items()):
                 members = [a for a in self.active.values() if a.key == key]
                 if not members:
                     continue
                 k = min(self.tick_steps, min(a.remaining for a in members))
-                b.step(k)
-                ticks.append((b, members, k))
+                try:
+                    b.step(k)
+                except N.MpmError as e:
+                    for a in members:
+                        self._drop(a, t0, f"batch step failed: {e}")
+                    continue
+                ticks.append((key, b, members, k))
             arrivals()
             admissions()  # builds overlap the enqueued steps; they join after the tick
-            for b, members, k in ticks:
-                b.synchronize()
+            done = []
+            for key, b, members, k in ticks:
+                try:
+                    b.synchronize()
+                except N.MpmError as e:
+                    for a in members:
+                        self._drop(a, t0, f"batch step failed: {e}")
+                    continue
                 for a in members:
                     a.remaining -= k
+                    a.rec.steps_done += k
+                done.append((key, b, members, k))
             tick_wall = (self.clock() - tick_start) * 1e3
-            kmax = max(k for _, _, k in ticks)
+            kmax = max((k for *_, k in ticks), default=0)
             if kmax > 0:
                 self.tick_step_ms.extend([tick_wall / kmax] * kmax)
-            for b, _, _ in ticks:
-                if hasattr(b, "read_times"):
-                    self.step_ms.extend(float(x) for x in b.read_times())
+            for key, b, members, k in done:
+                try:
+                    if hasattr(b, "read_times"):
+                        self.step_ms.extend(float(x) for x in b.read_times())
+                    self._health(b, [a for a in members if a.rec.rid in self.active], t0)
+                except N.MpmError as e:
+                    for a in [a for a in members if a.rec.rid in self.active]:
+                        self._drop(a, t0, f"health check failed: {e}")
             for a in [a for a in self.active.values() if a.remaining <= 0]:
-                self._retire(a, t0)
+                self._drop(a, t0)
         wall = self.clock() - t0
-        done = [r for r in self.records if r.finished >= 0]
+        done = [r for r in self.records if r.finished >= 0 and not r.error]
         pu = sum(r.particles * r.steps for r in done)
         return ServeReport(
             requests=len(self.records), completed=len(done), wall_s=wall, particle_updates=pu,
@@ -275,15 +406,19 @@ class Server:
             queue_s_p50=_pct([r.admitted - r.arrival for r in done], 50),
             queue_s_p99=_pct([r.admitted - r.arrival for r in done], 99),
             peak_bytes=self.peak_bytes, peak_scenes=self.peak_scenes,
-            max_cfl=max((r.cfl for r in done), default=0.0), records=self.records)
+            max_cfl=max((r.cfl for r in done), default=0.0),
+            failed=sum(1 for r in self.records if r.error), records=self.records)
 
     def close(self):
-        for a in list(self.active.values()):
-            a.solver.close()
+        for a in list(self.active.values()) + self.staged:
+            try:
+                a.solver.close()
+            except N.MpmError:
+                pass
         self.active.clear()
-        for b in self.batches.values():
-            b.close()
-        self.batches.clear()
+        self.staged = []
+        for key in list(self.batches):
+            self._close_batch(key)
 
 
 def requests_from_scenes(stream) -> list[ServeRequest]:

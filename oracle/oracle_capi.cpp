@@ -305,6 +305,35 @@ __attribute__((visibility("default"))) void orc_stress(const double* Fe, const d
     if (precision == 64) run(double(0)); else run(float(0));
 }
 
+// updateDeformationGradient of n particles (src/material_point3d.cu:53-70):
+// Fe, Fp (9n each, row-major, updated in place) under grad (9n) and dt,
+// with critical compression crit_c and stretch crit_s.
+__attribute__((visibility("default"))) void orc_update_F(double* Fe, double* Fp, const double* grad, int64_t n,
+                                                         double dt, double crit_c, double crit_s, int precision) {
+    auto run = [&](auto tag) {
+        using R = decltype(tag);
+        for (int64_t i = 0; i < n; ++i) {
+            Particle<R> p;
+            M3<R> g;
+            for (int r = 0; r < 3; ++r)
+                for (int c = 0; c < 3; ++c) {
+                    p.Fe[r][c] = R(Fe[9 * i + 3 * r + c]);
+                    p.Fp[r][c] = R(Fp[9 * i + 3 * r + c]);
+                    g[r][c] = R(grad[9 * i + 3 * r + c]);
+                }
+            p.crit_c = R(crit_c);
+            p.crit_s = R(crit_s);
+            update_deformation_gradient(p, g, R(dt));
+            for (int r = 0; r < 3; ++r)
+                for (int c = 0; c < 3; ++c) {
+                    Fe[9 * i + 3 * r + c] = double(p.Fe[r][c]);
+                    Fp[9 * i + 3 * r + c] = double(p.Fp[r][c]);
+                }
+        }
+    };
+    if (precision == 64) run(double(0)); else run(float(0));
+}
+
 // Boundary collision of one velocity (FP32 restatement).
 __attribute__((visibility("default"))) void orc_collide(const float* pos, const float* vel, const float* g000,
                                                         const float* g111, float mu, float* out) {

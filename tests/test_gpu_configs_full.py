@@ -1,0 +1,181 @@
+"""The benchmark workloads at their own sizes, checked on the device.
+
+* two colliding snow balls: 128^3 grid, ~140k particles at 8 per cell, cubic
+  B-splines, APIC, dt 5e-5 -- one step against the FP32 oracle (1e-5);
+* the uneven batch: 256 scenes of 1-4 primitives, 2k-200k particles each
+  (10.9M in all), 32^3/64^3/128^3 grids, APIC quadratic -- one batched step
+  against every member stepped alone (1e-5) and against the oracle for the
+  largest 128^3 member and the two smallest members, then 19 more steps
+  (mass exact, positions within 1e-3 dx, Fe within 1e-3);
+* the served request stream: the first 8 requests of the 20 req/s Poisson
+  stream (10k-300k particles, 64^3/128^3/256^3 grids, both transfers,
+  300-2000 steps) served together, each result against the same request
+  alone (mass exact, positions within 1e-3 dx, Fe within 1e-3); plus the
+  smallest APIC and PIC-FLIP requests cut to 2 steps and served beside a
+  large one, against the oracle (1e-5).
+"""
+from __future__ import annotations
+
+import dataclasses
+import time
+
+import numpy as np
+import pytest
+
+from mpm_amd import native as N
+from mpm_amd import scenes
+from mpm_amd.batch import Batch
+from mpm_amd.serve import Server, requests_from_scenes
+from mpm_amd.solver import Solver
+from oracle import pyoracle
+from tests.helpers import grid_errors, particle_errors, particle_errors_apic, rel_err, report
+
+pytestmark = pytest.mark.gpu
+
+TOL1 = 1e-5
+
+
+def _many_step(pm, pa, dx):
+    return {
+        "pos/dx": float(np.abs(pm.position.astype(np.float64) - pa.position).max() / dx),
+        "Fe": rel_err(pm.Fe, pa.Fe),
+        "mass": float(np.abs(pm.mass.astype(np.float64) - pa.mass).max()),
+    }
+
+
+def test_two_balls_full_size_one_step():
+    sc = scenes.two_balls()  # 128^3, cubic APIC, ppc 8, dt 5e-5
+    n = sc.particles.count
+    print(f"[two balls] N={n} res={sc.desc.resolution} interp={sc.desc.interp} transfer={sc.desc.transfer}",
+          flush=True)
+    assert 135_000 < n < 145_000 and sc.desc.resolution == (128, 128, 128)
+    gpu = Solver(sc.desc, sc.particles)
+    rec = Solver(sc.desc, sc.particles)
+    rec.set_grid_record(True)
+    t = time.perf_counter()
+    ref = pyoracle.OracleSolver(sc.desc, sc.particles, 32)
+    e = {"volume0": rel_err(gpu.read_particles().volume0, ref.read_particles().volume0)}
+    gpu.step(1)
+    rec.step(1)
+    ref.step(1)
+    print(f"[two balls] oracle initialize + step {time.perf_counter() - t:.1f} s", flush=True)
+    rp = ref.read_particles()
+    e.update(particle_errors_apic(gpu.read_particles(), rp, sc.desc.stride))
+    e.update({"rec_" + k: v for k, v in particle_errors(rec.read_particles(), rp).items()})
+    e.update({"grid_" + k: v for k, v in grid_errors(rec.read_grid(), ref.read_grid(),
+                                                    ("mass", "velocity", "force")).items()})
+    report(f"two balls full size N={n}", e)
+    assert all(v <= TOL1 for v in e.values()), e
+    gpu.close()
+    rec.close()
+
+
+def test_uneven_batch_256_scenes_full_size():
+    t = time.perf_counter()
+    scs = scenes.uneven_batch(seed=3, count=256)
+    ns = np.array([s.particles.count for s in scs])
+    res = np.array([s.desc.resolution[0] for s in scs])
+    print(f"[batch256] {len(scs)} scenes, {ns.sum()} particles ({ns.min()}-{ns.max()}), grids "
+          f"{dict(zip(*np.unique(res, return_counts=True)))}, generated in {time.perf_counter() - t:.1f} s",
+          flush=True)
+    assert len(scs) == 256 and ns.sum() > 10_000_000 and set(res) == {32, 64, 128}
+    members = [Solver(s.desc, s.particles) for s in scs]
+    alone = [Solver(s.desc, s.particles) for s in scs]
+    b = Batch(N.QUADRATIC_BSPLINE, N.APIC)
+    for m in members:
+        b.add(m)
+    info = b.info()
+    print(f"[batch256] batch: {info['scenes']} scenes, key bits {info['key_bits']}", flush=True)
+    assert info["scenes"] == 256 and info["particles"] == int(ns.sum())
+    b.step(1)
+    for a in alone:
+        a.step(1)
+    b.synchronize()
+    worst = {}
+    for k, (m, a) in enumerate(zip(members, alone)):
+        e = particle_errors_apic(m.read_particles(), a.read_particles(), scs[k].desc.stride)
+        for f, v in e.items():
+            worst[f] = max(worst.get(f, 0.0), v)
+        assert all(v <= TOL1 for v in e.values()), (k, ns[k], e)
+    report("[batch256] 1 step, worst member vs alone", worst)
+    # the oracle for the largest member (128^3) and the two smallest
+    for k in [int(np.argmax(ns))] + [int(i) for i in np.argsort(ns)[:2]]:
+        o = pyoracle.OracleSolver(scs[k].desc, scs[k].particles, 32)
+        o.step(1)
+        e = particle_errors_apic(members[k].read_particles(), o.read_particles(), scs[k].desc.stride)
+        report(f"[batch256] member {k} N={ns[k]} res={res[k]} vs oracle", e)
+        assert all(v <= TOL1 for v in e.values()), (k, e)
+    # 19 more steps (one store reorder of the batch at step 16)
+    b.step(19)
+    for a in alone:
+        a.step(19)
+    b.synchronize()
+    worst = {}
+    for k, (m, a) in enumerate(zip(members, alone)):
+        pm = m.read_particles()
+        e = _many_step(pm, a.read_particles(), scs[k].desc.stride)
+        assert np.all(np.isfinite(pm.position))
+        for f, v in e.items():
+            worst[f] = max(worst.get(f, 0.0), v)
+        assert e["mass"] == 0.0 and e["pos/dx"] <= 1e-3 and e["Fe"] <= 1e-3, (k, e)
+    report("[batch256] 20 steps, worst member vs alone", worst)
+    assert all(m.stats()["steps"] == 20 for m in members)
+    b.close()
+    for s in members + alone:
+        s.close()
+
+
+def test_served_stream_config_sizes():
+    stream = scenes.request_stream(seed=4, rate=20.0, duration=600.0, max_requests=8)
+    reqs = requests_from_scenes(stream)
+    desc = [(r.rid, r.particles.count, r.desc.resolution[0], "APIC" if r.desc.transfer == N.APIC else "PIC",
+             r.steps) for r in reqs]
+    print("[served] requests (id, N, res, transfer, steps):", desc, flush=True)
+    assert len(reqs) == 8 and {d[2] for d in desc} == {64, 128, 256} and {d[3] for d in desc} == {"APIC", "PIC"}
+    srv = Server(tick_steps=16, keep_results=True)
+    try:
+        rep = srv.run(reqs)
+    finally:
+        srv.close()
+    print("[served] report:", rep.summary(), flush=True)
+    assert rep.completed == len(reqs)
+    recs = {r.rid: r for r in rep.records}
+    for r in reqs:
+        alone = Solver(r.desc, r.particles)
+        alone.step(r.steps)
+        pa = alone.read_particles()
+        alone.close()
+        e = _many_step(recs[r.rid].result, pa, r.desc.stride)
+        report(f"[served] rid={r.rid} N={r.particles.count} res={r.desc.resolution[0]} steps={r.steps} vs alone", e)
+        assert e["mass"] == 0.0 and e["pos/dx"] <= 1e-3 and e["Fe"] <= 1e-3, e
+
+
+def test_served_requests_match_oracle():
+    """The smallest APIC and the smallest PIC-FLIP request of the stream,
+    cut to 2 steps, served beside the largest request (which keeps running):
+    both results match the FP32 oracle at the one-step bar."""
+    stream = scenes.request_stream(seed=4, rate=20.0, duration=600.0, max_requests=8)
+    reqs = requests_from_scenes(stream)
+    apic = min((r for r in reqs if r.desc.transfer == N.APIC), key=lambda r: r.particles.count)
+    pic = min((r for r in reqs if r.desc.transfer == N.PIC_FLIP), key=lambda r: r.particles.count)
+    big = max(reqs, key=lambda r: r.particles.count)
+    mix = [dataclasses.replace(big, rid=0, arrival=0.0, steps=64),
+           dataclasses.replace(apic, rid=1, arrival=0.0, steps=2),
+           dataclasses.replace(pic, rid=2, arrival=0.0, steps=2)]
+    print("[served/oracle] mix (N, res, transfer, steps):",
+          [(r.particles.count, r.desc.resolution[0], r.desc.transfer, r.steps) for r in mix], flush=True)
+    srv = Server(tick_steps=2, keep_results=True)
+    try:
+        rep = srv.run(mix)
+    finally:
+        srv.close()
+    recs = {r.rid: r for r in rep.records}
+    for r in mix[1:]:
+        o = pyoracle.OracleSolver(r.desc, r.particles, 32)
+        o.step(r.steps)
+        got, want = recs[r.rid].result, o.read_particles()
+        e = (particle_errors_apic(got, want, r.desc.stride) if r.desc.transfer == N.APIC
+             else particle_errors(got, want))
+        report(f"[served/oracle] N={r.particles.count} {'APIC' if r.desc.transfer == N.APIC else 'PIC-FLIP'} "
+               f"{r.steps} steps", e)
+        assert all(v <= TOL1 for v in e.values()), e

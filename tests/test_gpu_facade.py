@@ -114,7 +114,6 @@ def test_headless_runs_generated_binary_scene(tmp_path, transfer):
     # and Young's modulus); Transfer.Scheme selects the solver class
     from mpm_amd import scenes
     from mpm_amd.scene_io import write_scene
-    from mpm_amd.solver import Solver
 
     sc = scenes.snow_cube(seed=41, n=20000, res=48, transfer=transfer)
     rng = np.random.default_rng(4)
@@ -128,11 +127,13 @@ def test_headless_runs_generated_binary_scene(tmp_path, transfer):
     print(r.stdout, r.stderr, flush=True)
     assert r.returncode == 0
     pos = np.fromfile(out, dtype=np.float32).reshape(-1, 4)
-    s = Solver(sc.desc, sc.particles)
-    s.step(3)
-    ref = s.read_positions4()
-    e = {"position": rel_err(pos, ref), "moved": float(np.abs(ref[:, :3] - sc.particles.position).max())}
-    report(f"headless generated scene {'APIC' if transfer else 'PIC-FLIP'}", e)
+    assert pos.shape[0] == sc.particles.count and np.all(pos[:, 3] == 1.0)
+    # the FP32 oracle from the same inputs (what the binary group holds)
+    o = pyoracle.OracleSolver(sc.desc, sc.particles, 32)
+    o.step(3)
+    ref = o.read_particles().position
+    e = {"position": rel_err(pos[:, :3], ref), "moved": float(np.abs(ref - sc.particles.position).max())}
+    report(f"headless generated scene {'APIC' if transfer else 'PIC-FLIP'} vs oracle", e)
     assert e["moved"] > 0.0 and e["position"] <= 1e-5
 
 

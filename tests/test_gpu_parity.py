@@ -272,3 +272,75 @@ def test_apic_folded_grid_and_record_mode():
     assert all(v <= TOL1 for v in e.values()), e
     st = a.stats()
     assert st["steps"] == 5 and st["active_nodes"] > 0
+
+
+def test_graph_rebuilt_after_new_materials():
+    """A state write that adds materials replaces the device material table;
+    the captured step must be re-captured, not replayed with the old table's
+    address.  Half the particles get a new hardening and critical stretch
+    (new table entries past the old table's end) between graph replays."""
+    sc = scenes.snow_cube(seed=12, n=4000, res=32, transfer=N.APIC, velocity=(0.4, -0.6, 0.2))
+    gpu = Solver(sc.desc, sc.particles)
+    ref = pyoracle.OracleSolver(sc.desc, sc.particles, 32)
+    gpu.step_graph(1)
+    gpu.synchronize()
+    ref.step(1)
+    assert gpu.stats()["graph_captures"] == 1
+    st = gpu.read_particles()
+    n = st.count
+    st.hardening = st.hardening.copy()
+    st.critical_stretch = st.critical_stretch.copy()
+    st.critical_compression = st.critical_compression.copy()
+    st.hardening[: n // 2] = 3.0
+    st.critical_stretch[: n // 2] = 1.0e-4
+    st.critical_compression[n // 2:] = 1.0e-3
+    # a deformed Fe so that the new thresholds clamp at the next step
+    rng = scenes.Rng(5)
+    st.Fe = (np.eye(3)[None] + rng.uniform(9 * n, -0.01, 0.01).reshape(n, 3, 3)).astype(np.float32)
+    gpu.write_particles(st)
+    ref = pyoracle.OracleSolver(sc.desc, st, 32, run_initialize=False)
+    gpu.step_graph(2)
+    gpu.synchronize()
+    ref.step(2)
+    assert gpu.stats()["graph_captures"] == 2, "graph kept across a new material table"
+    e = particle_errors(gpu.read_particles(), ref.read_particles(), ("position", "velocity", "Fe", "Fp"))
+    report("graph after new materials", e)
+    assert all(v <= TOL1 for v in e.values()), e
+
+
+@pytest.mark.parametrize("alpha", [0.0, 0.95, 1.0])
+def test_pic_flip_blend_closed_form(alpha):
+    """Two interleaved regular lattices, zero stiffness: one device step gives
+    the closed-form blend alpha (u - g dt y) + (1 - alpha) (v - g dt y) for
+    every interior particle (the oracle's pin of the same scene)."""
+    from tests.test_oracle_pins import blend_expectation, blend_scene
+    g, dt = 9.8, 1e-4
+    d, p, kind, inner = blend_scene(alpha, g, dt)
+    v, eA, eB = blend_expectation(alpha, g, dt)
+    gpu = Solver(d, p)
+    gpu.step(1)
+    vel = gpu.read_particles().velocity.astype(np.float64)
+    a, b = inner & (kind == 0), inner & (kind == 1)
+    e = {"A": rel_err(vel[a], np.tile(eA, (a.sum(), 1))), "B": rel_err(vel[b], np.tile(eB, (b.sum(), 1)))}
+    report(f"blend {alpha} closed form", e)
+    assert all(x <= TOL1 for x in e.values()), e
+    gpu.close()
+
+
+@pytest.mark.parametrize("transfer", [N.APIC, N.PIC_FLIP])
+def test_light_node_branch_closed_form(transfer):
+    """One light particle: nodes with m w <= FLT_EPSILON keep v* = 0 in the
+    fused grid update, so the new velocity is the heavy nodes' weight sum
+    times v_p - g dt y."""
+    from tests.test_oracle_pins import FLT_EPS, light_node_scene
+    d, p, vp, W, base = light_node_scene(transfer)
+    g, dt = 9.8, 1e-4
+    heavy = np.float32(1e-6) * W.astype(np.float32) > FLT_EPS
+    expect = W[heavy].sum() * (vp - [0, g * dt, 0])
+    gpu = Solver(d, p)
+    gpu.step(1)
+    v = gpu.read_particles().velocity[0].astype(np.float64)
+    e = rel_err(v, expect)
+    report(f"light node {'apic' if transfer == N.APIC else 'picflip'}", {"v": e})
+    assert e <= TOL1, (v, expect)
+    gpu.close()

@@ -54,3 +54,52 @@ def test_served_results_equal_alone():
         assert e["mass"] == 0.0
         assert e["pos/dx"] <= 1e-3 and e["Fe"] <= 1e-3, e
     assert rep.step_ms_p50 > 0 and rep.step_ms_p99 >= rep.step_ms_p50
+
+
+def test_served_nan_scene_is_evicted_others_equal_alone():
+    """One request's state gets a NaN velocity (written through
+    mpm_write_particles after its build): the batch health pass evicts it
+    with an error record, and the requests sharing its batch still equal the
+    same requests run alone."""
+    from mpm_amd.serve import DeviceBackend
+    stream = scenes.request_stream(seed=42, rate=40.0, duration=0.5, n_lo=2000, n_hi=6000, steps_lo=24,
+                                   steps_hi=40, max_requests=6)
+    reqs = requests_from_scenes(stream)
+    # poison a request whose batch has company
+    tr = [r.desc.transfer for r in reqs]
+    bad = next(r for r in reqs if tr.count(r.desc.transfer) >= 2)
+    print("requests:", [(r.rid, r.particles.count, r.steps, r.desc.transfer) for r in reqs], "poisoned:", bad.rid,
+          flush=True)
+
+    class Poison(DeviceBackend):
+        def create_solver(self, desc, particles):
+            s = super().create_solver(desc, particles)
+            if particles is bad.particles:
+                st = s.read_particles()
+                st.velocity[7, 1] = np.nan
+                s.write_particles(st)
+            return s
+
+    srv = Server(tick_steps=8, keep_results=True, backend=Poison())
+    try:
+        rep = srv.run(reqs)
+    finally:
+        srv.close()
+    print("serve report:", rep.summary(), flush=True)
+    recs = {r.rid: r for r in rep.records}
+    print("poisoned record:", recs[bad.rid].error, flush=True)
+    assert "non-finite" in recs[bad.rid].error and recs[bad.rid].result is None
+    assert rep.failed == 1 and rep.completed == len(reqs) - 1
+    for r in reqs:
+        if r.rid == bad.rid:
+            continue
+        alone = Solver(r.desc, r.particles)
+        alone.step(r.steps)
+        pa = alone.read_particles()
+        alone.close()
+        pm = recs[r.rid].result
+        dx = r.desc.stride
+        e = {"pos/dx": float(np.abs(pm.position.astype(np.float64) - pa.position).max() / dx),
+             "Fe": rel_err(pm.Fe, pa.Fe)}
+        report(f"beside a NaN scene: rid={r.rid} N={r.particles.count}", e)
+        assert np.all(np.isfinite(pm.position)) and e["pos/dx"] <= 1e-3 and e["Fe"] <= 1e-3, e

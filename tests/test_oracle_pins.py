@@ -393,3 +393,244 @@ def test_threaded_oracle_matches_serial(transfer):
     assert all(v <= 1e-5 for v in e.values()), e
     with pytest.raises(ValueError):
         O.OracleSolver(sc.desc, serial.read_particles(), 32, threads=2)
+
+
+# ------------------------------------------------ deformation-gradient update
+# updateDeformationGradient (reference src/material_point3d.cu:53-70): trial
+# Fe* = (I + dt grad v) Fe, singular values of Fe* clamped to
+# [1 - theta_c, 1 + theta_s], and Fp absorbs the rest so that the product
+# Fe Fp follows the trial.  Pinned by what any correct reading must satisfy:
+# the product invariant, the clamp range with its two distinct bounds, the
+# no-op inside the range and the shared singular vectors.
+TH_C, TH_S = 2.5e-2, 7.5e-3
+
+
+def _rand_rot(rng, n):
+    q = rng.normal(size=(n, 4))
+    q /= np.linalg.norm(q, axis=1, keepdims=True)
+    w, x, y, z = q.T
+    return np.stack([
+        np.stack([1 - 2 * (y * y + z * z), 2 * (x * y - w * z), 2 * (x * z + w * y)], -1),
+        np.stack([2 * (x * y + w * z), 1 - 2 * (x * x + z * z), 2 * (y * z - w * x)], -1),
+        np.stack([2 * (x * z - w * y), 2 * (y * z + w * x), 1 - 2 * (x * x + y * y)], -1)], 1)
+
+
+def _with_sv(rng, sv):
+    """R1 diag(sv) R2 for random rotations (one per row of sv)."""
+    sv = np.asarray(sv, np.float64)
+    n = sv.shape[0]
+    return _rand_rot(rng, n) @ (sv[:, :, None] * _rand_rot(rng, n))
+
+
+def test_update_F_product_invariant_and_clamp_range():
+    rng = np.random.default_rng(11)
+    n = 400
+    Fe = _with_sv(rng, rng.uniform(0.95, 1.03, (n, 3)))
+    Fp = np.eye(3) + rng.uniform(-0.05, 0.05, (n, 3, 3))
+    grad = rng.uniform(-40.0, 40.0, (n, 3, 3))
+    dt = 2.5e-4
+    trial = (np.eye(3) + dt * grad) @ Fe
+    Fe1, Fp1 = O.update_F(Fe, Fp, grad, dt, TH_C, TH_S, 64)
+    # Fe Fp follows the trial exactly (real arithmetic; FP64 rounding here)
+    prod = Fe1 @ Fp1
+    want = trial @ Fp
+    assert np.abs(prod - want).max() <= 1e-12 * np.abs(want).max()
+    # clamped singular values inside [1 - theta_c, 1 + theta_s]
+    s1 = np.linalg.svd(Fe1, compute_uv=False)
+    assert s1.max() <= 1 + TH_S + 1e-12 and s1.min() >= 1 - TH_C - 1e-12
+    # Fe keeps the trial's singular vectors: U^T Fe V is diagonal with the
+    # clamped trial singular values (numpy's SVD of the trial)
+    U, s, Vt = np.linalg.svd(trial)
+    D = np.swapaxes(U, 1, 2) @ Fe1 @ np.swapaxes(Vt, 1, 2)
+    off = D - D * np.eye(3)
+    assert np.abs(off).max() <= 1e-12
+    np.testing.assert_allclose(np.diagonal(D, axis1=1, axis2=2), np.clip(s, 1 - TH_C, 1 + TH_S), atol=1e-12)
+    # the FP32 restatement follows the FP64 one
+    Fe32, Fp32 = O.update_F(Fe, Fp, grad, dt, TH_C, TH_S, 32)
+    assert np.abs(Fe32 - Fe1).max() < 2e-6 and np.abs(Fp32 - Fp1).max() < 2e-6
+
+
+def test_update_F_inside_range_is_noop():
+    rng = np.random.default_rng(12)
+    n = 200
+    # trials strictly inside the range (grad = 0: the trial is Fe itself)
+    Fe = _with_sv(rng, rng.uniform(1 - TH_C + 1e-3, 1 + TH_S - 1e-3, (n, 3)))
+    Fp = np.eye(3) + rng.uniform(-0.05, 0.05, (n, 3, 3))
+    Fe1, Fp1 = O.update_F(Fe, Fp, np.zeros((n, 3, 3)), 1e-4, TH_C, TH_S, 64)
+    assert np.abs(Fe1 - Fe).max() <= 1e-13
+    assert np.abs(Fp1 - Fp).max() <= 1e-13
+
+
+@pytest.mark.parametrize("sv,expect", [
+    ((1.05, 1.0, 0.99), (1 + TH_S, 1.0, 0.99)),          # stretched past 1 + theta_s
+    ((1.0, 0.95, 0.90), (1.0, 1 - TH_C, 1 - TH_C)),      # compressed past 1 - theta_c
+    ((1.2, 1.0, 0.5), (1 + TH_S, 1.0, 1 - TH_C)),        # both bounds at once
+])
+def test_update_F_clamps_to_the_named_bound(sv, expect):
+    """The critical stretch bounds from above and the critical compression
+    from below: swapping them (or clamping to the wrong side) moves the
+    end singular values off these exact numbers."""
+    rng = np.random.default_rng(13)
+    Fe = _with_sv(rng, np.tile(sv, (50, 1)))
+    Fp = np.tile(np.eye(3), (50, 1, 1))
+    Fe1, Fp1 = O.update_F(Fe, Fp, np.zeros((50, 3, 3)), 1e-4, TH_C, TH_S, 64)
+    s1 = np.linalg.svd(Fe1, compute_uv=False)
+    np.testing.assert_allclose(s1, np.tile(expect, (50, 1)), atol=1e-12)
+    # and the plastic part carries the excess: sv(Fp) = sv(trial) / sv(Fe)
+    sp = np.sort(np.linalg.svd(Fp1, compute_uv=False), axis=1)
+    np.testing.assert_allclose(sp, np.sort(np.tile(np.array(sv) / np.array(expect), (50, 1)), axis=1), atol=1e-12)
+
+
+# ------------------------------------------------------- PIC-FLIP blend
+def _two_lattices(res, lo, hi, oA, oB, mA, mB, uA, uB):
+    """One particle of kind A and one of kind B per cell of [lo, hi)^3, at
+    the fixed cell offsets oA, oB; zero Young's modulus (no stress)."""
+    from mpm_amd.solver import ParticleInput
+    h = 1.0 / res
+    c = np.arange(lo, hi)
+    cells = np.stack(np.meshgrid(c, c, c, indexing="ij"), -1).reshape(-1, 3).astype(np.float64)
+    pos = np.concatenate([(cells + oA) * h, (cells + oB) * h])
+    n = cells.shape[0]
+    vel = np.concatenate([np.tile(uA, (n, 1)), np.tile(uB, (n, 1))])
+    mass = np.concatenate([np.full(n, mA), np.full(n, mB)])
+    kind = np.concatenate([np.zeros(n, int), np.ones(n, int)])
+    p = ParticleInput(pos, vel, mass, 10.0, 0.0, 0.2, TH_C, TH_S, 0)
+    return p, kind, cells
+
+
+# h = 1/32 and offsets in eighths of a cell: every position is exact in FP32,
+# so both lattices are exactly regular
+BLEND_SCENE = dict(res=32, lo=6, hi=22, oA=np.array([0.375, 0.625, 0.25]), oB=np.array([0.75, 0.125, 0.5]),
+                   mA=2e-3, mB=1e-3, uA=np.array([1.0, 0.0, 0.2]), uB=np.array([-1.0, 0.5, 0.4]))
+
+
+def blend_expectation(alpha, g, dt):
+    """Closed form of one PIC-FLIP step of the two-lattice scene for the
+    particles whose whole stencil lies on interior nodes.  Every interior
+    node holds mass mA + mB and momentum mA uA + mB uB (each lattice's
+    weights sum to 1 at every interior node), so v = (mA uA + mB uB) / (mA +
+    mB) and v* = v - g dt y; G2P then gives v_pic = v* and v_flip = u - g dt y
+    (partition of unity over the stencil), blended as
+    alpha v_flip + (1 - alpha) v_pic."""
+    s = BLEND_SCENE
+    alpha = float(np.float32(alpha))  # the blend coefficient is a float (include/mpm_solver3d.h:31)
+    v = (s["mA"] * s["uA"] + s["mB"] * s["uB"]) / (s["mA"] + s["mB"])
+    gy = np.array([0.0, g * dt, 0.0])
+    eA = alpha * (s["uA"] - gy) + (1 - alpha) * (v - gy)
+    eB = alpha * (s["uB"] - gy) + (1 - alpha) * (v - gy)
+    return v, eA, eB
+
+
+def blend_scene(alpha, g=9.8, dt=1e-4):
+    from mpm_amd.solver import SolverDesc
+    s = BLEND_SCENE
+    p, kind, cells = _two_lattices(s["res"], s["lo"], s["hi"], s["oA"], s["oB"], s["mA"], s["mB"], s["uA"],
+                                   s["uB"])
+    d = SolverDesc(resolution=(s["res"],) * 3, stride=1.0 / s["res"], transfer=N.PIC_FLIP, blend=alpha,
+                   interp=N.QUADRATIC_BSPLINE, dt=dt, gravity=g)
+    cc = np.concatenate([cells, cells])
+    inner = np.all((cc >= s["lo"] + 3) & (cc < s["hi"] - 3), axis=1)
+    return d, p, kind, inner
+
+
+@pytest.mark.parametrize("alpha", [0.0, 0.95, 1.0])
+def test_pic_flip_blend_two_lattices(alpha):
+    g, dt = 9.8, 1e-4
+    d, p, kind, inner = blend_scene(alpha, g, dt)
+    v, eA, eB = blend_expectation(alpha, g, dt)
+    for prec, tol in ((64, 1e-12), (32, 2e-6)):
+        o = O.OracleSolver(d, p, prec)
+        # the grid the closed form describes
+        o.substep("reset")
+        o.substep("p2g")
+        o.substep("gravity")
+        o.substep("update", dt)
+        o.substep("collision", dt)
+        gr = o.read_grid()
+        res, s = d.resolution[0], BLEND_SCENE
+        idx = np.arange(res)
+        X, Y, Z = np.meshgrid(idx, idx, idx, indexing="ij")
+        nid = (X + res * (Y + res * Z)).ravel()
+        core = ((X >= s["lo"] + 2) & (X < s["hi"] - 2) & (Y >= s["lo"] + 2) & (Y < s["hi"] - 2) &
+                (Z >= s["lo"] + 2) & (Z < s["hi"] - 2)).ravel()
+        np.testing.assert_allclose(gr["mass"][nid[core]], s["mA"] + s["mB"], rtol=max(tol, 1e-7))
+        np.testing.assert_allclose(gr["velocity"][nid[core]], np.tile(v, (core.sum(), 1)), atol=max(tol, 1e-6))
+        np.testing.assert_allclose(gr["velocity_star"][nid[core]], np.tile(v - [0, g * dt, 0], (core.sum(), 1)),
+                                   atol=max(tol, 1e-6))
+        # one full step from the same start
+        o = O.OracleSolver(d, p, prec)
+        o.step(1)
+        vel = o.read_particles64()["velocity"] if prec == 64 else o.read_particles().velocity.astype(np.float64)
+        a = inner & (kind == 0)
+        b = inner & (kind == 1)
+        assert a.sum() > 100 and b.sum() > 100
+        np.testing.assert_allclose(vel[a], np.tile(eA, (a.sum(), 1)), atol=tol)
+        np.testing.assert_allclose(vel[b], np.tile(eB, (b.sum(), 1)), atol=tol)
+
+
+# ------------------------------------------------------ light-node branch
+# CollocatedGridData3D::updateVelocity (reference src/grid3d.cu:36-41) and
+# computeGravityForces (src/mpm_solver3d.cu:394-403) act only on nodes with
+# mass > FLT_EPSILON: a lighter node keeps its raw momentum in "velocity",
+# its velocity_star stays 0 and it gets no gravity.
+def light_node_scene(transfer):
+    from mpm_amd.solver import ParticleInput, SolverDesc
+    res = 32
+    h = 1.0 / res
+    x = (np.array([12, 15, 17]) + 0.25) * h   # x/h - i = 0.25 on every axis
+    vp = np.array([0.375, -0.25, 0.125])  # exact in FP32
+    p = ParticleInput(x[None], vp[None], 1e-6, 10.0, 1.4e5, 0.2, TH_C, TH_S, 0)
+    d = SolverDesc(resolution=(res,) * 3, stride=h, transfer=transfer, blend=0.0, interp=N.QUADRATIC_BSPLINE,
+                   dt=1e-4, gravity=9.8)
+    # quadratic weights at distances 1.25, 0.25, 0.75 (closed forms):
+    # N(1.25) = (1.5 - 1.25)^2 / 2 = 1/32, N(0.25) = 3/4 - 1/16, N(0.75) = (1.5 - 0.75)^2 / 2
+    w1 = np.array([1 / 32, 0.75 - 0.0625, 0.5 * 0.75 ** 2])
+    W = w1[:, None, None] * w1[None, :, None] * w1[None, None, :]
+    base = np.array([11, 14, 16])
+    return d, p, vp, W, base
+
+
+def test_light_node_keeps_momentum_no_gravity():
+    d, p, vp, W, base = light_node_scene(N.APIC)
+    m = 1e-6
+    g, dt = 9.8, 1e-4
+    heavy = np.float32(m) * W.astype(np.float32) > FLT_EPS
+    assert 0 < heavy.sum() < 27  # both branches are exercised
+    for prec in (64, 32):
+        o = O.OracleSolver(d, p, prec)
+        o.substep("reset")
+        o.substep("p2g")
+        o.substep("gravity")
+        o.substep("update", dt)
+        o.substep("collision", dt)
+        gr = o.read_grid()
+        res = d.resolution[0]
+        for a in range(3):
+            for b in range(3):
+                for c in range(3):
+                    i = (base[0] + a) + res * ((base[1] + b) + res * (base[2] + c))
+                    w = W[a, b, c]
+                    np.testing.assert_allclose(gr["mass"][i], m * w, rtol=1e-6)
+                    if heavy[a, b, c]:
+                        np.testing.assert_allclose(gr["velocity"][i], vp, rtol=1e-5)
+                        np.testing.assert_allclose(gr["velocity_star"][i], vp - [0, g * dt, 0], rtol=1e-5)
+                        np.testing.assert_allclose(gr["force"][i][1], -g * m * w, rtol=1e-5)
+                    else:
+                        np.testing.assert_allclose(gr["velocity"][i], m * w * vp, rtol=1e-5)
+                        assert np.all(gr["velocity_star"][i] == 0.0)
+                        assert gr["force"][i][1] == 0.0
+
+
+@pytest.mark.parametrize("transfer", [N.APIC, N.PIC_FLIP])
+def test_light_node_branch_in_one_step(transfer):
+    """After one step (PIC-FLIP with blend 0 or APIC), the particle's velocity
+    is the heavy nodes' weight sum times v_p - g dt y: light nodes add 0."""
+    d, p, vp, W, base = light_node_scene(transfer)
+    g, dt = 9.8, 1e-4
+    heavy = np.float32(1e-6) * W.astype(np.float32) > FLT_EPS
+    expect = W[heavy].sum() * (vp - [0, g * dt, 0])
+    for prec, tol in ((64, 1e-12), (32, 1e-6)):
+        o = O.OracleSolver(d, p, prec)
+        o.step(1)
+        v = o.read_particles64()["velocity"][0] if prec == 64 else o.read_particles().velocity[0].astype(np.float64)
+        np.testing.assert_allclose(v, expect, atol=tol * 10, rtol=tol)

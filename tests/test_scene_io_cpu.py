@@ -32,17 +32,53 @@ def _facade():
     return lib
 
 
-def _cread(path, stride):
+def _cread(path, stride, expect_rc=0):
     lib = _facade()
     cnt, fields = C.c_int64(0), C.c_int(0)
-    assert lib.chains_read_group(path.encode(), stride, None, None, None, None, 0, C.byref(cnt), C.byref(fields)) == 0
+    rc = lib.chains_read_group(path.encode(), stride, None, None, None, None, 0, C.byref(cnt), C.byref(fields))
+    assert rc == expect_rc
     n = cnt.value
     out = {"position": np.zeros((n, 3), np.float32), "velocity": np.zeros((n, 3), np.float32),
            "young": np.zeros(n, np.float32), "mass": np.zeros(n, np.float32)}
     fp = lambda a: a.ctypes.data_as(C.POINTER(C.c_float))
     assert lib.chains_read_group(path.encode(), stride, fp(out["position"]), fp(out["velocity"]), fp(out["young"]),
-                                 fp(out["mass"]), n, C.byref(cnt), C.byref(fields)) == 0
+                                 fp(out["mass"]), n, C.byref(cnt), C.byref(fields)) == expect_rc
     return out, fields.value
+
+
+# Text groups the reference's reader (src/mpm_solver3d_builder.cu:10-30, its
+# result ignored at :40) turns into these points: every line is read with
+# three float extractions, the first line that fails stops the file, and the
+# points before it are kept.
+TEXT_CASES = [
+    ("trailing_blank_line", "1 2 3\n4 5 6\n\n", [[1, 2, 3], [4, 5, 6]], False),
+    ("bad_line_mid_file", "1 2 3\n4 5\n7 8 9\n", [[1, 2, 3]], False),
+    ("crlf_and_extra_text", "1 2 3\r\n-.5 +2e1 3.25 tail\r\n", [[1, 2, 3], [-0.5, 20, 3.25]], True),
+    ("glued_signs", "1-2 3\n", [[1, -2, 3]], True),
+    ("no_final_newline", "\t0.125 0.25 0.5", [[0.125, 0.25, 0.5]], True),
+    ("leading_blank_line", "\n1 2 3\n", [], False),
+    ("letters", "1 2 x\n", [], False),
+    ("empty_file", "", [], True),
+]
+
+
+@pytest.mark.parametrize("name,text,points,complete", TEXT_CASES, ids=[c[0] for c in TEXT_CASES])
+def test_text_group_prefix_rule(tmp_path, name, text, points, complete):
+    """The C++ builder's reader, the Python loader and the reference's
+    observable result agree, including the prefix kept before a bad line."""
+    from mpm_amd.scene_io import read_text_group
+    p = str(tmp_path / (name + ".xyz"))
+    with open(p, "wb") as f:
+        f.write(text.encode())
+    want = np.array(points, np.float32).reshape(-1, 3)
+    pts, ok = read_text_group(p)
+    assert ok == complete and np.array_equal(pts, want), (pts, ok)
+    if len(points) == 0 and not complete:
+        _cread(p, 1.0, expect_rc=-1)  # nothing before the bad line: an empty group
+        return
+    cc, _ = _cread(p, 1.0, expect_rc=0 if complete else 1)
+    assert np.array_equal(cc["position"], want)
+    assert np.array_equal(read_group(p, 1.0)["position"], want)
 
 
 @pytest.mark.parametrize("world", [True, False])

@@ -63,7 +63,7 @@ class FakeBatch:
 
 class FakeBackend:
     def __init__(self, clock):
-        self.clock, self.log, self.solvers = clock, [], []
+        self.clock, self.log, self.solvers, self.batch_keys = clock, [], [], []
 
     def create_solver(self, desc, particles):
         self.clock.t += 2e-3  # build cost
@@ -72,6 +72,7 @@ class FakeBackend:
         return s
 
     def create_batch(self, interp, transfer):
+        self.batch_keys.append((interp, transfer))
         return FakeBatch((interp, transfer), self.clock, self.log)
 
 
@@ -97,7 +98,8 @@ def test_every_request_gets_exactly_its_steps():
         assert r.result == {"steps": r.steps}
     assert all(s.closed for s in be.solvers)
     assert srv.used == 0
-    assert set(srv.batches) == {(N.QUADRATIC_BSPLINE, N.APIC), (N.QUADRATIC_BSPLINE, N.PIC_FLIP)}
+    assert set(be.batch_keys) == {(N.QUADRATIC_BSPLINE, N.APIC), (N.QUADRATIC_BSPLINE, N.PIC_FLIP)}
+    assert srv.batches == {}  # emptied batches are freed
     assert rep.particle_updates == sum(r.particles * r.steps for r in rep.records)
 
 
@@ -165,3 +167,177 @@ def test_batch_key_space_limits_admission():
     srv, _, _ = _server(1e15)
     rep = srv.run(reqs)
     assert rep.completed == 5 and rep.peak_scenes == (1 << 32) // span
+
+
+# ------------------------------------------------------ failure isolation
+class HealthBatch(FakeBatch):
+    """Fake batch with a health pass, measured buffer bytes that grow by 1.5x
+    and never shrink (as the device batch's), and planted failures: members
+    named in `nan_after` report non-finite particles once they have taken
+    that many steps; a step raises MpmError when `fail_steps` is set."""
+
+    PER_PARTICLE = 4 * 19  # batch words per particle, as the device batch
+
+    def __init__(self, key, clock, log, nan_after, fail_steps, meter):
+        super().__init__(key, clock, log)
+        self.nan_after, self.fail_steps, self.meter, self.cap = nan_after, fail_steps, meter, 0
+
+    def add(self, s):
+        super().add(s)
+        n = sum(m.count for m in self.members)
+        if n > self.cap:
+            self.cap = max(n, int(self.cap * 1.5))
+        self.meter.check()
+
+    def step(self, k):
+        if self.fail_steps:
+            raise N.MpmError(N.MPM_ERR_HIP, "planted batch failure")
+        super().step(k)
+
+    def device_bytes(self):
+        return self.cap * self.PER_PARTICLE
+
+    def health(self):
+        out = []
+        for m in self.members:
+            bad = m.rid in self.nan_after and m.steps >= self.nan_after[m.rid]
+            out.append({"max_speed": float("nan") if bad else 1.0, "cfl": float("nan") if bad else 0.01,
+                        "nonfinite": 3 if bad else 0, "escaped": 0})
+        return out
+
+    def close(self):
+        self.closed = True
+        self.meter.batches.remove(self)
+
+
+class Meter:
+    """Real device bytes of the fake backend at every change, against the budget."""
+
+    def __init__(self, budget):
+        self.budget, self.solvers, self.batches, self.peak = budget, [], [], 0
+
+    def check(self):
+        now = sum(s.bytes for s in self.solvers if not s.closed) + sum(b.device_bytes() for b in self.batches)
+        self.peak = max(self.peak, now)
+        assert now <= self.budget, (now, self.budget)
+
+
+class MeteredSolver(FakeSolver):
+    def __init__(self, desc, particles, log, rid, meter):
+        super().__init__(desc, particles, log)
+        self.rid, self.meter = rid, meter
+        # own bytes: the estimate without the batch share (what the handle holds)
+        from mpm_amd.serve import _BATCH_GROWTH, _BATCH_WORDS
+        self.bytes = estimate_bytes(desc, self.count) - int(self.count * _BATCH_GROWTH * 4 * _BATCH_WORDS)
+
+    def stats(self):
+        return {"device_bytes": self.bytes, "cfl": 0.01}
+
+    def close(self):
+        self.closed = True
+        self.meter.check()
+
+
+class FailingBackend(FakeBackend):
+    def __init__(self, clock, budget, nan_after=None, fail_keys=(), fail_build=()):
+        super().__init__(clock)
+        self.meter = Meter(budget)
+        self.nan_after, self.fail_keys, self.fail_build = nan_after or {}, set(fail_keys), set(fail_build)
+        self.made = []
+
+    def create_solver(self, desc, particles):
+        rid = len(self.made)
+        self.made.append(rid)
+        if rid in self.fail_build:
+            raise N.MpmError(N.MPM_ERR_OOM, "planted build failure")
+        self.clock.t += 2e-3
+        s = MeteredSolver(desc, particles, self.log, rid, self.meter)
+        self.solvers.append(s)
+        self.meter.solvers.append(s)
+        self.meter.check()
+        return s
+
+    def create_batch(self, interp, transfer):
+        self.batch_keys.append((interp, transfer))
+        b = HealthBatch((interp, transfer), self.clock, self.log, self.nan_after, (interp, transfer) in self.fail_keys,
+                        self.meter)
+        self.meter.batches.append(b)
+        return b
+
+
+def _failing(budget=1e12, **kw):
+    c = Clock()
+    be = FailingBackend(c, budget, **{k: kw.pop(k) for k in ("nan_after", "fail_keys", "fail_build") if k in kw})
+    return Server(budget_bytes=budget, backend=be, clock=c, sleep=c.sleep, keep_results=True, **kw), be
+
+
+def test_nan_scene_is_evicted_and_others_continue():
+    reqs = [_req(i, 0.0, 1000 + 10 * i, 60) for i in range(4)]
+    srv, be = _failing(tick_steps=16, nan_after={2: 20})
+    rep = srv.run(reqs)
+    recs = {r.rid: r for r in rep.records}
+    assert "non-finite" in recs[2].error and recs[2].result is None
+    assert recs[2].steps_done == 32  # found by the health pass of the tick that produced it
+    assert rep.failed == 1 and rep.completed == 3
+    for k in (0, 1, 3):
+        assert recs[k].error == "" and recs[k].result == {"steps": 60}
+    assert all(s.closed for s in be.solvers) and srv.used == 0 and not srv.batches
+    assert rep.particle_updates == sum(recs[k].particles * 60 for k in (0, 1, 3))
+
+
+def test_failing_batch_does_not_stop_the_other():
+    reqs = [_req(i, 0.0, 1000, 40, N.APIC if i % 2 else N.PIC_FLIP) for i in range(6)]
+    srv, be = _failing(tick_steps=8, fail_keys=[(N.QUADRATIC_BSPLINE, N.APIC)])
+    rep = srv.run(reqs)
+    for r in rep.records:
+        if r.transfer == N.APIC:
+            assert "planted batch failure" in r.error
+        else:
+            assert r.error == "" and r.result == {"steps": 40}
+    assert rep.failed == 3 and rep.completed == 3
+    assert all(s.closed for s in be.solvers) and not srv.batches
+
+
+def test_failed_build_rejects_only_that_request():
+    reqs = [_req(i, 0.0, 1000, 20) for i in range(3)]
+    srv, be = _failing(fail_build=[1])
+    rep = srv.run(reqs)
+    recs = {r.rid: r for r in rep.records}
+    assert "build failed" in recs[1].error
+    assert recs[0].result == {"steps": 20} and recs[2].result == {"steps": 20}
+    assert rep.failed == 1 and rep.completed == 2
+
+
+def test_unexpected_error_frees_everything():
+    reqs = [_req(i, 0.0, 1000, 50) for i in range(3)]
+    srv, be = _failing()
+
+    def broken_read():
+        raise RuntimeError("host failure")
+
+    be_create = be.create_solver
+
+    def create(desc, particles):
+        s = be_create(desc, particles)
+        s.read_particles = broken_read
+        return s
+
+    be.create_solver = create
+    with pytest.raises(RuntimeError):
+        srv.run(reqs)
+    assert all(s.closed for s in be.solvers)
+    assert not srv.active and not srv.batches and not be.meter.batches
+
+
+def test_budget_holds_real_bytes_under_churn():
+    """Joins and departures of uneven scenes: the device bytes the fake
+    backend really holds (handles + batch buffers that only grow) stay within
+    the budget at every change."""
+    rng = np.random.default_rng(3)
+    reqs = [_req(i, 0.002 * i, int(rng.integers(1000, 20000)), int(rng.integers(8, 80))) for i in range(40)]
+    budget = 4 * estimate_bytes(reqs[0].desc, 20000)
+    srv, be = _failing(budget=budget, tick_steps=8)
+    rep = srv.run(reqs)
+    assert rep.completed == 40 and rep.failed == 0
+    assert be.meter.peak <= budget and rep.peak_bytes <= budget
+    assert srv.used == 0
