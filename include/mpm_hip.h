@@ -233,6 +233,16 @@ MPM_API int mpm_sort(void* handle);
 /* Copy particle state to the host in the original particle order (the
  * reference never reorders: src/mpm_solver3d_builder.cu:38-65). */
 MPM_API int mpm_read_particles(void* handle, mpm_particle_state* out);
+/* The same readback without allocating device memory: no counterpart in the
+ * reference.  For callers that cannot call hipMalloc (tight serving loops,
+ * where a hipFree would also wait for the whole device, and harnesses that
+ * forbid allocation).  The device staging is a file-scope array of the
+ * library (39.3 MB per GPU, held until the process ends) that every call on
+ * that GPU shares: at most 262,144 particles (larger handles get
+ * MPM_ERR_UNSUPPORTED; use mpm_read_particles).  Calls from several host
+ * threads on one GPU would race on the array, so the library serialises
+ * them; the array keeps the last call's contents. */
+MPM_API int mpm_read_particles_noalloc(void* handle, mpm_particle_state* out);
 /* Overwrite the particle state (same count as at creation). */
 MPM_API int mpm_write_particles(void* handle, const mpm_particle_state* in);
 /* Copy the dense grid (reference node order) to the host. */

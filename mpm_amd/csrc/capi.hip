@@ -14,6 +14,7 @@
 #include <cstdio>
 #include <cstring>
 #include <map>
+#include <mutex>
 #include <new>
 #include <string>
 #include <tuple>
@@ -843,6 +844,46 @@ MPM_API int mpm_sort(void* handle) {
     return finish(*s);
 }
 
+// The host-layout staging H (filled by launch_unpack on the solver's stream)
+// copied into the caller's arrays; waits for the copies.
+static hipError_t copy_out_staging(Solver& s, const HostLayoutDev& H, mpm_particle_state* out,
+                                   std::vector<uint16_t>& mat) {
+    const int64_t n = s.N;
+    const size_t f = (size_t)n * 4;
+    mat.resize((size_t)n);
+    auto cp = [&](void* dst, const void* src, size_t bytes) -> hipError_t {
+        if (!dst) return hipSuccess;
+        return hipMemcpyAsync(dst, src, bytes, hipMemcpyDeviceToHost, s.st);
+    };
+    hipError_t e = hipSuccess;
+    if (e == hipSuccess) e = cp(out->volume0, H.volume0, f);
+    if (e == hipSuccess) e = cp(out->mass, H.mass, f);
+    if (e == hipSuccess) e = cp(out->position, H.pos, 3 * f);
+    if (e == hipSuccess) e = cp(out->velocity, H.vel, 3 * f);
+    if (e == hipSuccess) e = cp(out->Fe, H.Fe, 9 * f);
+    if (e == hipSuccess) e = cp(out->Fp, H.Fp, 9 * f);
+    if (e == hipSuccess) e = cp(out->mu0, H.mu0, f);
+    if (e == hipSuccess) e = cp(out->lambda0, H.lam0, f);
+    if (e == hipSuccess && out->B) {
+        if (s.apic) e = cp(out->B, H.B, 9 * f);
+        else std::memset(out->B, 0, 9 * f);
+    }
+    if (e == hipSuccess) e = cp(mat.data(), H.mat, (size_t)n * 2);
+    if (e == hipSuccess) e = hipStreamSynchronize(s.st);
+    return e;
+}
+
+// Per-particle material constants from the material indices.
+static void fill_materials(const Solver& s, const std::vector<uint16_t>& mat, mpm_particle_state* out) {
+    for (int64_t i = 0; i < s.N; ++i) {
+        const MaterialDev& m = s.mats[mat[(size_t)i]];
+        if (out->hardening) out->hardening[i] = m.hardening;
+        if (out->critical_compression) out->critical_compression[i] = m.crit_c;
+        if (out->critical_stretch) out->critical_stretch[i] = m.crit_s;
+        if (out->model) out->model[i] = m.model;
+    }
+}
+
 MPM_API int mpm_read_particles(void* handle, mpm_particle_state* out) {
     GET_SOLVER(s, handle);
     if (!out || out->count != s->N) return fail(MPM_ERR_INVALID, "output count must equal the particle count");
@@ -852,36 +893,86 @@ MPM_API int mpm_read_particles(void* handle, mpm_particle_state* out) {
     int rc = alloc_staging(*s, g, n);
     if (rc) { free_staging(*s, g); return rc; }
     launch_unpack(s->P, g.H, s->st);
-    const size_t f = (size_t)n * 4;
-    std::vector<uint16_t> mat((size_t)n);
-    auto cp = [&](void* dst, const void* src, size_t bytes) -> hipError_t {
-        if (!dst) return hipSuccess;
-        return hipMemcpyAsync(dst, src, bytes, hipMemcpyDeviceToHost, s->st);
-    };
-    hipError_t e = hipSuccess;
-    if (e == hipSuccess) e = cp(out->volume0, g.H.volume0, f);
-    if (e == hipSuccess) e = cp(out->mass, g.H.mass, f);
-    if (e == hipSuccess) e = cp(out->position, g.H.pos, 3 * f);
-    if (e == hipSuccess) e = cp(out->velocity, g.H.vel, 3 * f);
-    if (e == hipSuccess) e = cp(out->Fe, g.H.Fe, 9 * f);
-    if (e == hipSuccess) e = cp(out->Fp, g.H.Fp, 9 * f);
-    if (e == hipSuccess) e = cp(out->mu0, g.H.mu0, f);
-    if (e == hipSuccess) e = cp(out->lambda0, g.H.lam0, f);
-    if (e == hipSuccess && out->B) {
-        if (s->apic) e = cp(out->B, g.H.B, 9 * f);
-        else std::memset(out->B, 0, 9 * f);
-    }
-    if (e == hipSuccess) e = cp(mat.data(), g.H.mat, (size_t)n * 2);
-    if (e == hipSuccess) e = hipStreamSynchronize(s->st);
+    std::vector<uint16_t> mat;
+    const hipError_t e = copy_out_staging(*s, g.H, out, mat);
     free_staging(*s, g);
     if (e != hipSuccess) return fail(MPM_ERR_HIP, std::string("read particles: ") + hipGetErrorString(e));
-    for (int64_t i = 0; i < n; ++i) {
-        const MaterialDev& m = s->mats[mat[(size_t)i]];
-        if (out->hardening) out->hardening[i] = m.hardening;
-        if (out->critical_compression) out->critical_compression[i] = m.crit_c;
-        if (out->critical_stretch) out->critical_stretch[i] = m.crit_s;
-        if (out->model) out->model[i] = m.model;
+    fill_materials(*s, mat, out);
+    return finish(*s);
+}
+
+// ------------------------------------------- allocation-free readback
+// mpm_read_particles_noalloc stages through this file-scope device array
+// instead of hipMalloc'd buffers: 37 floats per particle (volume0, mass,
+// mu0, lambda0, x3, v3, Fe9, Fp9, B9) for up to kNoAllocParticles
+// particles, then their 16-bit material indices.  Each GPU holds its own
+// copy (39.3 MB) until the process ends.  Host code never names the array
+// except to take its address: `scratch` is a host-side stand-in there.
+constexpr int64_t kNoAllocParticles = int64_t(1) << 18;
+constexpr int64_t kNoAllocFloats = 37 * kNoAllocParticles + kNoAllocParticles / 2;
+static __device__ __align__(256) float scratch[kNoAllocFloats];
+
+constexpr int kNoAllocDevices = 64;
+static std::mutex g_scratch_mu;  // guards g_scratch_ptr
+static float* g_scratch_ptr[kNoAllocDevices] = {};
+static std::mutex g_scratch_use[kNoAllocDevices];  // one readback at a time per GPU
+
+// This device's address of `scratch`, taken once per device with that
+// device current (hipGetSymbolAddress; every device has its own copy).
+static int scratch_address(int dev, float** out) {
+    if (dev < 0 || dev >= kNoAllocDevices) return fail(MPM_ERR_UNSUPPORTED, "device index out of range");
+    std::lock_guard<std::mutex> g(g_scratch_mu);
+    if (!g_scratch_ptr[dev]) {
+        void* p = nullptr;
+        const hipError_t e = hipGetSymbolAddress(&p, HIP_SYMBOL(scratch));
+        if (e != hipSuccess || !p)
+            return fail(MPM_ERR_HIP, std::string("hipGetSymbolAddress(scratch): ") + hipGetErrorString(e));
+        g_scratch_ptr[dev] = static_cast<float*>(p);
     }
+    *out = g_scratch_ptr[dev];
+    return MPM_OK;
+}
+
+MPM_API int mpm_read_particles_noalloc(void* handle, mpm_particle_state* out) {
+    GET_SOLVER(s, handle);
+    if (!out || out->count != s->N) return fail(MPM_ERR_INVALID, "output count must equal the particle count");
+    const int64_t n = s->N;
+    if (n > kNoAllocParticles)
+        return fail(MPM_ERR_UNSUPPORTED, "mpm_read_particles_noalloc reads at most " +
+                                             std::to_string(kNoAllocParticles) + " particles (this handle has " +
+                                             std::to_string(n) + "); use mpm_read_particles");
+    if (n == 0) return MPM_OK;
+    // the solver's device is current for the address and the launch
+    int prev = 0;
+    HIP_TRY(hipGetDevice(&prev));
+    if (prev != s->device) HIP_TRY(hipSetDevice(s->device));
+    struct Restore {
+        int dev, prev;
+        ~Restore() { if (dev != prev) (void)hipSetDevice(prev); }
+    } restore{s->device, prev};
+    float* base = nullptr;
+    int rc = scratch_address(s->device, &base);
+    if (rc) return rc;
+    std::lock_guard<std::mutex> use(g_scratch_use[s->device]);
+    // regions sized for kNoAllocParticles: every entry below n is written
+    // by the unpack (pid is a permutation), so nothing needs clearing
+    const int64_t c = kNoAllocParticles;
+    HostLayoutDev H{};
+    H.volume0 = base;
+    H.mass = base + c;
+    H.mu0 = base + 2 * c;
+    H.lam0 = base + 3 * c;
+    H.pos = base + 4 * c;
+    H.vel = base + 7 * c;
+    H.Fe = base + 10 * c;
+    H.Fp = base + 19 * c;
+    H.B = base + 28 * c;
+    H.mat = reinterpret_cast<uint16_t*>(base + 37 * c);
+    launch_unpack(s->P, H, s->st);
+    std::vector<uint16_t> mat;
+    const hipError_t e = copy_out_staging(*s, H, out, mat);
+    if (e != hipSuccess) return fail(MPM_ERR_HIP, std::string("read particles (no alloc): ") + hipGetErrorString(e));
+    fill_materials(*s, mat, out);
     return finish(*s);
 }
 

@@ -168,6 +168,7 @@ SIGNATURES = {
     "mpm_set_reorder_interval": (C.c_int, [C.c_void_p, C.c_int]),
     "mpm_sort": (C.c_int, [C.c_void_p]),
     "mpm_read_particles": (C.c_int, [C.c_void_p, C.POINTER(MpmParticleState)]),
+    "mpm_read_particles_noalloc": (C.c_int, [C.c_void_p, C.POINTER(MpmParticleState)]),
     "mpm_write_particles": (C.c_int, [C.c_void_p, C.POINTER(MpmParticleState)]),
     "mpm_read_grid": (C.c_int, [C.c_void_p, C.POINTER(MpmGridHost)]),
     "mpm_read_positions4": (C.c_int, [C.c_void_p, _fp]),

@@ -259,12 +259,16 @@ class Solver:
         N.check(self._lib.mpm_sort(self._h))
 
     # -- state I/O
-    def read_particles(self) -> ParticleState:
+    def read_particles(self, noalloc: bool = False) -> ParticleState:
+        """State in the caller's particle order.  noalloc: stage through the
+        library's static device array (no hipMalloc; at most 262,144
+        particles)."""
         st = ParticleState.empty(self.count)
         if self.desc.transfer != N.APIC:
             st.B = None
         c = st.to_c()
-        N.check(self._lib.mpm_read_particles(self._h, C.byref(c)))
+        fn = self._lib.mpm_read_particles_noalloc if noalloc else self._lib.mpm_read_particles
+        N.check(fn(self._h, C.byref(c)))
         return st
 
     def write_particles(self, st: ParticleState):
