@@ -146,6 +146,10 @@ typedef struct mpm_stats {
      * dropped when the particle collision switch, the grid record mode or
      * the material table changes) */
     int64_t graph_captures;
+    /* steps whose cell sort updated the previous step's order instead of
+     * sorting in full (scenes of at least 2^20 particles: few particles
+     * change cell per step; the result is the same permutation) */
+    int64_t incremental_sorts;
 } mpm_stats;
 
 /* Per-kernel timing (hipEvent pairs on the handle's stream), enabled by
@@ -221,6 +225,17 @@ MPM_API int mpm_set_grid_record(void* handle, int enable);
  * 1000 steps without it).  Results do not depend on the interval beyond
  * float-atomic order; reads return the caller's particle order. */
 MPM_API int mpm_set_reorder_interval(void* handle, int steps);
+/* Transfer fusion (no counterpart in the reference, which runs
+ * gridToParticles and the next step's particlesToGrid as separate passes
+ * with a fresh grid between them, src/apic_mpm_solver3d.cu:99-124): for an
+ * APIC scene with the quadratic stencil stepped alone (not in a
+ * batch, not under mpm_step_graph, grid record off), one kernel gathers
+ * step n and, with each particle still in registers, scatters step n+1's
+ * particlesToGrid into a second grid buffer (one more grid of device
+ * memory).  The sums are the same up to float order.  Any other call that
+ * touches particles or grid first restores the two-pass state.  1 (the
+ * default) enables it, 0 disables it. */
+MPM_API int mpm_set_transfer_fusion(void* handle, int enable);
 
 /* Particle-by-cell key generation + stable LSD radix sort (no reference
  * counterpart: the reference scatters in array order,

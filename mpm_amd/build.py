@@ -23,7 +23,7 @@ LIB = os.path.join(PKG, "lib")
 OBJ = os.path.join(PKG, "lib", "obj")
 INCLUDE = os.path.join(ROOT, "include")
 
-HIP_SOURCES = ["sort.hip", "kernels.hip", "capi.hip", "batch.hip"]
+HIP_SOURCES = ["sort.hip", "kernels.hip", "capi.hip", "batch.hip", "incsort.hip"]
 FACADE_SOURCES = ["mpm3d_parser.cpp", "material_point3d.cpp", "mpm_solver3d.cpp", "mpm_solver3d_builder.cpp"]
 
 HIPCC = os.environ.get("HIPCC", "/opt/rocm/bin/hipcc")

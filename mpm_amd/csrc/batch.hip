@@ -528,6 +528,7 @@ MPM_API int mpm_batch_add(void* batch, void* scene) {
         return fail(MPM_ERR_UNSUPPORTED, "batch key space exhausted: the members' grids would total more than 2^32 "
                                          "cell keys");
     // Finish the scene's own work, then move it onto the batch stream.
+    unfuse(*s);
     if (s->grid_dirty && (!s->count_on_clear || s->vstar_dirty)) run_clear(*s);
     ensure_stress(*s);
     HIP_TRY(hipStreamSynchronize(s->st));

@@ -239,6 +239,7 @@ static void drop_graph(Solver& s) {
 
 int upload_materials(Solver& s) {
     drop_graph(s);
+    unfuse(s);  // the pre-scattered forces used the old materials
     if (s.mats_dev) dfree(s, s.mats_dev);
     s.mats_dev = nullptr;
     size_t bytes = std::max<size_t>(1, s.mats.size()) * sizeof(MaterialDev);
@@ -311,6 +312,7 @@ void free_staging(Solver& s, Staging& g) {
 // ---- device phases
 void run_sort(Solver& s) {
     ProfileHook h = hook(s);
+    s.snap_ok = false;  // a sort outside the fused step: the move list no longer applies
     if (s.N <= 0) { s.sort_valid = true; return; }
     if (!s.keys_valid) {
         Scope sc(h, "keys");
@@ -333,6 +335,7 @@ void run_sort(Solver& s) {
 
 void run_clear(Solver& s) {
     ProfileHook h = hook(s);
+    unfuse(s);
     Scope sc(h, "grid_clear");
     launch_clear(s.S, s.nsb, s.vstar_dirty ? GF_ALL : (s.force_dirty ? GF_PRIMARY : GF_FOLDED),
                  s.count_on_clear ? s.acc : nullptr, s.st);
@@ -351,25 +354,130 @@ void ensure_stress(Solver& s) {
     s.stress_valid = true;
 }
 
+// The fused step's sort of a large scene: the plan word chooses between the
+// incremental update of the last step's order (few moved particles) and the
+// full radix sort; both write the same buffers, the other's launches return
+// at once.  Bit-identical to run_sort.
+void run_sort_incremental(Solver& s) {
+    ProfileHook h = hook(s);
+    if (!s.keys_valid) {
+        Scope sc(h, "keys");
+        launch_keys(s.S, s.P, s.interp, s.kin, s.st);
+        s.keys_valid = true;
+        s.snap_ok = false;
+    }
+    if (!s.snap_ok) launch_inc_set(s.plan, 0u, s.st);  // the full sort this time
+    {
+        Scope sc(h, "sort_plan");
+        launch_inc_plan(s.plan, s.st);
+    }
+    const int sel = radix_sort_pairs(s.kin, s.iota, s.ka, s.va, s.kb, s.vb, s.N, s.key_bits, s.hist, s.scan_temp,
+                                     s.st, h, s.plan);
+    uint32_t* out_k = sel ? s.kb : s.ka;
+    uint32_t* out_p = sel ? s.vb : s.va;
+    s.I.sorted_keys = out_k;
+    s.I.perm = out_p;
+    s.I.start = s.items;
+    s.I.blk = s.items + s.N;
+    s.I.count = s.counters;
+    s.I.n = s.N;
+    {
+        Scope sc(h, "sort_incr");
+        launch_inc_sort(s.plan, s.moves, s.snap_k, s.snap_p, s.N, out_k, out_p, s.st);
+    }
+    {
+        Scope sc(h, "items");
+        launch_items(s.I.sorted_keys, s.N, s.flags, s.scan_temp, s.items, s.counters, s.st);
+    }
+    s.sort_valid = true;
+}
+
+// G2P fused with the next step's P2G (k_g2p2g) for the folded APIC step of
+// a single scene with the quadratic stencil (APIC with the linear one is
+// rejected at creation; the cubic one keeps the two-kernel step: its 7^3
+// tiles leave no room for a second set).
+static bool use_fused_transfer(const Solver& s) {
+    return s.fuse_transfer && fused_folds(s) && s.interp == MPM_INTERP_QUADRATIC_BSPLINE && !s.capturing &&
+           !s.batch && s.N > 0;
+}
+
+// The second grid buffer of the fused transfer (allocated on first use).
+static int ensure_grid2(Solver& s) {
+    if (s.grid2) return MPM_OK;
+    const size_t gbytes = (size_t)s.S.plane * GF_ALL * sizeof(float);
+    int rc = dalloc(s, (void**)&s.grid2, gbytes);
+    if (rc) return rc;
+    if ((rc = dalloc(s, (void**)&s.touched2, (size_t)s.nsb * 4))) return rc;
+    HIP_TRY(hipMemsetAsync(s.grid2, 0, gbytes, s.st));
+    HIP_TRY(hipMemsetAsync(s.touched2, 0, (size_t)s.nsb * 4, s.st));
+    s.grid2_dirty = false;
+    return MPM_OK;
+}
+
 void run_fused_step(Solver& s) {
     ProfileHook h = hook(s);
-    if (s.grid_dirty) run_clear(s);
-    ensure_stress(s);
+    const bool fuse = use_fused_transfer(s) && ensure_grid2(s) == MPM_OK;
+    const bool scattered = fuse && s.prescat;  // the last step already did this step's P2G
+    if (!scattered) {
+        if (s.grid_dirty) run_clear(s);
+        ensure_stress(s);
+    }
     // a raster or sub-step since the last change of the positions has
     // already sorted them
-    if (!s.sort_valid) run_sort(s);
+    if (!s.sort_valid) {
+        if (s.incr) run_sort_incremental(s);
+        else run_sort(s);
+    } else if (s.incr) {
+        // sorted elsewhere (a store reorder, a raster): this step's P2G
+        // takes the snapshot of that order and its G2P starts a new list
+        launch_inc_set(s.plan, 1u, s.st);
+    }
+    ItemsDev I = s.I;
+    if (s.incr) {
+        I.snap_k = s.snap_k;
+        I.snap_p = s.snap_p;
+        I.plan = s.plan;
+        I.moves = s.moves_dev;
+    }
     // APIC's G2P reads v* alone, so by default its P2G sums p + dt f into
     // one set of planes (4 instead of 7 per node)
     const bool fold = fused_folds(s);
-    {
-        Scope sc(h, "p2g");
-        launch_p2g(s.S, s.P, s.I, s.interp, s.apic, fold ? 2 : 0, s.item_wgs, s.st);
+    if (fuse) {
+        if (!scattered) {
+            Scope sc(h, "p2g");
+            launch_p2g(s.S, s.P, I, s.interp, s.apic, 2, s.item_wgs, s.st);
+        }
+        if (s.grid2_dirty) {  // the grid the last step read (its active nodes counted), cleared
+            SceneDev S2 = s.S;
+            S2.grid = s.grid2;
+            S2.touched = s.touched2;
+            Scope sc(h, "grid_clear");
+            launch_clear(S2, s.nsb, GF_FOLDED, s.grid2_count ? s.acc : nullptr, s.st);
+            s.grid2_dirty = false;
+        }
+        {
+            Scope sc(h, "g2p2g");
+            launch_g2p2g(s.S, s.P, I, s.S.particle_collision, s.grid2, s.touched2, s.kin, s.st);
+        }
+        // the grid just scattered is the next step's
+        std::swap(s.S.grid, s.grid2);
+        std::swap(s.S.touched, s.touched2);
+        s.grid2_dirty = true;
+        s.grid2_count = true;
+        s.prescat = true;
+        s.stress_valid = false;  // the fused kernel uses the new stress at once
+    } else {
+        {
+            Scope sc(h, "p2g");
+            launch_p2g(s.S, s.P, I, s.interp, s.apic, fold ? 2 : 0, s.item_wgs, s.st);
+        }
+        {
+            Scope sc(h, "g2p");
+            launch_g2p(s.S, s.P, I, s.interp, s.apic, fold ? 3 : 0, 1, s.S.particle_collision, s.item_wgs, s.st,
+                       s.kin, nullptr, MPM_SOLVER_G2P_WG);
+        }
     }
-    {
-        Scope sc(h, "g2p");
-        launch_g2p(s.S, s.P, s.I, s.interp, s.apic, fold ? 3 : 0, 1, s.S.particle_collision, s.item_wgs, s.st,
-                   s.kin, nullptr, MPM_SOLVER_G2P_WG);
-    }
+    s.snap_ok = s.incr;
     s.grid_dirty = true;
     s.count_on_clear = true;
     if (!fold) s.force_dirty = true;
@@ -381,6 +489,7 @@ void run_fused_step(Solver& s) {
 }
 
 void run_initialize(Solver& s) {
+    unfuse(s);
     if (s.grid_dirty) run_clear(s);
     run_sort(s);
     launch_p2g(s.S, s.P, s.I, s.interp, s.apic, 1, s.item_wgs, s.st);
@@ -434,6 +543,21 @@ int build(Solver& s, HostState& H, int run_init) {
     HIP_TRY(hipMemsetAsync(s.counters, 0, 64, s.st));
     if ((rc = dalloc(s, (void**)&s.acc, 64))) return rc;
     HIP_TRY(hipMemsetAsync(s.acc, 0, 64, s.st));
+    // incremental sort of a large scene
+    s.incr = s.N >= kIncMinParticles;
+    if (s.incr) {
+        if ((rc = dalloc(s, (void**)&s.snap_k, (size_t)n * 4))) return rc;
+        if ((rc = dalloc(s, (void**)&s.snap_p, (size_t)n * 4))) return rc;
+        if ((rc = dalloc(s, (void**)&s.plan, kPlanWords * 4))) return rc;
+        HIP_TRY(hipMemsetAsync(s.plan, 0, kPlanWords * 4, s.st));  // snapshot not valid yet
+        uint32_t** lists[] = {&s.moves.key, &s.moves.slot, &s.moves.pos, &s.moves.lb, &s.moves.rank,
+                              &s.moves.drank, &s.moves.dbelow, &s.moves.del_sorted, &s.moves.lb_sorted};
+        for (auto pp : lists)
+            if ((rc = dalloc(s, (void**)pp, (size_t)kMoveMax * 4))) return rc;
+        if ((rc = dalloc(s, (void**)&s.moves_dev, sizeof(MoveList)))) return rc;
+        HIP_TRY(hipMemcpyAsync(s.moves_dev, &s.moves, sizeof(MoveList), hipMemcpyHostToDevice, s.st));
+        HIP_TRY(hipStreamSynchronize(s.st));  // the host copy is a member: it outlives the copy anyway
+    }
     // particles
     if ((rc = alloc_particles(s, s.P, s.N))) return rc;
     if (s.N > 0) {
@@ -618,6 +742,10 @@ static int reorder_store(Solver& s) {
     launch_reorder_store(s.P, nullptr, 0, s.I.perm, s.N, s.reorder_scratch, s.apic != 0, s.st);
     HIP_TRY(hipMemcpyAsync(s.kin, s.I.sorted_keys, (size_t)s.N * 4, hipMemcpyDeviceToDevice, s.st));
     s.I.perm = s.iota;
+    // slots renumbered: the last move list refers to the old ones (a
+    // replayed step sorts in full; an eager step takes a new snapshot)
+    if (s.incr) launch_inc_set(s.plan, 0u, s.st);
+    s.snap_ok = false;
     s.keys_valid = true;
     s.sort_valid = true;
     return MPM_OK;
@@ -631,6 +759,13 @@ MPM_API int mpm_set_reorder_interval(void* handle, int steps) {
     GET_SOLVER(s, handle);
     if (steps < 0) return fail(MPM_ERR_INVALID, "steps < 0");
     s->reorder_interval = steps;
+    return MPM_OK;
+}
+
+MPM_API int mpm_set_transfer_fusion(void* handle, int enable) {
+    GET_SOLVER(s, handle);
+    if (!enable) unfuse(*s);
+    s->fuse_transfer = enable != 0;
     return MPM_OK;
 }
 
@@ -663,12 +798,34 @@ MPM_API int mpm_step_graph(void* handle, int n) {
     if (n == 0) return MPM_OK;
     // A grid left by initialize / sub-steps is cleared outside the graph
     // (without counting); the captured step always starts with a clear.
+    unfuse(*s);
     const bool fold = fused_folds(*s);
     if (s->grid_dirty && (!s->count_on_clear || s->vstar_dirty || (fold && s->force_dirty))) run_clear(*s);
     ensure_stress(*s);  // outside the graph: replays rely on G2P keeping it current
     if (!s->keys_valid) {  // likewise the sort input keys
         launch_keys(s->S, s->P, s->interp, s->kin, s->st);
         s->keys_valid = true;
+        s->snap_ok = false;
+    }
+    if (s->gexec && s->S.grid != s->graph_grid) {
+        // fused steps since the capture left the grid in the other buffer:
+        // both cleared here (the last step's grid counted), then swapped
+        if (s->grid_dirty) run_clear(*s);
+        if (s->grid2_dirty) {
+            SceneDev S2 = s->S;
+            S2.grid = s->grid2;
+            S2.touched = s->touched2;
+            launch_clear(S2, s->nsb, GF_FOLDED, s->grid2_count ? s->acc : nullptr, s->st);
+            s->grid2_dirty = false;
+        }
+        std::swap(s->S.grid, s->grid2);
+        std::swap(s->S.touched, s->touched2);
+    }
+    if (s->incr && !s->snap_ok) {
+        // the next replay sorts in full (outside the graph, which always
+        // holds the planned sort)
+        launch_inc_set(s->plan, 0u, s->st);
+        s->snap_ok = true;
     }
     if (!s->gexec) {
         bool prof = s->prof;
@@ -682,7 +839,9 @@ MPM_API int mpm_step_graph(void* handle, int n) {
         s->vstar_dirty = false;
         s->force_dirty = !fold;
         s->sort_valid = false;
+        s->capturing = true;  // the captured step is the two-kernel one
         run_fused_step(*s);
+        s->capturing = false;
         s->steps -= 1;  // the capture ran nothing
         s->version -= 1;
         hipError_t e = hipStreamEndCapture(s->st, &s->graph);
@@ -691,6 +850,8 @@ MPM_API int mpm_step_graph(void* handle, int n) {
         if (e != hipSuccess) return fail(MPM_ERR_HIP, std::string("graph capture: ") + hipGetErrorString(e));
         HIP_TRY(hipGraphInstantiate(&s->gexec, s->graph, nullptr, nullptr, 0));
         s->graph_captures += 1;
+        s->graph_I = s->I;  // the sort buffers the replays write
+        s->graph_grid = s->S.grid;
     }
     auto t0 = std::chrono::steady_clock::now();
     for (int i = 0; i < n; ++i) {
@@ -711,6 +872,8 @@ MPM_API int mpm_step_graph(void* handle, int n) {
     s->force_dirty = !fold;
     s->phase = fold ? PH_FOLDED : PH_RAW;
     s->sort_valid = false;
+    s->snap_ok = s->incr;
+    s->I = s->graph_I;  // a reorder between replays pointed the host copy at the identity
     auto t1 = std::chrono::steady_clock::now();
     s->last_step_ms = std::chrono::duration<double, std::milli>(t1 - t0).count();
     return finish(*s);
@@ -731,6 +894,7 @@ MPM_API int mpm_reset_grid(void* handle) {
 
 MPM_API int mpm_particles_to_grid(void* handle) {
     GET_SOLVER(s, handle);
+    unfuse(*s);
     ensure_stress(*s);
     if (!s->sort_valid) run_sort(*s);
     launch_p2g(s->S, s->P, s->I, s->interp, s->apic, 0, s->item_wgs, s->st);
@@ -763,6 +927,7 @@ MPM_API int mpm_set_grid_record(void* handle, int enable) {
 MPM_API int mpm_compute_gravity_forces(void* handle) {
     GET_SOLVER(s, handle);
     if (int rc = require_unfolded(*s, "computeGravityForces")) return rc;
+    unfuse(*s);
     s->force_dirty = true;
     launch_grid_op(s->S, s->nsb, 0, s->S.dt, s->st);
     s->phase = PH_SUBSTEP;
@@ -774,6 +939,7 @@ MPM_API int mpm_compute_external_forces(void* handle) { return mpm_compute_gravi
 MPM_API int mpm_update_grid_velocities(void* handle, float dt) {
     GET_SOLVER(s, handle);
     if (int rc = require_unfolded(*s, "updateGridVelocities")) return rc;
+    unfuse(*s);
     launch_grid_op(s->S, s->nsb, 1, dt, s->st);
     s->vstar_dirty = true;
     s->phase = PH_SUBSTEP;
@@ -790,6 +956,7 @@ MPM_API int mpm_solve_linear_system(void* handle, float dt) {
 MPM_API int mpm_grid_collision(void* handle, float dt) {
     GET_SOLVER(s, handle);
     if (int rc = require_unfolded(*s, "gridCollision")) return rc;
+    unfuse(*s);
     launch_grid_op(s->S, s->nsb, 2, dt, s->st);
     s->vstar_dirty = true;
     s->phase = PH_SUBSTEP;
@@ -799,6 +966,7 @@ MPM_API int mpm_grid_collision(void* handle, float dt) {
 MPM_API int mpm_grid_to_particles(void* handle, float dt, float blend) {
     GET_SOLVER(s, handle);
     if (int rc = require_unfolded(*s, "gridToParticles")) return rc;
+    unfuse(*s);
     if (s->phase == PH_RAW) materialize(*s);
     if (!s->sort_valid) run_sort(*s);
     SceneDev S = s->S;
@@ -810,12 +978,14 @@ MPM_API int mpm_grid_to_particles(void* handle, float dt, float blend) {
 
 MPM_API int mpm_particles_collision(void* handle, float dt) {
     GET_SOLVER(s, handle);
+    unfuse(*s);
     launch_particles_collision(s->S, s->P, dt, s->st);
     return finish(*s);
 }
 
 MPM_API int mpm_advect_particles(void* handle, float dt) {
     GET_SOLVER(s, handle);
+    unfuse(*s);
     launch_advect(s->P, dt, s->st);
     s->sort_valid = false;
     s->keys_valid = false;
@@ -984,6 +1154,7 @@ MPM_API int mpm_write_particles(void* handle, const mpm_particle_state* in) {
     if (!in->volume0 || !in->mass || !in->position || !in->velocity || !in->Fe || !in->Fp || !in->mu0 ||
         !in->lambda0 || !in->hardening || !in->critical_compression || !in->critical_stretch || !in->model)
         return fail(MPM_ERR_INVALID, "state has a NULL array");
+    unfuse(*s);
     std::vector<uint16_t> mat;
     size_t before = s->mats.size();
     int rc = dedupe_materials(*s, n, in->hardening, in->critical_compression, in->critical_stretch, in->model, mat);
@@ -1299,9 +1470,10 @@ MPM_API int mpm_get_stats(void* handle, mpm_stats* out) {
     HIP_TRY(hipMemcpyAsync(&s->acc[3], &zero, 8, hipMemcpyHostToDevice, s->st));
     launch_max_speed2(s->P, reinterpret_cast<unsigned int*>(&s->acc[3]), s->st);
     unsigned long long a[4] = {0, 0, 0, 0};
-    uint32_t items = 0;
+    uint32_t items = 0, plan[kPlanWords] = {0};
     HIP_TRY(hipMemcpyAsync(a, s->acc, 32, hipMemcpyDeviceToHost, s->st));
     HIP_TRY(hipMemcpyAsync(&items, s->counters, 4, hipMemcpyDeviceToHost, s->st));
+    if (s->plan) HIP_TRY(hipMemcpyAsync(plan, s->plan, sizeof(plan), hipMemcpyDeviceToHost, s->st));
     HIP_TRY(hipStreamSynchronize(s->st));
     out->particles = s->N;
     out->steps = s->steps;
@@ -1317,6 +1489,7 @@ MPM_API int mpm_get_stats(void* handle, mpm_stats* out) {
     out->max_speed = std::sqrt((double)v2);
     out->cfl = out->max_speed * (double)s->S.dt / (double)s->S.h;
     out->graph_captures = s->graph_captures;
+    out->incremental_sorts = plan[kPlanIncSteps];
     return finish(*s);
 }
 

@@ -99,6 +99,27 @@ struct SceneEntry {
     uint32_t index;
 };
 
+// Incremental cell sort of one large scene (incsort.hip): G2P appends each
+// particle whose key changed -- new key, slot, old sorted position -- to a
+// list of at most kMoveMax entries; the next step's plan word picks the
+// incremental update or the full radix sort.
+constexpr int kMoveMax = 16384;
+enum PlanWord {
+    kPlanMoved = 0,     // moves appended by this step's G2P (past kMoveMax: counted, not stored)
+    kPlanMode = 1,      // this step's choice: kPlanFull or kPlanIncremental
+    kPlanCount = 2,     // list entries the update consumes
+    kPlanSnapOk = 3,    // the snapshot is the previous step's order
+    kPlanIncSteps = 4,  // steps sorted by the update since creation
+    kPlanWords = 16
+};
+constexpr uint32_t kPlanFull = 0u;
+constexpr uint32_t kPlanIncremental = 1u;
+struct MoveList {
+    uint32_t *key, *slot, *pos;            // appended by G2P
+    uint32_t *lb, *rank, *drank, *dbelow;  // per entry, during the update
+    uint32_t *del_sorted, *lb_sorted;      // removed positions, insertion places (ascending)
+};
+
 // Work items of the block-tiled kernels: a contiguous run of sorted
 // particles that share one key block (and so one scene).
 struct ItemsDev {
@@ -109,4 +130,10 @@ struct ItemsDev {
     const uint32_t* count;        // device counter: number of items
     const uint32_t* scene;        // per item scene index (batches) or null
     int64_t n;
+    // the fused step of one large scene (else null): P2G copies the sorted
+    // keys and permutation to snap_k/snap_p, G2P appends moved particles
+    uint32_t* snap_k;
+    uint32_t* snap_p;
+    uint32_t* plan;
+    const MoveList* moves;  // device copy of the list's pointers
 };

@@ -348,22 +348,107 @@ __device__ __forceinline__ void stress_from_svd(const float U[3][3], const float
     }
 }
 
-// updateDeformationGradient, src/material_point3d.cu:53-70: trial Fe, SVD,
-// clamp singular values to [1-theta_c, 1+theta_s], split into Fe and Fp.
-// Returns the SVD of the new Fe (U, clamped s, V) for the stress.
-// The reference's Fp = V diag(1/sc) U^T (Fn Fp) is evaluated as
-// V diag(s/sc) V^T Fp, using U^T Fn = diag(s) V^T for the SVD Fn = U diag(s)
-// V^T: the same matrix in exact arithmetic, two 3x3 products instead of four,
-// and Fn Fp need not be kept through the SVD.
-__device__ __forceinline__ void update_deformation(float Fe[3][3], float Fp[3][3], const float grad[3][3], float dt,
-                                                   const MaterialDev& mat, float U[3][3], float s[3],
-                                                   float V[3][3]) {
-    float L[3][3], Fn[3][3];
+// Trial elastic deformation gradient Fn = (I + dt grad v) Fe
+// (src/material_point3d.cu:57).
+__device__ __forceinline__ void trial_deformation(const float Fe[3][3], const float grad[3][3], float dt,
+                                                  float Fn[3][3]) {
+    float L[3][3];
 #pragma unroll
     for (int i = 0; i < 3; ++i)
 #pragma unroll
         for (int j = 0; j < 3; ++j) L[i][j] = (i == j ? 1.0f : 0.0f) + dt * grad[i][j];
     m3_mul(L, Fe, Fn);
+}
+
+// Symmetric 3x3 matrices as (xx, xy, xz, yy, yz, zz).  sym_mul is the
+// product of two symmetric matrices that commute (polynomials in one
+// matrix), which is symmetric again.
+__device__ __forceinline__ float sym_at(const float a[6], int i, int j) {
+    const int k = i <= j ? (i == 0 ? j : (i == 1 ? 2 + j : 5)) : (j == 0 ? i : (j == 1 ? 2 + i : 5));
+    return a[k];
+}
+__device__ __forceinline__ void sym_mul(const float a[6], const float b[6], float c[6]) {
+    const int ii[6] = {0, 0, 0, 1, 1, 2}, jj[6] = {0, 1, 2, 1, 2, 2};
+#pragma unroll
+    for (int k = 0; k < 6; ++k) {
+        const int i = ii[k], j = jj[k];
+        c[k] = (sym_at(a, i, 0) * sym_at(b, 0, j) + sym_at(a, i, 1) * sym_at(b, 1, j)) + sym_at(a, i, 2) * sym_at(b, 2, j);
+    }
+}
+// Leading principal minors of a symmetric matrix all positive (Sylvester):
+// the matrix is positive definite.
+__device__ __forceinline__ bool sym_positive_definite(const float a[6]) {
+    const float m2 = a[0] * a[3] - a[1] * a[1];
+    const float m3 = a[0] * (a[3] * a[5] - a[4] * a[4]) - a[1] * (a[1] * a[5] - a[4] * a[2]) +
+                     a[2] * (a[1] * a[4] - a[3] * a[2]);
+    return a[0] > 0.0f && m2 > 0.0f && m3 > 0.0f;
+}
+
+// The elastic case of updateDeformationGradient (src/material_point3d.cu:
+// 53-70).  When every singular value of the trial Fn lies inside
+// [1 - theta_c, 1 + theta_s] and det Fn > 0, the clamp changes nothing, so
+// the reference's split gives Fe = Fn and leaves Fp as it is (exact
+// arithmetic).  The singular values are the square roots of the eigenvalues
+// of b = Fn Fn^T = I + E, so the test is that (hi^2 - 1) I - E and
+// E + (1 - lo^2) I are positive definite.  The bounds are also capped at
+// eigenvalues of E within +-0.05 so that the series below is accurate to
+// ~3e-10 (materials with wider thresholds use the SVD path beyond that).
+// Returns E (symmetric) and J = det Fn for the stress.
+__device__ __forceinline__ bool elastic_trial(const float Fn[3][3], const MaterialDev& mat, float E[6], float& J) {
+    const int ii[6] = {0, 0, 0, 1, 1, 2}, jj[6] = {0, 1, 2, 1, 2, 2};
+#pragma unroll
+    for (int k = 0; k < 6; ++k) {
+        const int i = ii[k], j = jj[k];
+        E[k] = (Fn[i][0] * Fn[j][0] + Fn[i][1] * Fn[j][1]) + Fn[i][2] * Fn[j][2] - (i == j ? 1.0f : 0.0f);
+    }
+    J = m3_det(Fn);
+    const float hi = 1.0f + mat.crit_s, lo = 1.0f - mat.crit_c;
+    const float ah = fminf(hi * hi - 1.0f, 0.05f);
+    const float al = lo > 0.0f ? fminf(1.0f - lo * lo, 0.05f) : 0.05f;
+    const float A[6] = {ah - E[0], -E[1], -E[2], ah - E[3], -E[4], ah - E[5]};
+    const float C[6] = {E[0] + al, E[1], E[2], E[3] + al, E[4], E[5] + al};
+    return J > 0.0f && sym_positive_definite(A) && sym_positive_definite(C);
+}
+
+// Fixed-corotated -V0 P(Fe) Fe^T (src/material_point3d.cu:110-128, with
+// plasticLame :158-162) for Fe = Fn of an elastic step:
+//   2 mu (Fe - R) Fe^T + la (J - 1) J I,  (Fe - R) Fe^T = b - sqrt(b)
+// because R Fe^T = U diag(s) U^T = sqrt(Fe Fe^T) for Fe = U diag(s) V^T.
+// With b = I + E, b - sqrt(b) is the series of 1 + x - sqrt(1 + x):
+//   E/2 + E^2/8 - E^3/16 + 5 E^4/128 - 7 E^5/256 + O(E^6)
+// (Horner form; the next term is below 21 |E|^6 / 1024 <= 3.2e-10).
+__device__ __forceinline__ void elastic_fc_stress(const float E[6], float J, const float Fp[3][3], float mu0,
+                                                  float lam0, float V0, const MaterialDev& mat, float tau[6]) {
+    const float Jp = m3_det(Fp);
+    const float factor = __expf(mat.hardening * (1.0f - Jp));
+    const float mu = mu0 * factor, la = lam0 * factor;
+    const bool dg[6] = {true, false, false, true, false, true};
+    float M[6], T[6];
+#pragma unroll
+    for (int k = 0; k < 6; ++k) M[k] = (dg[k] ? 5.0f / 128.0f : 0.0f) - (7.0f / 256.0f) * E[k];
+    const float coef[3] = {-1.0f / 16.0f, 1.0f / 8.0f, 1.0f / 2.0f};
+#pragma unroll
+    for (int r = 0; r < 3; ++r) {
+        sym_mul(E, M, T);
+#pragma unroll
+        for (int k = 0; k < 6; ++k) M[k] = T[k] + (dg[k] ? coef[r] : 0.0f);
+    }
+    sym_mul(E, M, T);  // b - sqrt(b)
+    const float c = la * (J - 1.0f) * J;
+#pragma unroll
+    for (int k = 0; k < 6; ++k) tau[k] = -(V0 * ((2.0f * mu) * T[k] + (dg[k] ? c : 0.0f)));
+}
+
+// updateDeformationGradient, src/material_point3d.cu:53-70, general case:
+// SVD of the trial Fn, clamp singular values to [1-theta_c, 1+theta_s],
+// split into Fe and Fp.  Returns the SVD of the new Fe (U, clamped s, V)
+// for the stress.  The reference's Fp = V diag(1/sc) U^T (Fn Fp) is
+// evaluated as V diag(s/sc) V^T Fp, using U^T Fn = diag(s) V^T for the SVD
+// Fn = U diag(s) V^T: the same matrix in exact arithmetic, two 3x3 products
+// instead of four, and Fn Fp need not be kept through the SVD.
+__device__ __forceinline__ void update_deformation(const float Fn[3][3], float Fe[3][3], float Fp[3][3],
+                                                   const MaterialDev& mat, float U[3][3], float s[3],
+                                                   float V[3][3]) {
     svd3(Fn, U, s, V);
     const float hi = 1.0f + mat.crit_s, lo = 1.0f - mat.crit_c;
     float r[3];
@@ -528,7 +613,14 @@ __global__ void __launch_bounds__(64) __attribute__((amdgpu_waves_per_eu(P2GWave
         item_scene<BATCH>(S0, P0, I, tab, it, S, P, pbase, koff);
         const int64_t N = P.n;
         ItemGeom g = item_geom(S, I, it, nitems, koff);
-        if (g.escape) continue;  // no stencil node inside the grid
+        if (g.escape) {  // no stencil node inside the grid
+            if (I.snap_k)
+                for (uint32_t i = g.s + lane; i < g.e; i += 64) {
+                    I.snap_k[i] = I.sorted_keys[i];
+                    I.snap_p[i] = I.perm[i];
+                }
+            continue;
+        }
         for (int q = lane; q < NF * TN; q += 64) tile[q] = 0.0f;
         __syncthreads();
         for (uint32_t base = g.s; base < g.e; base += 64) {
@@ -540,7 +632,12 @@ __global__ void __launch_bounds__(64) __attribute__((amdgpu_waves_per_eu(P2GWave
             const uint32_t prev = __shfl_up(key, 1, 64);
             const SegPlan sp = seg_plan(__ballot(lane == 0 || prev != key), active, lane);
             const bool writer = sp.writer;
-            const uint32_t p = I.perm[ic] - pbase;
+            const uint32_t praw = I.perm[ic];
+            const uint32_t p = praw - pbase;
+            if (I.snap_k && active) {  // the sorted order, kept for the next step's incremental sort
+                I.snap_k[i] = key;
+                I.snap_p[i] = praw;
+            }
             const float x0 = P.x[p], x1 = P.x[N + p], x2 = P.x[2 * N + p];
             int bx, by, bz;
             float Nx[W], Ny[W], Nz[W], dNx[W], dNy[W], dNz[W];
@@ -702,6 +799,29 @@ void launch_p2g(const SceneDev& S, const ParticlesDev& P, const ItemsDev& I, int
             p2g_dispatch_apic<MPM_INTERP_QUADRATIC_BSPLINE>(S, P, I, tab, apic, mode, grid_wgs, st);
             break;
         default: p2g_dispatch_apic<MPM_INTERP_CUBIC_BSPLINE>(S, P, I, tab, apic, mode, grid_wgs, st); break;
+    }
+}
+
+// A particle whose cell key changed in this step goes on the move list of
+// the incremental sort: one counter atomic per wave, then each moved lane
+// writes (new key, slot, old sorted position) at its rank.  Entries past
+// kMoveMax are counted but not stored (the next step then sorts in full).
+__device__ __forceinline__ void record_move(const ItemsDev& I, bool moved, uint32_t key, uint32_t slot,
+                                            uint32_t pos) {
+    const uint64_t m = __ballot(moved);
+    if (m == 0ull) return;
+    const int lane = threadIdx.x & 63;
+    const int leader = __ffsll((unsigned long long)m) - 1;
+    uint32_t base = 0;
+    if (lane == leader) base = atomicAdd(&I.plan[kPlanMoved], (uint32_t)__popcll(m));
+    base = __shfl(base, leader, 64);
+    if (moved) {
+        const uint32_t idx = base + (uint32_t)__popcll(m & ((1ull << lane) - 1ull));
+        if (idx < (uint32_t)kMoveMax) {
+            I.moves->key[idx] = key;
+            I.moves->slot[idx] = slot;
+            I.moves->pos[idx] = pos;
+        }
     }
 }
 
@@ -1037,8 +1157,12 @@ __global__ void __launch_bounds__(WG) __attribute__((amdgpu_waves_per_eu(G2PWave
                 P.x[p] = y0;
                 P.x[N + p] = y1;
                 P.x[2 * N + p] = y2;
-                if (next_keys)
-                    next_keys[pbase + p] = koff + cell_key(S, y0, y1, y2, Stencil<INTERP>::SHIFT, Stencil<INTERP>::W);
+                if (next_keys) {
+                    const uint32_t knew = koff + cell_key(S, y0, y1, y2, Stencil<INTERP>::SHIFT, Stencil<INTERP>::W);
+                    next_keys[pbase + p] = knew;
+                    if (I.plan)  // uniform: the fused step of one large scene (the key it was sorted by)
+                        record_move(I, knew != I.sorted_keys[i], knew, pbase + p, i);
+                }
             }
             if constexpr (!PSTAGE) {
                 load_m3(P.Fe, N, p, Fe);
@@ -1048,17 +1172,400 @@ __global__ void __launch_bounds__(WG) __attribute__((amdgpu_waves_per_eu(G2PWave
                 V0 = P.V0[p];
             }
             const MaterialDev mat = P.mats[P.mat[p]];
-            float U[3][3], sv[3], V[3][3], tau[6];
-            update_deformation(Fe, Fp, grad, S.dt, mat, U, sv, V);
-            // stress for the next P2G from the same decomposition
-            stress_from_svd(U, sv, V, Fp, mu0, lam0, V0, mat, tau, Fe);
-            store_m3(P.Fe, N, p, Fe);
-            store_m3(P.Fp, N, p, Fp);
+            float Fn[3][3], E[6], J, tau[6];
+            trial_deformation(Fe, grad, S.dt, Fn);
+            const bool elastic = mat.model == MPM_MODEL_FIXED_COROTATED && elastic_trial(Fn, mat, E, J);
+            if (__ballot(!elastic) == 0ull) {
+                // every particle of the wave stays inside its thresholds:
+                // Fe = Fn, Fp unchanged (not stored), stress without an SVD
+                elastic_fc_stress(E, J, Fp, mu0, lam0, V0, mat, tau);
+                store_m3(P.Fe, N, p, Fn);
+            } else {
+                float U[3][3], sv[3], V[3][3];
+                update_deformation(Fn, Fe, Fp, mat, U, sv, V);
+                // stress for the next P2G from the same decomposition
+                stress_from_svd(U, sv, V, Fp, mu0, lam0, V0, mat, tau, Fe);
+                store_m3(P.Fe, N, p, Fe);
+                store_m3(P.Fp, N, p, Fp);
+            }
 #pragma unroll
             for (int k = 0; k < 6; ++k) P.tau[(int64_t)k * N + p] = tau[k];
         }
         __syncthreads();
     }
+}
+
+// ---------------------------------------------------------- fused G2P+P2G
+// APIC sums over the W^3 stencil of the v* tile: v = sum w v*, grad v =
+// sum v* (grad w)^T, B = sum w v* (x_i - x_p)^T, as nested z/y/x sums (the
+// gather of k_g2p's APIC path).
+template <int W, int T, int TN>
+__device__ __forceinline__ void apic_gather(const float* tq, const float (&Nx)[W], const float (&Ny)[W],
+                                            const float (&Nz)[W], const float (&dNx)[W], const float (&dNy)[W],
+                                            const float (&dNz)[W], const float (&dxn)[3][W], float (&v)[3],
+                                            float (&grad)[3][3], float (&Bm)[3][3]) {
+    float zdx[W], ydx[W];
+#pragma unroll
+    for (int j = 0; j < W; ++j) {
+        zdx[j] = Nz[j] * dxn[2][j];
+        ydx[j] = Ny[j] * dxn[1][j];
+    }
+#pragma unroll
+    for (int a = 0; a < W; ++a) {
+        float Y0[3] = {0.0f, 0.0f, 0.0f}, Yd[3] = {0.0f, 0.0f, 0.0f}, Yz[3] = {0.0f, 0.0f, 0.0f};
+        float Yx[3] = {0.0f, 0.0f, 0.0f}, Yzx[3] = {0.0f, 0.0f, 0.0f};
+#pragma unroll
+        for (int b = 0; b < W; ++b) {
+            float Z0[3] = {0.0f, 0.0f, 0.0f}, Zd[3] = {0.0f, 0.0f, 0.0f}, Zx[3] = {0.0f, 0.0f, 0.0f};
+#pragma unroll
+            for (int c = 0; c < W; ++c) {
+                const int q = (c * T + b) * T + a;
+                const float wv[3] = {tq[q], tq[TN + q], tq[2 * TN + q]};
+#pragma unroll
+                for (int k = 0; k < 3; ++k) {
+                    Z0[k] += Nz[c] * wv[k];
+                    Zd[k] += dNz[c] * wv[k];
+                    Zx[k] += zdx[c] * wv[k];
+                }
+            }
+#pragma unroll
+            for (int k = 0; k < 3; ++k) {
+                Y0[k] += Ny[b] * Z0[k];
+                Yd[k] += dNy[b] * Z0[k];
+                Yz[k] += Ny[b] * Zd[k];
+                Yx[k] += ydx[b] * Z0[k];
+                Yzx[k] += Ny[b] * Zx[k];
+            }
+        }
+        const float xa = Nx[a], dxa = dNx[a];
+#pragma unroll
+        for (int k = 0; k < 3; ++k) {
+            v[k] += xa * Y0[k];
+            grad[k][0] += dxa * Y0[k];
+            grad[k][1] += xa * Yd[k];
+            grad[k][2] += xa * Yz[k];
+            Bm[k][0] += (xa * dxn[0][a]) * Y0[k];
+            Bm[k][1] += xa * Yx[k];
+            Bm[k][2] += xa * Yzx[k];
+        }
+    }
+}
+
+// One particle's folded-grid contributions to the nodes of its stencil
+// (particlesToGrid, src/apic_mpm_solver3d.cu:245-304, with p + dt f summed
+// per node): mass m w and m w (v + C (x_i - x_p)) + vs grad w, vs = dt tau.
+// Tensor-product factors as in k_p2g.  Each node's four values are summed
+// over the lanes of the particle's cell (segmented scan) and the cell's
+// last lane adds them to the tile at tq (the stencil's first node).
+template <int W, int T, int TN>
+__device__ __forceinline__ void folded_cell_sums(float* tq, const SegPlan& sp, float m, const float (&vel)[3],
+                                                 const float (&Cm)[3][3], const float (&vs)[3][3],
+                                                 const float (&dxn)[3][W], const float (&Nx)[W],
+                                                 const float (&Ny)[W], const float (&Nz)[W], const float (&dNx)[W],
+                                                 const float (&dNy)[W], const float (&dNz)[W]) {
+    float yz[W][W], dyz[W][W], ydz[W][W];
+#pragma unroll
+    for (int b = 0; b < W; ++b)
+#pragma unroll
+        for (int c = 0; c < W; ++c) {
+            yz[b][c] = Ny[b] * Nz[c];
+            dyz[b][c] = dNy[b] * Nz[c];
+            ydz[b][c] = Ny[b] * dNz[c];
+        }
+#pragma unroll
+    for (int a = 0; a < W; ++a) {
+        const float ma = m * Nx[a];
+        float F0[3], F1[3], F2[3], cx[3];
+#pragma unroll
+        for (int k = 0; k < 3; ++k) {
+            F0[k] = vs[k][0] * dNx[a];
+            F1[k] = vs[k][1] * Nx[a];
+            F2[k] = vs[k][2] * Nx[a];
+            cx[k] = vel[k] + Cm[k][0] * dxn[0][a];
+        }
+#pragma unroll
+        for (int b = 0; b < W; ++b)
+#pragma unroll
+            for (int c = 0; c < W; ++c) {
+                const float mip = ma * yz[b][c];
+                float v[4];
+                v[0] = mip;
+#pragma unroll
+                for (int k = 0; k < 3; ++k) {
+                    const float ck = cx[k] + (Cm[k][1] * dxn[1][b] + Cm[k][2] * dxn[2][c]);
+                    const float fk = (F0[k] * yz[b][c] + F1[k] * dyz[b][c]) + F2[k] * ydz[b][c];
+                    v[1 + k] = fmaf(mip, ck, fk);
+                }
+                seg_scan<4>(v, sp);
+                const int q = (c * T + b) * T + a;
+                if (sp.writer) {
+                    float o[4];
+#pragma unroll
+                    for (int f = 0; f < 4; ++f) o[f] = tq[f * TN + q];
+#pragma unroll
+                    for (int f = 0; f < 4; ++f) tq[f * TN + q] = o[f] + v[f];
+                }
+            }
+    }
+}
+
+// A particle that changed cell: its folded contributions straight to the
+// grid with global atomics (its stencil is not the item's tile).
+template <int INTERP>
+__device__ __forceinline__ void scatter_direct(const SceneDev& S, float* grid, uint32_t* touched, float x0, float x1,
+                                               float x2, float m, const float (&vel)[3], const float (&Cm)[3][3],
+                                               const float (&vs)[3][3]) {
+    constexpr int W = Stencil<INTERP>::W;
+    int bx, by, bz;
+    float Nx[W], Ny[W], Nz[W], dNx[W], dNy[W], dNz[W];
+    axis_weights<INTERP>(x0, S.ox, S.h, S.inv_h, bx, Nx, dNx);
+    axis_weights<INTERP>(x1, S.oy, S.h, S.inv_h, by, Ny, dNy);
+    axis_weights<INTERP>(x2, S.oz, S.h, S.inv_h, bz, Nz, dNz);
+    for (int a = 0; a < W; ++a)
+        for (int b = 0; b < W; ++b)
+            for (int c = 0; c < W; ++c) {
+                const int gi = bx + a, gj = by + b, gk = bz + c;
+                if (!in_grid(S, gi, gj, gk)) continue;
+                const float w = (Nx[a] * Ny[b]) * Nz[c];
+                const float gw[3] = {dNx[a] * Ny[b] * Nz[c], Nx[a] * dNy[b] * Nz[c], Nx[a] * Ny[b] * dNz[c]};
+                const float dx[3] = {(S.ox + S.h * (float)gi) - x0, (S.oy + S.h * (float)gj) - x1,
+                                     (S.oz + S.h * (float)gk) - x2};
+                const float mip = m * w;
+                const int64_t ni = node_index(S, gi, gj, gk);
+                unsafeAtomicAdd(&grid[ni], mip);
+#pragma unroll
+                for (int k = 0; k < 3; ++k) {
+                    const float ck = vel[k] + ((Cm[k][0] * dx[0] + Cm[k][1] * dx[1]) + Cm[k][2] * dx[2]);
+                    const float fk = (vs[k][0] * gw[0] + vs[k][1] * gw[1]) + vs[k][2] * gw[2];
+                    unsafeAtomicAdd(&grid[(int64_t)(GF_VEL + k) * S.plane + ni], fmaf(mip, ck, fk));
+                }
+                touched[storage_block(S, gi, gj, gk)] = 1u;
+            }
+}
+
+// gridToParticles of step n fused with particlesToGrid of step n+1 (APIC,
+// folded grid, one scene): the item's v* tile is gathered as in k_g2p
+// (grid update fused into the tile load, F update, stress, advection, next
+// key); then, with the particle still in registers, its contributions to
+// the next step's grid (ogrid/otouched, the other grid buffer) are summed
+// per cell and added into the wave's own LDS tile, which the item flushes
+// with one atomic per non-zero node value.  The reference runs the two
+// transfers as separate passes with a fresh grid between them
+// (src/apic_mpm_solver3d.cu:99-124); the sums are the same up to float
+// order.  Lanes are consecutive sorted particles, so a cell's lanes are
+// adjacent; a particle whose new key differs from its sorted key (a few per
+// 10^4) keeps its lane in the cell's segment with zero contribution and
+// scatters itself with global atomics.  The stress of the new state is used
+// at once, so it is not cached.
+template <int INTERP>
+#ifndef MPM_G2P2G_WAVES
+#define MPM_G2P2G_WAVES 3
+#endif
+// 3 waves per SIMD: the kernel needs ~168 VGPRs (4 waves would spill ~34)
+// and its 24 KB of LDS per workgroup allow 6 workgroups (12 waves) per CU.
+__global__ void __launch_bounds__(128) __attribute__((amdgpu_waves_per_eu(MPM_G2P2G_WAVES)))
+k_g2p2g(SceneDev S, ParticlesDev P, ItemsDev I, float* __restrict__ ogrid, uint32_t* __restrict__ otouched,
+        int pcollide, uint32_t* __restrict__ next_keys) {
+    constexpr int WG = 128;
+    constexpr int W = Stencil<INTERP>::W;
+    constexpr int T = 4 + W - 1;
+    constexpr int TN = T * T * T;
+    constexpr int TR = (TN + WG - 1) / WG;
+    constexpr int PW = 21;
+    __shared__ float tile[3 * TN];         // v* of this step
+    __shared__ float stage[TR * 4 * WG];   // the next item's folded tile in flight
+    __shared__ float pstage[PW * WG];      // Fe, Fp, mu0, lam0, V0 of each lane's particle
+    __shared__ float otile[2 * 4 * TN];    // next step's sums, one tile per wave
+    const int wave = threadIdx.x >> 6, lane = threadIdx.x & 63;
+    const int64_t N = P.n;
+    const uint32_t nitems = *I.count;
+    float* const ot = otile + wave * (4 * TN);
+    ItemGeom gnext{};
+    if (blockIdx.x < nitems) {
+        gnext = item_geom(S, I, blockIdx.x, nitems, 0);
+        fetch_folded_tile<T, TR, WG>(S, gnext, stage);
+    }
+    for (uint32_t it = blockIdx.x; it < nitems; it += gridDim.x) {
+        const ItemGeom g = gnext;
+        if (!g.escape) commit_folded_tile<T, TR, WG>(S, g, stage, tile);
+        for (int q = threadIdx.x; q < 2 * 4 * TN; q += WG) otile[q] = 0.0f;
+        __syncthreads();
+        const uint32_t itn = it + gridDim.x;
+        if (itn < nitems) {
+            gnext = item_geom(S, I, itn, nitems, 0);
+            fetch_folded_tile<T, TR, WG>(S, gnext, stage);
+        }
+        // uniform trip count: every lane takes part in the cell sums
+        for (uint32_t base = g.s; base < g.e; base += WG) {
+            const uint32_t i = base + threadIdx.x;
+            const bool active = i < g.e;
+            float xn[3] = {0.0f, 0.0f, 0.0f}, vn[3] = {0.0f, 0.0f, 0.0f}, Cn[3][3], vs[3][3], m = 0.0f;
+            uint32_t kold = 0xFFFFFFFFu, knew = 0xFFFFFFFFu;
+#pragma unroll
+            for (int r = 0; r < 3; ++r)
+#pragma unroll
+                for (int c = 0; c < 3; ++c) { Cn[r][c] = 0.0f; vs[r][c] = 0.0f; }
+            if (active) {
+                // ---- step n: gridToParticles (src/apic_mpm_solver3d.cu:326-388)
+                const uint32_t p = I.perm[i];
+                kold = I.sorted_keys[i];
+                if (I.snap_k) {  // the sorted order, kept for the next step's incremental sort
+                    I.snap_k[i] = kold;
+                    I.snap_p[i] = p;
+                }
+                float* const pst = pstage + wave * (PW * 64);
+#pragma unroll 1
+                for (int f = 0; f < 9; ++f) {
+                    __builtin_amdgcn_global_load_lds((const void*)(P.Fe + (int64_t)f * N + p), (lds_void*)(pst + f * 64),
+                                                     4, 0, 0);
+                    __builtin_amdgcn_global_load_lds((const void*)(P.Fp + (int64_t)f * N + p),
+                                                     (lds_void*)(pst + (9 + f) * 64), 4, 0, 0);
+                }
+                __builtin_amdgcn_global_load_lds((const void*)(P.mu0 + p), (lds_void*)(pst + 18 * 64), 4, 0, 0);
+                __builtin_amdgcn_global_load_lds((const void*)(P.lam0 + p), (lds_void*)(pst + 19 * 64), 4, 0, 0);
+                __builtin_amdgcn_global_load_lds((const void*)(P.V0 + p), (lds_void*)(pst + 20 * 64), 4, 0, 0);
+                const float x0 = P.x[p], x1 = P.x[N + p], x2 = P.x[2 * N + p];
+                m = P.m[p];
+                float grad[3][3], Bm[3][3], v[3] = {0.0f, 0.0f, 0.0f};
+#pragma unroll
+                for (int r = 0; r < 3; ++r)
+#pragma unroll
+                    for (int c = 0; c < 3; ++c) { grad[r][c] = 0.0f; Bm[r][c] = 0.0f; }
+                if (!g.escape) {
+                    int bx, by, bz;
+                    float Nx[W], Ny[W], Nz[W], dNx[W], dNy[W], dNz[W], dxn[3][W];
+                    axis_weights<INTERP>(x0, S.ox, S.h, S.inv_h, bx, Nx, dNx);
+                    axis_weights<INTERP>(x1, S.oy, S.h, S.inv_h, by, Ny, dNy);
+                    axis_weights<INTERP>(x2, S.oz, S.h, S.inv_h, bz, Nz, dNz);
+#pragma unroll
+                    for (int j = 0; j < W; ++j) {
+                        dxn[0][j] = (S.ox + S.h * (float)(bx + j)) - x0;
+                        dxn[1][j] = (S.oy + S.h * (float)(by + j)) - x1;
+                        dxn[2][j] = (S.oz + S.h * (float)(bz + j)) - x2;
+                    }
+                    const float* const tq = tile + (((bz - g.oz) * T + (by - g.oy)) * T + (bx - g.ox));
+                    apic_gather<W, T, TN>(tq, Nx, Ny, Nz, dNx, dNy, dNz, dxn, v, grad, Bm);
+                }
+                store_m3(P.B, N, p, Bm);
+                if (pcollide) boundary_collision(S, x0 + S.dt * v[0], x1 + S.dt * v[1], x2 + S.dt * v[2], v[0], v[1], v[2]);
+                P.v[p] = v[0];
+                P.v[N + p] = v[1];
+                P.v[2 * N + p] = v[2];
+                xn[0] = x0 + S.dt * v[0];
+                xn[1] = x1 + S.dt * v[1];
+                xn[2] = x2 + S.dt * v[2];
+                P.x[p] = xn[0];
+                P.x[N + p] = xn[1];
+                P.x[2 * N + p] = xn[2];
+                knew = cell_key(S, xn[0], xn[1], xn[2], Stencil<INTERP>::SHIFT, Stencil<INTERP>::W);
+                next_keys[p] = knew;
+                if (I.plan) record_move(I, knew != kold, knew, p, i);
+                // updateDeformationGradient and the stress of the new state
+                float Fe[3][3], Fp[3][3];
+                asm volatile("s_waitcnt vmcnt(0)" ::: "memory");  // staged words have landed
+                float* const q = pst + lane;
+#pragma unroll
+                for (int f = 0; f < 9; ++f) {
+                    Fe[f / 3][f % 3] = q[f * 64];
+                    Fp[f / 3][f % 3] = q[(9 + f) * 64];
+                }
+                const float mu0 = q[18 * 64], lam0 = q[19 * 64], V0 = q[20 * 64];
+                // the lane's staging words are free now: v, B and the new
+                // position wait there through the F update (fewer registers)
+#pragma unroll
+                for (int k = 0; k < 3; ++k) {
+                    q[k * 64] = v[k];
+                    q[(3 + k) * 64] = xn[k];
+                }
+#pragma unroll
+                for (int f = 0; f < 9; ++f) q[(6 + f) * 64] = Bm[f / 3][f % 3];
+                const MaterialDev mat = P.mats[P.mat[p]];
+                float Fn[3][3], E[6], J, tau[6];
+                trial_deformation(Fe, grad, S.dt, Fn);
+                const bool elastic = mat.model == MPM_MODEL_FIXED_COROTATED && elastic_trial(Fn, mat, E, J);
+                if (__ballot(!elastic) == 0ull) {
+                    elastic_fc_stress(E, J, Fp, mu0, lam0, V0, mat, tau);
+                    store_m3(P.Fe, N, p, Fn);
+                } else {
+                    float U[3][3], sv[3], V[3][3];
+                    update_deformation(Fn, Fe, Fp, mat, U, sv, V);
+                    stress_from_svd(U, sv, V, Fp, mu0, lam0, V0, mat, tau, Fe);
+                    store_m3(P.Fe, N, p, Fe);
+                    store_m3(P.Fp, N, p, Fp);
+                }
+#pragma unroll
+                for (int k = 0; k < 3; ++k) {
+                    vn[k] = q[k * 64];
+                    xn[k] = q[(3 + k) * 64];
+                }
+#pragma unroll
+                for (int f = 0; f < 9; ++f) Cn[f / 3][f % 3] = q[(6 + f) * 64] * S.dinv;
+                const float t[6] = {S.dt * tau[0], S.dt * tau[1], S.dt * tau[2], S.dt * tau[3], S.dt * tau[4],
+                                    S.dt * tau[5]};
+                vs[0][0] = t[0]; vs[0][1] = t[1]; vs[0][2] = t[2];
+                vs[1][0] = t[1]; vs[1][1] = t[3]; vs[1][2] = t[4];
+                vs[2][0] = t[2]; vs[2][1] = t[4]; vs[2][2] = t[5];
+            }
+            // ---- step n+1: particlesToGrid into the next grid
+            const bool moved = active && knew != kold;
+            if (moved) {
+                scatter_direct<INTERP>(S, ogrid, otouched, xn[0], xn[1], xn[2], m, vn, Cn, vs);
+#pragma unroll
+                for (int r = 0; r < 3; ++r)
+#pragma unroll
+                    for (int c = 0; c < 3; ++c) vs[r][c] = 0.0f;  // no force term in the old cell's sums
+            }
+            if (!g.escape) {
+                // segments by the sorted key: a moved lane stays in its old
+                // cell's segment with zero weight
+                const bool inscan = active && !moved;
+                const uint32_t skey = active ? kold : 0xFFFFFFFFu;
+                const uint32_t prev = __shfl_up(skey, 1, 64);
+                const SegPlan sp = seg_plan(__ballot(lane == 0 || prev != skey), active, lane);
+                int bx = 0, by = 0, bz = 0;
+                float Nx[W], Ny[W], Nz[W], dNx[W], dNy[W], dNz[W], dxn[3][W];
+                axis_weights<INTERP>(inscan ? xn[0] : 0.0f, S.ox, S.h, S.inv_h, bx, Nx, dNx);
+                axis_weights<INTERP>(inscan ? xn[1] : 0.0f, S.oy, S.h, S.inv_h, by, Ny, dNy);
+                axis_weights<INTERP>(inscan ? xn[2] : 0.0f, S.oz, S.h, S.inv_h, bz, Nz, dNz);
+#pragma unroll
+                for (int j = 0; j < W; ++j) {
+                    dxn[0][j] = (S.ox + S.h * (float)(bx + j)) - xn[0];
+                    dxn[1][j] = (S.oy + S.h * (float)(by + j)) - xn[1];
+                    dxn[2][j] = (S.oz + S.h * (float)(bz + j)) - xn[2];
+                }
+                // the segment's cell in the tile, from its key (cell bits)
+                const int cxk = (int)(skey & 3u), cyk = (int)((skey >> 2) & 3u), czk = (int)((skey >> 4) & 3u);
+                float* const tq = ot + ((czk * T + cyk) * T + cxk);
+                folded_cell_sums<W, T, TN>(tq, sp, inscan ? m : 0.0f, vn, Cn, vs, dxn, Nx, Ny, Nz, dNx, dNy, dNz);
+            }
+        }
+        __syncthreads();
+        if (!g.escape) {
+            for (int q = threadIdx.x; q < TN; q += WG) {
+                const int lx = q % T, ly = (q / T) % T, lz = q / (T * T);
+                const int gi = g.ox + lx, gj = g.oy + ly, gk = g.oz + lz;
+                if (!in_grid(S, gi, gj, gk)) continue;
+                const int64_t ni = node_index(S, gi, gj, gk);
+                bool any = false;
+#pragma unroll
+                for (int f = 0; f < 4; ++f) {
+                    const float val = otile[f * TN + q] + otile[4 * TN + f * TN + q];
+                    if (val != 0.0f) {
+                        unsafeAtomicAdd(&ogrid[(int64_t)f * S.plane + ni], val);
+                        any = true;
+                    }
+                }
+                if (any) otouched[storage_block(S, gi, gj, gk)] = 1u;
+            }
+        }
+        __syncthreads();
+    }
+}
+
+void launch_g2p2g(const SceneDev& S, const ParticlesDev& P, const ItemsDev& I, int pcollide, float* ogrid,
+                  uint32_t* otouched, uint32_t* next_keys, hipStream_t st) {
+    if (I.n <= 0) return;
+    MPM_PERSISTENT(k_g2p2g<MPM_INTERP_QUADRATIC_BSPLINE>, 128, 0, st, S, P, I, ogrid, otouched, pcollide, next_keys);
 }
 
 template <int INTERP, int APIC, int SRC, bool BATCH>

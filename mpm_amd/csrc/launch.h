@@ -21,13 +21,27 @@ struct ProfileHook {
 
 // ---- sort.hip
 int64_t scan_temp_words(int64_t n);
-void exclusive_scan(uint32_t* data, int64_t n, uint32_t* temp, uint32_t* total, hipStream_t st);
+// plan (device, may be null): every launch returns at once unless
+// plan[kPlanMode] == kPlanFull.
+void exclusive_scan(uint32_t* data, int64_t n, uint32_t* temp, uint32_t* total, hipStream_t st,
+                    const uint32_t* plan = nullptr);
 int64_t radix_tiles(int64_t n);
 int64_t radix_hist_words(int64_t n);
 int radix_passes(int key_bits, int64_t n);
 int radix_sort_pairs(const uint32_t* kin, const uint32_t* vin, uint32_t* ka, uint32_t* va, uint32_t* kb, uint32_t* vb,
                      int64_t n, int key_bits, uint32_t* hist, uint32_t* scan_temp, hipStream_t st,
-                     const ProfileHook& prof);
+                     const ProfileHook& prof, const uint32_t* plan = nullptr);
+
+// ---- incsort.hip
+// plan words: reset the move counter and set whether the snapshot is the
+// previous step's (0 forces the full sort next).
+void launch_inc_set(uint32_t* plan, uint32_t snap_ok, hipStream_t st);
+// The step's choice between the incremental update and the full sort.
+void launch_inc_plan(uint32_t* plan, hipStream_t st);
+// The incremental update (returns at once unless the plan chose it): the
+// snapshot (sorted by the previous keys) and the move list -> out_k/out_p.
+void launch_inc_sort(const uint32_t* plan, const MoveList& L, const uint32_t* snap_k, const uint32_t* snap_p,
+                     int64_t n, uint32_t* out_k, uint32_t* out_p, hipStream_t st);
 
 // ---- kernels.hip
 // Persistent launch width of the item kernels.
@@ -63,6 +77,12 @@ void launch_p2g(const SceneDev& S, const ParticlesDev& P, const ItemsDev& I, int
 void launch_g2p(const SceneDev& S, const ParticlesDev& P, const ItemsDev& I, int interp, int apic, int src,
                 int advect, int pcollide, int grid_wgs, hipStream_t st, uint32_t* next_keys = nullptr,
                 const SceneEntry* tab = nullptr, int wg = 64);
+
+// gridToParticles of this step fused with particlesToGrid of the next (APIC,
+// folded grid, one scene, quadratic stencil): gathers from S.grid and
+// scatters into ogrid / otouched (a second grid of the same layout, clear).
+void launch_g2p2g(const SceneDev& S, const ParticlesDev& P, const ItemsDev& I, int pcollide, float* ogrid,
+                  uint32_t* otouched, uint32_t* next_keys, hipStream_t st);
 
 void launch_clear(const SceneDev& S, int64_t storage_blocks, int nfields, unsigned long long* active_nodes,
                   hipStream_t st);

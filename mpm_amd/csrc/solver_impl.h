@@ -88,6 +88,7 @@ struct Solver {
     hipGraph_t graph = nullptr;
     hipGraphExec_t gexec = nullptr;
     int64_t graph_captures = 0;
+    ItemsDev graph_I{};  // the items the captured step reads (its sort buffers)
     // batch membership: while in a batch the solver works on the batch's
     // stream (own_st keeps its own)
     struct Batch* batch = nullptr;
@@ -115,7 +116,40 @@ struct Solver {
     int reorder_interval = 100;
     int64_t last_reorder = 0;
     uint32_t* reorder_scratch = nullptr;
+    // incremental cell sort of the fused step (scenes of at least
+    // kIncMinParticles): snapshot of the sorted order, plan words, move list;
+    // snap_ok: the device snapshot and move list describe the current keys
+    bool incr = false;
+    bool snap_ok = false;
+    uint32_t* snap_k = nullptr;
+    uint32_t* snap_p = nullptr;
+    uint32_t* plan = nullptr;
+    MoveList moves{};
+    MoveList* moves_dev = nullptr;  // device copy of `moves` (the kernels' view)
+    // fused transfer (APIC folded step, linear or quadratic): G2P scatters
+    // the next step's P2G into the second grid buffer (same size as the
+    // first: the two alternate), which then becomes the grid.  prescat:
+    // S.grid holds the P2G of the current particle state and grid2 the grid
+    // the last step read; grid2_dirty: grid2 holds values to clear,
+    // grid2_count: they are a step's grid (its active nodes are counted).
+    // capturing: a graph capture runs the two-kernel path.
+    bool fuse_transfer = true;
+    bool capturing = false;
+    bool prescat = false;
+    bool grid2_dirty = false;
+    bool grid2_count = false;
+    decltype(SceneDev::grid) grid2 = nullptr;
+    decltype(SceneDev::touched) touched2 = nullptr;
+    decltype(SceneDev::grid) graph_grid = nullptr;  // the grid buffer the captured step uses
 };
+
+// Below this many particles the fused step sorts in full every step: the
+// radix sort of a small scene is a few launches and the update would add
+// more than it saves.
+#ifndef MPM_INC_MIN_PARTICLES
+#define MPM_INC_MIN_PARTICLES (int64_t(1) << 20)
+#endif
+constexpr int64_t kIncMinParticles = MPM_INC_MIN_PARTICLES;
 
 
 int dalloc(Solver& s, void** p, size_t bytes);
@@ -138,6 +172,18 @@ void batch_forget(struct Batch* b, Solver* s);
 void batch_invalidate(struct Batch* b);
 // Planes of the grid that P2G writes in the fused step.
 inline bool fused_folds(const Solver& s) { return s.apic && !s.record_grid; }
+// Before anything but a fused step touches the particles or the grid: the
+// grid becomes again the one the last step read (as the two-kernel step
+// leaves it) and the next step's pre-scattered sums wait in grid2 to be
+// cleared.
+inline void unfuse(Solver& s) {
+    if (!s.prescat) return;
+    std::swap(s.S.grid, s.grid2);
+    std::swap(s.S.touched, s.touched2);
+    s.grid2_dirty = true;
+    s.grid2_count = false;  // never a step's grid
+    s.prescat = false;
+}
 
 }  // namespace host
 }  // namespace mpm

@@ -40,7 +40,9 @@ __device__ __forceinline__ uint64_t digit_match(uint32_t digit, uint64_t active)
 
 template <int BITS>
 __global__ void __launch_bounds__(kSortWG) k_radix_hist(const uint32_t* __restrict__ keys, int64_t n, int shift,
-                                                        uint32_t mask, uint32_t* __restrict__ hist, int64_t tiles) {
+                                                        uint32_t mask, uint32_t* __restrict__ hist, int64_t tiles,
+                                                        const uint32_t* __restrict__ plan) {
+    if (plan && plan[kPlanMode] != kPlanFull) return;
     constexpr int kRadix = 1 << BITS;
     __shared__ uint32_t h[kRadix];
     const int tid = threadIdx.x;
@@ -71,7 +73,8 @@ __global__ void __launch_bounds__(kSortWG) k_radix_scatter(const uint32_t* __res
                                                            uint32_t* __restrict__ keys_out,
                                                            uint32_t* __restrict__ vals_out, int64_t n, int shift,
                                                            uint32_t mask, const uint32_t* __restrict__ hist_scan,
-                                                           int64_t tiles) {
+                                                           int64_t tiles, const uint32_t* __restrict__ plan) {
+    if (plan && plan[kPlanMode] != kPlanFull) return;
     constexpr int kRadix = 1 << BITS;
     // digits per thread: 2 (9 bits), 1 (8 bits), or 1 for the first kRadix
     // threads and none for the rest (narrower digits)
@@ -212,7 +215,9 @@ __global__ void __launch_bounds__(kSortWG) k_radix_scatter(const uint32_t* __res
 // (kScanWG, kScanIPT, block_exclusive_scan: blockscan.h)
 
 __global__ void __launch_bounds__(kScanWG) k_scan_reduce(const uint32_t* __restrict__ in, int64_t n,
-                                                         uint32_t* __restrict__ partial) {
+                                                         uint32_t* __restrict__ partial,
+                                                         const uint32_t* __restrict__ plan) {
+    if (plan && plan[kPlanMode] != kPlanFull) return;
     __shared__ uint32_t ws[kScanWG / 64];
     int64_t base = (int64_t)blockIdx.x * kScanTile;
     uint32_t s = 0;
@@ -233,7 +238,8 @@ __global__ void __launch_bounds__(kScanWG) k_scan_reduce(const uint32_t* __restr
 // null, the last thread of the last tile writes the grand total.
 template <bool RAW>
 __global__ void __launch_bounds__(kScanWG) k_scan_down(const uint32_t* in, int64_t n, const uint32_t* partial_scan,
-                                                       uint32_t* out, uint32_t* total) {
+                                                       uint32_t* out, uint32_t* total, const uint32_t* plan) {
+    if (plan && plan[kPlanMode] != kPlanFull) return;
     __shared__ uint32_t ws[kScanWG / 64];
     uint32_t tile_prefix = 0;
     if (partial_scan) {
@@ -278,24 +284,25 @@ int64_t scan_temp_words(int64_t n) {
 
 // Exclusive scan in place over n words; temp holds scan_temp_words(n).
 // total (device, may be null) receives the sum.
-void exclusive_scan(uint32_t* data, int64_t n, uint32_t* temp, uint32_t* total, hipStream_t st) {
+void exclusive_scan(uint32_t* data, int64_t n, uint32_t* temp, uint32_t* total, hipStream_t st,
+                    const uint32_t* plan) {
     if (n <= 0) return;
     if (n <= kScanTile) {
         hipLaunchKernelGGL(k_scan_down<false>, dim3(1), dim3(kScanWG), 0, st, data, n, (const uint32_t*)nullptr, data,
-                           total);
+                           total, plan);
         return;
     }
     int64_t tiles = (n + kScanTile - 1) / kScanTile;
     uint32_t* partial = temp;
-    hipLaunchKernelGGL(k_scan_reduce, dim3((unsigned)tiles), dim3(kScanWG), 0, st, data, n, partial);
+    hipLaunchKernelGGL(k_scan_reduce, dim3((unsigned)tiles), dim3(kScanWG), 0, st, data, n, partial, plan);
     if (tiles <= 2048) {  // each tile sums at most 2047 tile totals itself
         hipLaunchKernelGGL(k_scan_down<true>, dim3((unsigned)tiles), dim3(kScanWG), 0, st, data, n,
-                           (const uint32_t*)partial, data, total);
+                           (const uint32_t*)partial, data, total, plan);
         return;
     }
-    exclusive_scan(partial, tiles, temp + tiles, nullptr, st);
+    exclusive_scan(partial, tiles, temp + tiles, nullptr, st, plan);
     hipLaunchKernelGGL(k_scan_down<false>, dim3((unsigned)tiles), dim3(kScanWG), 0, st, data, n,
-                       (const uint32_t*)partial, data, total);
+                       (const uint32_t*)partial, data, total, plan);
 }
 
 int64_t radix_tiles(int64_t n) { return (n + kSortTile - 1) / kSortTile; }
@@ -325,7 +332,7 @@ static int digit_bits(int key_bits, int64_t n, int pass) {
 // returns 0 if the result is in (ka,va), 1 if in (kb,vb).
 int radix_sort_pairs(const uint32_t* kin, const uint32_t* vin, uint32_t* ka, uint32_t* va, uint32_t* kb, uint32_t* vb,
                      int64_t n, int key_bits, uint32_t* hist, uint32_t* scan_temp, hipStream_t st,
-                     const ProfileHook& prof) {
+                     const ProfileHook& prof, const uint32_t* plan) {
     if (n <= 0) return 0;
     int64_t tiles = radix_tiles(n);
     int passes = radix_passes(key_bits, n);
@@ -340,7 +347,7 @@ int radix_sort_pairs(const uint32_t* kin, const uint32_t* vin, uint32_t* ka, uin
         const int radix = 1 << dw;
         prof.begin("sort_hist");
 #define MPM_HIST(B) hipLaunchKernelGGL(k_radix_hist<B>, dim3((unsigned)tiles), dim3(kSortWG), 0, st, ki, n, shift, \
-                                       mask, hist, tiles)
+                                       mask, hist, tiles, plan)
         if (dw == 9) MPM_HIST(9);
         else if (dw == 8) MPM_HIST(8);
         else if (dw == 7) MPM_HIST(7);
@@ -351,11 +358,11 @@ int radix_sort_pairs(const uint32_t* kin, const uint32_t* vin, uint32_t* ka, uin
         // (one launch fewer per pass for small sorts)
         const bool raw = tiles <= kRawTiles;
         prof.begin("sort_scan");
-        if (!raw) exclusive_scan(hist, tiles * radix, scan_temp, nullptr, st);
+        if (!raw) exclusive_scan(hist, tiles * radix, scan_temp, nullptr, st, plan);
         prof.end("sort_scan");
         prof.begin("sort_scatter");
 #define MPM_SCATTER(B, R) hipLaunchKernelGGL((k_radix_scatter<B, R>), dim3((unsigned)tiles), dim3(kSortWG), 0, st, ki, \
-                                             vi, ko, vo, n, shift, mask, hist, tiles)
+                                             vi, ko, vo, n, shift, mask, hist, tiles, plan)
         if (raw) {
             if (dw == 9) MPM_SCATTER(9, true);
             else if (dw == 8) MPM_SCATTER(8, true);

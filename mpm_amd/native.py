@@ -105,6 +105,7 @@ class MpmStats(C.Structure):
         ("max_speed", C.c_double),
         ("cfl", C.c_double),
         ("graph_captures", C.c_int64),
+        ("incremental_sorts", C.c_int64),
     ]
 
 
@@ -166,6 +167,7 @@ SIGNATURES = {
     "mpm_set_particles_collision": (C.c_int, [C.c_void_p, C.c_int]),
     "mpm_set_grid_record": (C.c_int, [C.c_void_p, C.c_int]),
     "mpm_set_reorder_interval": (C.c_int, [C.c_void_p, C.c_int]),
+    "mpm_set_transfer_fusion": (C.c_int, [C.c_void_p, C.c_int]),
     "mpm_sort": (C.c_int, [C.c_void_p]),
     "mpm_read_particles": (C.c_int, [C.c_void_p, C.POINTER(MpmParticleState)]),
     "mpm_read_particles_noalloc": (C.c_int, [C.c_void_p, C.POINTER(MpmParticleState)]),

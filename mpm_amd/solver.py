@@ -221,6 +221,11 @@ class Solver:
         (0 = never; default 100)."""
         N.check(self._lib.mpm_set_reorder_interval(self._h, int(steps)))
 
+    def set_transfer_fusion(self, on: bool):
+        """APIC, quadratic stencil, stepped alone: G2P of one step and P2G
+        of the next in one kernel (default on)."""
+        N.check(self._lib.mpm_set_transfer_fusion(self._h, 1 if on else 0))
+
     def initialize(self):
         N.check(self._lib.mpm_initialize(self._h))
 
