@@ -22,6 +22,15 @@ Algorithmic bytes (FP32): APIC per particle-update 364 B = P2G 148 B
 (x, v, B, Fe, Fp, m, V0, mu0, lambda0) + G2P 216 B (reads x, Fe, Fp; writes
 x, v, B, Fe, Fp); PIC-FLIP 304 B = 112 + 192.  Grid: 28 B per active node
 written by P2G and 28 B read by G2P (active = mass > 0, counted per step).
+The fused APIC kernel (g2p2g: one step's G2P and the next step's P2G) is
+credited with both: 364 B per particle and 56 B per active node.
+
+``roofline.traffic`` is the kernel's DRAM bytes per launch from the L2's
+request-size-resolved counters (TCC_EA0_RDREQ_DRAM_32B and
+TCC_EA0_WRREQ_WRITE_DRAM_32B / _ATOMIC_DRAM_32B, in 32-byte units: a 64-
+or 128-byte request counts 2 or 4), checked on a plain 1 GiB device copy;
+``traffic_detail`` also carries FETCH_SIZE / WRITE_SIZE scaled by what they
+read for that copy.
 """
 from __future__ import annotations
 
@@ -40,10 +49,10 @@ HBM_PEAK = 8.0e12
 METRIC = "particle-updates/sec & %HBM roofline; batched p50/p99 step latency vs plain"
 
 BYTES = {  # per particle-update, per kernel
-    1: {"p2g": 148, "g2p": 216},   # APIC
-    0: {"p2g": 112, "g2p": 192},   # PIC-FLIP
+    1: {"p2g": 148, "g2p": 216, "g2p2g": 364},   # APIC
+    0: {"p2g": 112, "g2p": 192},                 # PIC-FLIP
 }
-GRID_BYTES = {"p2g": 28, "g2p": 28}
+GRID_BYTES = {"p2g": 28, "g2p": 28, "g2p2g": 56}
 
 
 def log(msg):
@@ -157,61 +166,97 @@ def density_bench(solver, sc, frames=5, cpu=True):
     return out
 
 
-def _pmc_pass(counter, args, outdir, tag):
-    """One rocprofv3 counter pass over a short run of this script (child
-    process); returns {kernel name: mean counter value per dispatch}."""
+def _pmc_pass(counters, args, outdir, tag):
+    """One rocprofv3 counter pass (counters: a space-separated list that fits
+    one pass) over a short run of this script (child process group, killed
+    after 240 s); returns {kernel name: {counter: mean value per dispatch}}."""
     import csv
     import shutil
+    import signal
     import subprocess
 
     prof = shutil.which("rocprofv3")
     if prof is None:
         return None
-    d = os.path.join(outdir, f"{tag}_{counter}")
-    cmd = [prof, "--pmc", counter, "--output-format", "csv", "-d", d, "-o", tag, "--",
-           sys.executable, os.path.abspath(__file__)] + args
+    names = counters.split()
+    d = os.path.join(outdir, f"{tag}_{names[0]}")
+    cmd = [prof, "--pmc"] + names + ["--output-format", "csv", "-d", d, "-o", tag, "--",
+                                     sys.executable, os.path.abspath(__file__)] + args
     env = dict(os.environ, TMPDIR=os.environ.get("TMPDIR", "/tmp"))
+    p = subprocess.Popen(cmd, stdout=subprocess.DEVNULL, stderr=subprocess.DEVNULL, env=env, start_new_session=True)
     try:
-        subprocess.run(cmd, stdout=subprocess.DEVNULL, stderr=subprocess.DEVNULL, timeout=240, env=env, check=False)
+        p.wait(timeout=240)
     except subprocess.TimeoutExpired:
+        os.killpg(p.pid, signal.SIGKILL)
+        p.wait()
         return None
-    sums, counts = {}, {}
+    sums, disp = {}, {}
     for root, _, files in os.walk(d):
         for f in files:
             if f.endswith("counter_collection.csv"):
                 for r in csv.DictReader(open(os.path.join(root, f))):
                     k = r["Kernel_Name"]
-                    sums[k] = sums.get(k, 0.0) + float(r["Counter_Value"])
-                    counts[k] = counts.get(k, 0) + 1
-    return {k: sums[k] / counts[k] for k in sums} or None
+                    c = sums.setdefault(k, {})
+                    c[r["Counter_Name"]] = c.get(r["Counter_Name"], 0.0) + float(r["Counter_Value"])
+                    disp.setdefault(k, set()).add(r.get("Dispatch_Id", r.get("Correlation_Id", "")))
+    return {k: {c: v / max(len(disp[k]), 1) for c, v in cs.items()} for k, cs in sums.items()} or None
+
+
+DRAM_COUNTERS = ["TCC_EA0_RDREQ_DRAM_32B_sum", "TCC_EA0_WRREQ_WRITE_DRAM_32B_sum",
+                 "TCC_EA0_WRREQ_ATOMIC_DRAM_32B_sum"]
 
 
 def pmc_traffic(workload, seed_args, outdir):
-    """HBM bytes per launch of P2G and G2P: FETCH_SIZE and WRITE_SIZE (KiB per
-    dispatch) of a 3-step run, each scaled by what the same counter reads
-    for a plain 1 GiB device copy."""
+    """DRAM bytes per launch of the step's item kernels (P2G and G2P, or the
+    fused g2p2g) over a 3-step run: the L2's request-size-resolved DRAM
+    counters (32-byte units) in one pass, checked on a plain 1 GiB device
+    copy; beside them FETCH_SIZE and WRITE_SIZE (KiB per dispatch), each
+    scaled by what it reads for that copy."""
     run = ["--workload", workload, "--steps", "3", "--warmup", "1", "--no-cpu-baseline", "--batch-scenes", "0",
            "--serve-seconds", "0", "--no-small", "--no-traffic"] + seed_args
     copy = ["--calibrate-copy", "1024"]
+    dram, cdram = _pmc_pass(" ".join(DRAM_COUNTERS), run, outdir, "step"), \
+        _pmc_pass(" ".join(DRAM_COUNTERS), copy, outdir, "copy")
     fe, wr = _pmc_pass("FETCH_SIZE", run, outdir, "step"), _pmc_pass("WRITE_SIZE", run, outdir, "step")
     cfe, cwr = _pmc_pass("FETCH_SIZE", copy, outdir, "copy"), _pmc_pass("WRITE_SIZE", copy, outdir, "copy")
-    if not (fe and wr and cfe and cwr):
+    if not (dram and cdram):
         return None
-    ck = [k for k in cfe if "copyBuffer" in k]
-    if not ck or ck[0] not in cwr:
-        return None
-    kib = 1024 * 1024  # 1 GiB each way
-    sf, sw = kib / cfe[ck[0]], kib / cwr[ck[0]]
-    out = {"fetch_scale": sf, "write_scale": sw}
-    # the fused step's instances: P2G mode 0 (7 planes) or 2 (folded APIC
-    # grid), G2P source 0 or 3; not initialize()'s mass / volume passes
-    fused = {"p2g": (", 0, false>", ", 2, false>"), "g2p": (", 0, false,", ", 3, false,")}
-    for name, key in (("p2g", "k_p2g<"), ("g2p", "k_g2p<")):
-        ks = [k for k in fe if key in k and any(t in k for t in fused[name]) and k in wr]
-        if ks:
-            k = max(ks, key=lambda q: fe[q])
-            out[name] = {"kernel": k.split("(")[0], "fetch_kib": fe[k], "write_kib": wr[k],
-                         "bytes": (fe[k] * sf + wr[k] * sw) * 1024.0}
+    gib = float(1 << 30)
+
+    def rw(table, k):  # (read, write) bytes from the DRAM counters of kernel k
+        v = table[k]
+        return (32.0 * v.get(DRAM_COUNTERS[0], 0.0),
+                32.0 * (v.get(DRAM_COUNTERS[1], 0.0) + v.get(DRAM_COUNTERS[2], 0.0)))
+
+    ck = [k for k in cdram if "copyBuffer" in k]
+    out = {"method": "TCC_EA0_*_DRAM_32B x 32 B"}
+    if ck:
+        r, w = rw(cdram, ck[0])
+        out["copy_check"] = {"read_bytes": r, "write_bytes": w, "read_frac": r / gib, "write_frac": w / gib}
+    sf = sw = None
+    if fe and wr and cfe and cwr:
+        cks = [k for k in cfe if "copyBuffer" in k]
+        if cks and cks[0] in cwr:
+            sf = 1024 * 1024 / cfe[cks[0]]["FETCH_SIZE"]
+            sw = 1024 * 1024 / cwr[cks[0]]["WRITE_SIZE"]
+            out.update({"fetch_scale": sf, "write_scale": sw})
+    # the step's instances: P2G mode 0 (7 planes) or 2 (folded APIC grid),
+    # G2P source 0 or 3, the fused kernel; not initialize()'s mass / volume
+    # passes
+    step_kernels = {"p2g": ("k_p2g<", (", 0, false>", ", 2, false>")),
+                    "g2p": ("k_g2p<", (", 0, false,", ", 3, false,")),
+                    "g2p2g": ("k_g2p2g<", ("",))}
+    for name, (key, tags) in step_kernels.items():
+        ks = [k for k in dram if key in k and any(t in k for t in tags)]
+        if not ks:
+            continue
+        k = max(ks, key=lambda q: rw(dram, q)[0])
+        r, w = rw(dram, k)
+        out[name] = {"kernel": k.split("(")[0], "read_bytes": r, "write_bytes": w, "bytes": r + w}
+        if sf and k in fe and k in wr:
+            f_kib, w_kib = fe[k]["FETCH_SIZE"], wr[k]["WRITE_SIZE"]
+            out[name].update({"fetch_kib": f_kib, "write_kib": w_kib,
+                              "scaled_bytes": (f_kib * sf + w_kib * sw) * 1024.0})
     return out
 
 
@@ -407,7 +452,8 @@ def main(argv=None):
         finally:
             shutil.rmtree(outdir, ignore_errors=True)
         log(f"pmc traffic passes in {time.time() - t0:.1f}s: "
-            + (", ".join(f"{k}={v['bytes'] / 1e9:.3f} GB" for k, v in traffic.items() if isinstance(v, dict))
+            + (", ".join(f"{k}={v['bytes'] / 1e9:.3f} GB" for k, v in traffic.items()
+                          if isinstance(v, dict) and "bytes" in v)
                if traffic else "unavailable"))
 
     kw = {} if a.seed is None else {"seed": a.seed}

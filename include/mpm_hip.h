@@ -325,7 +325,8 @@ MPM_API int mpm_get_stats(void* handle, mpm_stats* out);
 /* ------------------------------------------------------------- profiling */
 /* enable: 0 off; 1 every kernel group (two events around each launch group:
  * about 30 per step, which add ~0.1 ms of stream gaps per step); 2 only the
- * item kernels p2g and g2p (4 events per step) and the store reorder. */
+ * item kernels p2g and g2p, or the fused g2p2g (2-4 events per step), and
+ * the store reorder. */
 MPM_API int mpm_profile_enable(void* handle, int enable);
 MPM_API int mpm_profile_reset(void* handle);
 /* Fills up to max entries; *count gets the number of kernels tracked. */

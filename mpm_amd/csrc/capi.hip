@@ -69,7 +69,7 @@ hipEvent_t take_event(Solver& s) {
 // Level 2 times the item kernels and the store reorder only.
 bool prof_wants(const Solver& s, const char* name) {
     return s.prof != 2 || std::strcmp(name, "p2g") == 0 || std::strcmp(name, "g2p") == 0 ||
-           std::strcmp(name, "reorder") == 0;
+           std::strcmp(name, "g2p2g") == 0 || std::strcmp(name, "reorder") == 0;
 }
 void prof_begin(void* ctx, const char* name) {
     Solver& s = *static_cast<Solver*>(ctx);
@@ -261,6 +261,7 @@ int alloc_particles(Solver& s, ParticlesDev& P, int64_t n) {
     if (!s.apic) P.B = nullptr;
     if ((rc = dalloc(s, (void**)&P.Fe, 9 * f))) return rc;
     if ((rc = dalloc(s, (void**)&P.Fp, 9 * f))) return rc;
+    if ((rc = dalloc(s, (void**)&P.Jp, f))) return rc;
     if ((rc = dalloc(s, (void**)&P.m, f))) return rc;
     if ((rc = dalloc(s, (void**)&P.V0, f))) return rc;
     if ((rc = dalloc(s, (void**)&P.mu0, f))) return rc;
@@ -272,7 +273,7 @@ int alloc_particles(Solver& s, ParticlesDev& P, int64_t n) {
     return MPM_OK;
 }
 void free_particles(Solver& s, ParticlesDev& P) {
-    dfree(s, P.x); dfree(s, P.v); dfree(s, P.B); dfree(s, P.Fe); dfree(s, P.Fp); dfree(s, P.m);
+    dfree(s, P.x); dfree(s, P.v); dfree(s, P.B); dfree(s, P.Fe); dfree(s, P.Fp); dfree(s, P.Jp); dfree(s, P.m);
     dfree(s, P.V0); dfree(s, P.mu0); dfree(s, P.lam0); dfree(s, P.tau); dfree(s, P.mat); dfree(s, P.pid);
 }
 
@@ -401,9 +402,13 @@ static bool use_fused_transfer(const Solver& s) {
            !s.batch && s.N > 0;
 }
 
-// The second grid buffer of the fused transfer (allocated on first use).
+// The second grid buffer of the fused transfer and the stream that clears
+// it (allocated on first use).
 static int ensure_grid2(Solver& s) {
     if (s.grid2) return MPM_OK;
+    if (!s.aux_st) HIP_TRY(hipStreamCreateWithFlags(&s.aux_st, hipStreamNonBlocking));
+    if (!s.aux_fork) HIP_TRY(hipEventCreateWithFlags(&s.aux_fork, hipEventDisableTiming));
+    if (!s.aux_join) HIP_TRY(hipEventCreateWithFlags(&s.aux_join, hipEventDisableTiming));
     const size_t gbytes = (size_t)s.S.plane * GF_ALL * sizeof(float);
     int rc = dalloc(s, (void**)&s.grid2, gbytes);
     if (rc) return rc;
@@ -414,6 +419,37 @@ static int ensure_grid2(Solver& s) {
     return MPM_OK;
 }
 
+// Clear of the second grid buffer (the grid the last fused step read, its
+// active nodes counted, or pre-scattered sums that were dropped).  Unless
+// every launch group is being timed (profile level 1) it runs on aux_st,
+// overlapping the sort; the step joins it before the fused kernel writes
+// the buffer.
+// Returns true when a join is due.
+static bool clear_grid2(Solver& s) {
+    if (!s.grid2_dirty) return false;
+    SceneDev S2 = s.S;
+    S2.grid = s.grid2;
+    S2.touched = s.touched2;
+    unsigned long long* cnt = s.grid2_count ? s.acc : nullptr;
+    s.grid2_dirty = false;
+    if (s.prof == 1) {
+        ProfileHook h = hook(s);
+        Scope sc(h, "grid_clear");
+        launch_clear(S2, s.nsb, GF_FOLDED, cnt, s.st);
+        return false;
+    }
+    if (hipEventRecord(s.aux_fork, s.st) != hipSuccess || hipStreamWaitEvent(s.aux_st, s.aux_fork, 0) != hipSuccess) {
+        launch_clear(S2, s.nsb, GF_FOLDED, cnt, s.st);
+        return false;
+    }
+    launch_clear(S2, s.nsb, GF_FOLDED, cnt, s.aux_st);
+    if (hipEventRecord(s.aux_join, s.aux_st) != hipSuccess) {
+        (void)hipStreamSynchronize(s.aux_st);
+        return false;
+    }
+    return true;
+}
+
 void run_fused_step(Solver& s) {
     ProfileHook h = hook(s);
     const bool fuse = use_fused_transfer(s) && ensure_grid2(s) == MPM_OK;
@@ -422,6 +458,7 @@ void run_fused_step(Solver& s) {
         if (s.grid_dirty) run_clear(s);
         ensure_stress(s);
     }
+    const bool join = fuse && clear_grid2(s);
     // a raster or sub-step since the last change of the positions has
     // already sorted them
     if (!s.sort_valid) {
@@ -447,14 +484,7 @@ void run_fused_step(Solver& s) {
             Scope sc(h, "p2g");
             launch_p2g(s.S, s.P, I, s.interp, s.apic, 2, s.item_wgs, s.st);
         }
-        if (s.grid2_dirty) {  // the grid the last step read (its active nodes counted), cleared
-            SceneDev S2 = s.S;
-            S2.grid = s.grid2;
-            S2.touched = s.touched2;
-            Scope sc(h, "grid_clear");
-            launch_clear(S2, s.nsb, GF_FOLDED, s.grid2_count ? s.acc : nullptr, s.st);
-            s.grid2_dirty = false;
-        }
+        if (join && hipStreamWaitEvent(s.st, s.aux_join, 0) != hipSuccess) (void)hipStreamSynchronize(s.aux_st);
         {
             Scope sc(h, "g2p2g");
             launch_g2p2g(s.S, s.P, I, s.S.particle_collision, s.grid2, s.touched2, s.kin, s.st);
@@ -606,6 +636,12 @@ void destroy(Solver* s) {
         (void)hipStreamSynchronize(s->copy_st);
         (void)hipStreamDestroy(s->copy_st);
     }
+    if (s->aux_st) {
+        (void)hipStreamSynchronize(s->aux_st);
+        (void)hipStreamDestroy(s->aux_st);
+    }
+    if (s->aux_fork) (void)hipEventDestroy(s->aux_fork);
+    if (s->aux_join) (void)hipEventDestroy(s->aux_join);
     if (s->pos4_packed) (void)hipEventDestroy(s->pos4_packed);
     if (s->pos4_copied) (void)hipEventDestroy(s->pos4_copied);
     if (s->gexec) (void)hipGraphExecDestroy(s->gexec);
@@ -828,8 +864,8 @@ MPM_API int mpm_step_graph(void* handle, int n) {
         s->snap_ok = true;
     }
     if (!s->gexec) {
-        bool prof = s->prof;
-        s->prof = false;
+        const int prof = s->prof;  // no events inside the capture
+        s->prof = 0;
         HIP_TRY(hipStreamBeginCapture(s->st, hipStreamCaptureModeThreadLocal));
         bool dirty = s->grid_dirty;
         // force the clear node (of the planes the step writes, counting the

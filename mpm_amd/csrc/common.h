@@ -76,6 +76,7 @@ struct ParticlesDev {
     float MPM_GLOBAL* B;      // 9N (APIC) or null
     float MPM_GLOBAL* Fe;     // 9N row-major
     float MPM_GLOBAL* Fp;     // 9N
+    float MPM_GLOBAL* Jp;     // N det(Fp), stored with Fp (the stress's hardening needs only it)
     float MPM_GLOBAL* m;      // N
     float MPM_GLOBAL* V0;     // N
     float MPM_GLOBAL* mu0;    // N

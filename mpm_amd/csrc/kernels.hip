@@ -309,10 +309,10 @@ __device__ __forceinline__ bool in_grid(const SceneDev& S, int i, int j, int k) 
 // xz, yy, yz, zz.  For Neo-Hookean the singular values are first raised to
 // FLT_EPSILON and the projected Fe = U diag(s) V^T is returned in Fe (the
 // reference writes it back into the particle, :142).
-__device__ __forceinline__ void stress_from_svd(const float U[3][3], const float sv[3], const float V[3][3],
-                                                const float Fp[3][3], float mu0, float lam0, float V0,
-                                                const MaterialDev& mat, float tau[6], float Fe[3][3]) {
-    const float Jp = m3_det(Fp);
+// Jp = det(Fp) (the hardening).
+__device__ __forceinline__ void stress_from_svd(const float U[3][3], const float sv[3], const float V[3][3], float Jp,
+                                                float mu0, float lam0, float V0, const MaterialDev& mat, float tau[6],
+                                                float Fe[3][3]) {
     // exp to a few ulp (hardware exp2 of x log2 e); the reference's FP64
     // exp is narrowed to float (src/material_point3d.cu:160)
     const float factor = __expf(mat.hardening * (1.0f - Jp));
@@ -417,9 +417,8 @@ __device__ __forceinline__ bool elastic_trial(const float Fn[3][3], const Materi
 // With b = I + E, b - sqrt(b) is the series of 1 + x - sqrt(1 + x):
 //   E/2 + E^2/8 - E^3/16 + 5 E^4/128 - 7 E^5/256 + O(E^6)
 // (Horner form; the next term is below 21 |E|^6 / 1024 <= 3.2e-10).
-__device__ __forceinline__ void elastic_fc_stress(const float E[6], float J, const float Fp[3][3], float mu0,
-                                                  float lam0, float V0, const MaterialDev& mat, float tau[6]) {
-    const float Jp = m3_det(Fp);
+__device__ __forceinline__ void elastic_fc_stress(const float E[6], float J, float Jp, float mu0, float lam0,
+                                                  float V0, const MaterialDev& mat, float tau[6]) {
     const float factor = __expf(mat.hardening * (1.0f - Jp));
     const float mu = mu0 * factor, la = lam0 * factor;
     const bool dg[6] = {true, false, false, true, false, true};
@@ -828,8 +827,10 @@ __device__ __forceinline__ void record_move(const ItemsDev& I, bool moved, uint3
 // -------------------------------------------------------------------- G2P
 // Folded-grid tile of one item, in two halves.  fetch_folded_tile issues
 // the lane's tile-node loads (mass and p + dt f) as global_load_lds_dword:
-// the data goes straight from HBM into the wave's staging area in LDS (word
-// (r * 4 + f) * 64 + lane of its region) without passing through VGPRs.
+// the data goes straight from HBM into the staging area in LDS (word
+// f * TN + q for value f of tile node q: 4 TN words, the wave's 64
+// consecutive nodes land at one wave-uniform base) without passing through
+// VGPRs.
 // commit_folded_tile waits for them (vmcnt(0): the compiler does not track
 // these LDS writes) and turns them into v* (the fused grid update) in the
 // tile.  The G2P loop fetches the next item's tile before the current item's
@@ -839,22 +840,23 @@ typedef __attribute__((address_space(3))) void lds_void;
 template <int T, int TR, int WG>
 __device__ __forceinline__ void fetch_folded_tile(const SceneDev& S, const ItemGeom& g, float* stage) {
     constexpr int TN = T * T * T;
-    float* wst = stage + (threadIdx.x >> 6) * (TR * 4 * 64);
 #pragma unroll 1  // unrolled, the address arithmetic of all rounds stays live (spills at 64 threads)
     for (int r = 0; r < TR; ++r) {
         const int q = (int)threadIdx.x + r * WG;
+        // the wave's first node this round: its lanes write words base + lane
+        float* const wst = stage + r * WG + (int)(threadIdx.x & ~63u);
         const int lx = q % T, ly = (q / T) % T, lz = q / (T * T);
         const int gi = g.ox + lx, gj = g.oy + ly, gk = g.oz + lz;
         if (q < TN && !g.escape && in_grid(S, gi, gj, gk)) {
             const int64_t ni = node_index(S, gi, gj, gk);
             const int64_t pl = S.plane;
-            __builtin_amdgcn_global_load_lds((const void*)(S.grid + ni), (lds_void*)(wst + (r * 4 + 0) * 64), 4, 0, 0);
-            __builtin_amdgcn_global_load_lds((const void*)(S.grid + (GF_VEL + 0) * pl + ni),
-                                             (lds_void*)(wst + (r * 4 + 1) * 64), 4, 0, 0);
+            __builtin_amdgcn_global_load_lds((const void*)(S.grid + ni), (lds_void*)wst, 4, 0, 0);
+            __builtin_amdgcn_global_load_lds((const void*)(S.grid + (GF_VEL + 0) * pl + ni), (lds_void*)(wst + TN), 4,
+                                             0, 0);
             __builtin_amdgcn_global_load_lds((const void*)(S.grid + (GF_VEL + 1) * pl + ni),
-                                             (lds_void*)(wst + (r * 4 + 2) * 64), 4, 0, 0);
+                                             (lds_void*)(wst + 2 * TN), 4, 0, 0);
             __builtin_amdgcn_global_load_lds((const void*)(S.grid + (GF_VEL + 2) * pl + ni),
-                                             (lds_void*)(wst + (r * 4 + 3) * 64), 4, 0, 0);
+                                             (lds_void*)(wst + 3 * TN), 4, 0, 0);
         }
     }
 }
@@ -863,7 +865,6 @@ __device__ __forceinline__ void commit_folded_tile(const SceneDev& S, const Item
                                                    float* tile) {
     constexpr int TN = T * T * T;
     asm volatile("s_waitcnt vmcnt(0)" ::: "memory");
-    const float* wst = stage + (threadIdx.x >> 6) * (TR * 4 * 64) + (threadIdx.x & 63);
 #pragma unroll 1  // as in fetch_folded_tile (the cubic stencil spills at 64 threads otherwise)
     for (int r = 0; r < TR; ++r) {
         const int q = (int)threadIdx.x + r * WG;
@@ -872,8 +873,8 @@ __device__ __forceinline__ void commit_folded_tile(const SceneDev& S, const Item
         const int gi = g.ox + lx, gj = g.oy + ly, gk = g.oz + lz;
         float v0 = 0.0f, v1 = 0.0f, v2 = 0.0f;
         if (in_grid(S, gi, gj, gk))
-            grid_node_update_folded(S, gi, gj, gk, wst[(r * 4 + 0) * 64], wst[(r * 4 + 1) * 64],
-                                    wst[(r * 4 + 2) * 64], wst[(r * 4 + 3) * 64], v0, v1, v2);
+            grid_node_update_folded(S, gi, gj, gk, stage[q], stage[TN + q], stage[2 * TN + q], stage[3 * TN + q], v0,
+                                    v1, v2);
         tile[q] = v0;
         tile[TN + q] = v1;
         tile[2 * TN + q] = v2;
@@ -902,7 +903,7 @@ __global__ void __launch_bounds__(WG) __attribute__((amdgpu_waves_per_eu(G2PWave
     constexpr bool PSTAGE = PREFETCH && INTERP != MPM_INTERP_CUBIC_BSPLINE;
     constexpr int PW = 21;  // staged words per particle
     __shared__ float tile[NF * TN];
-    __shared__ float stage[PREFETCH ? TR * 4 * WG : 1];
+    __shared__ float stage[PREFETCH ? 4 * TN : 1];
     __shared__ float pstage[PSTAGE ? PW * WG : 1];
     const uint32_t nitems = *I.count;
     ItemGeom gnext{};
@@ -1178,15 +1179,17 @@ __global__ void __launch_bounds__(WG) __attribute__((amdgpu_waves_per_eu(G2PWave
             if (__ballot(!elastic) == 0ull) {
                 // every particle of the wave stays inside its thresholds:
                 // Fe = Fn, Fp unchanged (not stored), stress without an SVD
-                elastic_fc_stress(E, J, Fp, mu0, lam0, V0, mat, tau);
+                elastic_fc_stress(E, J, m3_det(Fp), mu0, lam0, V0, mat, tau);
                 store_m3(P.Fe, N, p, Fn);
             } else {
                 float U[3][3], sv[3], V[3][3];
                 update_deformation(Fn, Fe, Fp, mat, U, sv, V);
                 // stress for the next P2G from the same decomposition
-                stress_from_svd(U, sv, V, Fp, mu0, lam0, V0, mat, tau, Fe);
+                const float Jp = m3_det(Fp);
+                stress_from_svd(U, sv, V, Jp, mu0, lam0, V0, mat, tau, Fe);
                 store_m3(P.Fe, N, p, Fe);
                 store_m3(P.Fp, N, p, Fp);
+                P.Jp[p] = Jp;
             }
 #pragma unroll
             for (int k = 0; k < 6; ++k) P.tau[(int64_t)k * N + p] = tau[k];
@@ -1254,48 +1257,46 @@ __device__ __forceinline__ void apic_gather(const float* tq, const float (&Nx)[W
 // One particle's folded-grid contributions to the nodes of its stencil
 // (particlesToGrid, src/apic_mpm_solver3d.cu:245-304, with p + dt f summed
 // per node): mass m w and m w (v + C (x_i - x_p)) + vs grad w, vs = dt tau.
-// Tensor-product factors as in k_p2g.  Each node's four values are summed
-// over the lanes of the particle's cell (segmented scan) and the cell's
-// last lane adds them to the tile at tq (the stencil's first node).
+// Tensor-product factors, (y, z) pairs outermost so that few of them are
+// live at once:
+//   mass     m w                   = (m Nx[a]) yz
+//   momentum m w (v + C (x_i - x_p)) = mip (cx(a) + cyz(b,c))
+//   force    vs grad w             = dNx[a] (vs_0 yz) + Nx[a] (vs_1 dyz + vs_2 ydz)
+// Each node's four values are summed over the lanes of the particle's cell
+// (segmented scan) and the cell's last lane adds them to the tile at tq
+// (the stencil's first node).
 template <int W, int T, int TN>
 __device__ __forceinline__ void folded_cell_sums(float* tq, const SegPlan& sp, float m, const float (&vel)[3],
                                                  const float (&Cm)[3][3], const float (&vs)[3][3],
                                                  const float (&dxn)[3][W], const float (&Nx)[W],
                                                  const float (&Ny)[W], const float (&Nz)[W], const float (&dNx)[W],
                                                  const float (&dNy)[W], const float (&dNz)[W]) {
-    float yz[W][W], dyz[W][W], ydz[W][W];
+    float ma[W], cx[W][3];
+#pragma unroll
+    for (int a = 0; a < W; ++a) {
+        ma[a] = m * Nx[a];
+#pragma unroll
+        for (int k = 0; k < 3; ++k) cx[a][k] = vel[k] + Cm[k][0] * dxn[0][a];
+    }
 #pragma unroll
     for (int b = 0; b < W; ++b)
 #pragma unroll
         for (int c = 0; c < W; ++c) {
-            yz[b][c] = Ny[b] * Nz[c];
-            dyz[b][c] = dNy[b] * Nz[c];
-            ydz[b][c] = Ny[b] * dNz[c];
-        }
+            const float yz = Ny[b] * Nz[c], dyz = dNy[b] * Nz[c], ydz = Ny[b] * dNz[c];
+            float cyz[3], g0[3], g12[3];
 #pragma unroll
-    for (int a = 0; a < W; ++a) {
-        const float ma = m * Nx[a];
-        float F0[3], F1[3], F2[3], cx[3];
+            for (int k = 0; k < 3; ++k) {
+                cyz[k] = Cm[k][1] * dxn[1][b] + Cm[k][2] * dxn[2][c];
+                g0[k] = vs[k][0] * yz;
+                g12[k] = vs[k][1] * dyz + vs[k][2] * ydz;
+            }
 #pragma unroll
-        for (int k = 0; k < 3; ++k) {
-            F0[k] = vs[k][0] * dNx[a];
-            F1[k] = vs[k][1] * Nx[a];
-            F2[k] = vs[k][2] * Nx[a];
-            cx[k] = vel[k] + Cm[k][0] * dxn[0][a];
-        }
-#pragma unroll
-        for (int b = 0; b < W; ++b)
-#pragma unroll
-            for (int c = 0; c < W; ++c) {
-                const float mip = ma * yz[b][c];
+            for (int a = 0; a < W; ++a) {
+                const float mip = ma[a] * yz;
                 float v[4];
                 v[0] = mip;
 #pragma unroll
-                for (int k = 0; k < 3; ++k) {
-                    const float ck = cx[k] + (Cm[k][1] * dxn[1][b] + Cm[k][2] * dxn[2][c]);
-                    const float fk = (F0[k] * yz[b][c] + F1[k] * dyz[b][c]) + F2[k] * ydz[b][c];
-                    v[1 + k] = fmaf(mip, ck, fk);
-                }
+                for (int k = 0; k < 3; ++k) v[1 + k] = fmaf(mip, cx[a][k] + cyz[k], dNx[a] * g0[k] + Nx[a] * g12[k]);
                 seg_scan<4>(v, sp);
                 const int q = (c * T + b) * T + a;
                 if (sp.writer) {
@@ -1371,11 +1372,11 @@ k_g2p2g(SceneDev S, ParticlesDev P, ItemsDev I, float* __restrict__ ogrid, uint3
     constexpr int T = 4 + W - 1;
     constexpr int TN = T * T * T;
     constexpr int TR = (TN + WG - 1) / WG;
-    constexpr int PW = 21;
-    __shared__ float tile[3 * TN];         // v* of this step
-    __shared__ float stage[TR * 4 * WG];   // the next item's folded tile in flight
-    __shared__ float pstage[PW * WG];      // Fe, Fp, mu0, lam0, V0 of each lane's particle
-    __shared__ float otile[2 * 4 * TN];    // next step's sums, one tile per wave
+    constexpr int PW = 13;
+    __shared__ float tile[3 * TN];       // v* of this step
+    __shared__ float stage[4 * TN];      // the next item's folded tile in flight
+    __shared__ float pstage[PW * WG];    // Fe, Jp, mu0, lam0, V0 of each lane's particle
+    __shared__ float otile[2 * 4 * TN];  // next step's sums, one tile per wave
     const int wave = threadIdx.x >> 6, lane = threadIdx.x & 63;
     const int64_t N = P.n;
     const uint32_t nitems = *I.count;
@@ -1395,9 +1396,12 @@ k_g2p2g(SceneDev S, ParticlesDev P, ItemsDev I, float* __restrict__ ogrid, uint3
             gnext = item_geom(S, I, itn, nitems, 0);
             fetch_folded_tile<T, TR, WG>(S, gnext, stage);
         }
-        // uniform trip count: every lane takes part in the cell sums
-        for (uint32_t base = g.s; base < g.e; base += WG) {
-            const uint32_t i = base + threadIdx.x;
+        // each wave takes every other 64-particle chunk of the item (no
+        // barrier inside: a wave's trip count is its own, and every pass
+        // has at least one particle); all lanes of a wave take part in the
+        // cell sums
+        for (uint32_t base = g.s + 64u * (uint32_t)wave; base < g.e; base += WG) {
+            const uint32_t i = base + lane;
             const bool active = i < g.e;
             float xn[3] = {0.0f, 0.0f, 0.0f}, vn[3] = {0.0f, 0.0f, 0.0f}, Cn[3][3], vs[3][3], m = 0.0f;
             uint32_t kold = 0xFFFFFFFFu, knew = 0xFFFFFFFFu;
@@ -1415,15 +1419,13 @@ k_g2p2g(SceneDev S, ParticlesDev P, ItemsDev I, float* __restrict__ ogrid, uint3
                 }
                 float* const pst = pstage + wave * (PW * 64);
 #pragma unroll 1
-                for (int f = 0; f < 9; ++f) {
+                for (int f = 0; f < 9; ++f)
                     __builtin_amdgcn_global_load_lds((const void*)(P.Fe + (int64_t)f * N + p), (lds_void*)(pst + f * 64),
                                                      4, 0, 0);
-                    __builtin_amdgcn_global_load_lds((const void*)(P.Fp + (int64_t)f * N + p),
-                                                     (lds_void*)(pst + (9 + f) * 64), 4, 0, 0);
-                }
-                __builtin_amdgcn_global_load_lds((const void*)(P.mu0 + p), (lds_void*)(pst + 18 * 64), 4, 0, 0);
-                __builtin_amdgcn_global_load_lds((const void*)(P.lam0 + p), (lds_void*)(pst + 19 * 64), 4, 0, 0);
-                __builtin_amdgcn_global_load_lds((const void*)(P.V0 + p), (lds_void*)(pst + 20 * 64), 4, 0, 0);
+                __builtin_amdgcn_global_load_lds((const void*)(P.Jp + p), (lds_void*)(pst + 9 * 64), 4, 0, 0);
+                __builtin_amdgcn_global_load_lds((const void*)(P.mu0 + p), (lds_void*)(pst + 10 * 64), 4, 0, 0);
+                __builtin_amdgcn_global_load_lds((const void*)(P.lam0 + p), (lds_void*)(pst + 11 * 64), 4, 0, 0);
+                __builtin_amdgcn_global_load_lds((const void*)(P.V0 + p), (lds_void*)(pst + 12 * 64), 4, 0, 0);
                 const float x0 = P.x[p], x1 = P.x[N + p], x2 = P.x[2 * N + p];
                 m = P.m[p];
                 float grad[3][3], Bm[3][3], v[3] = {0.0f, 0.0f, 0.0f};
@@ -1461,45 +1463,43 @@ k_g2p2g(SceneDev S, ParticlesDev P, ItemsDev I, float* __restrict__ ogrid, uint3
                 next_keys[p] = knew;
                 if (I.plan) record_move(I, knew != kold, knew, p, i);
                 // updateDeformationGradient and the stress of the new state
-                float Fe[3][3], Fp[3][3];
+                float Fe[3][3];
                 asm volatile("s_waitcnt vmcnt(0)" ::: "memory");  // staged words have landed
                 float* const q = pst + lane;
 #pragma unroll
-                for (int f = 0; f < 9; ++f) {
-                    Fe[f / 3][f % 3] = q[f * 64];
-                    Fp[f / 3][f % 3] = q[(9 + f) * 64];
-                }
-                const float mu0 = q[18 * 64], lam0 = q[19 * 64], V0 = q[20 * 64];
-                // the lane's staging words are free now: v, B and the new
+                for (int f = 0; f < 9; ++f) Fe[f / 3][f % 3] = q[f * 64];
+                const float Jp = q[9 * 64], mu0 = q[10 * 64], lam0 = q[11 * 64], V0 = q[12 * 64];
+                // the lane's staging words are free now: B and the new
                 // position wait there through the F update (fewer registers)
 #pragma unroll
-                for (int k = 0; k < 3; ++k) {
-                    q[k * 64] = v[k];
-                    q[(3 + k) * 64] = xn[k];
-                }
+                for (int k = 0; k < 3; ++k) q[k * 64] = xn[k];
 #pragma unroll
-                for (int f = 0; f < 9; ++f) q[(6 + f) * 64] = Bm[f / 3][f % 3];
+                for (int f = 0; f < 9; ++f) q[(3 + f) * 64] = Bm[f / 3][f % 3];
                 const MaterialDev mat = P.mats[P.mat[p]];
                 float Fn[3][3], E[6], J, tau[6];
                 trial_deformation(Fe, grad, S.dt, Fn);
                 const bool elastic = mat.model == MPM_MODEL_FIXED_COROTATED && elastic_trial(Fn, mat, E, J);
                 if (__ballot(!elastic) == 0ull) {
-                    elastic_fc_stress(E, J, Fp, mu0, lam0, V0, mat, tau);
+                    elastic_fc_stress(E, J, Jp, mu0, lam0, V0, mat, tau);
                     store_m3(P.Fe, N, p, Fn);
                 } else {
-                    float U[3][3], sv[3], V[3][3];
+                    // a plastic update somewhere in the wave: Fp from memory
+                    float Fp[3][3], U[3][3], sv[3], V[3][3];
+                    load_m3(P.Fp, N, p, Fp);
                     update_deformation(Fn, Fe, Fp, mat, U, sv, V);
-                    stress_from_svd(U, sv, V, Fp, mu0, lam0, V0, mat, tau, Fe);
+                    const float Jn = m3_det(Fp);
+                    stress_from_svd(U, sv, V, Jn, mu0, lam0, V0, mat, tau, Fe);
                     store_m3(P.Fe, N, p, Fe);
                     store_m3(P.Fp, N, p, Fp);
+                    P.Jp[p] = Jn;
                 }
 #pragma unroll
                 for (int k = 0; k < 3; ++k) {
-                    vn[k] = q[k * 64];
-                    xn[k] = q[(3 + k) * 64];
+                    vn[k] = v[k];
+                    xn[k] = q[k * 64];
                 }
 #pragma unroll
-                for (int f = 0; f < 9; ++f) Cn[f / 3][f % 3] = q[(6 + f) * 64] * S.dinv;
+                for (int f = 0; f < 9; ++f) Cn[f / 3][f % 3] = q[(3 + f) * 64] * S.dinv;
                 const float t[6] = {S.dt * tau[0], S.dt * tau[1], S.dt * tau[2], S.dt * tau[3], S.dt * tau[4],
                                     S.dt * tau[5]};
                 vs[0][0] = t[0]; vs[0][1] = t[1]; vs[0][2] = t[2];
@@ -1628,12 +1628,11 @@ __global__ void k_stress(ParticlesDev P) {
     int64_t p = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
     if (p >= P.n) return;
     const int64_t N = P.n;
-    float Fe[3][3], Fp[3][3], U[3][3], V[3][3], sv[3], tau[6];
+    float Fe[3][3], U[3][3], V[3][3], sv[3], tau[6];
     load_m3(P.Fe, N, (uint32_t)p, Fe);
-    load_m3(P.Fp, N, (uint32_t)p, Fp);
     const MaterialDev mat = P.mats[P.mat[p]];
     svd3(Fe, U, sv, V);
-    stress_from_svd(U, sv, V, Fp, P.mu0[p], P.lam0[p], P.V0[p], mat, tau, Fe);
+    stress_from_svd(U, sv, V, P.Jp[p], P.mu0[p], P.lam0[p], P.V0[p], mat, tau, Fe);
     if (mat.model == MPM_MODEL_NEO_HOOKEAN) store_m3(P.Fe, N, (uint32_t)p, Fe);
 #pragma unroll
     for (int k = 0; k < 6; ++k) P.tau[(int64_t)k * N + p] = tau[k];
@@ -1659,9 +1658,9 @@ __device__ __forceinline__ void wg_count_add(unsigned long long* dst, unsigned l
 }
 
 // Grid clear: one lane per storage block reads the touched flags
-// (coalesced); the wave then clears the touched blocks it found, one block
-// (64 nodes, one per lane) at a time, and counts their nodes with mass > 0
-// (the active nodes of the step that wrote them).
+// (coalesced); the wave then clears the touched blocks it found (64 nodes,
+// one per lane) and counts their nodes with mass > 0 (the active nodes of
+// the step that wrote them).  clear_block: one block (the batched clear).
 __device__ __forceinline__ unsigned long long clear_block(float* grid, int64_t plane, uint32_t* touched, int64_t b,
                                                           int nfields, int lane) {
     const int64_t ni = b * 64 + lane;
@@ -1671,6 +1670,11 @@ __device__ __forceinline__ unsigned long long clear_block(float* grid, int64_t p
     return c;
 }
 
+// Each wave takes the touched blocks among 64 consecutive storage blocks,
+// kClearBatch at a time: their mass loads are all issued before the first
+// count waits on one, so a wave pays one memory latency per batch rather
+// than one per block.
+constexpr int kClearBatch = 8;
 __global__ void __launch_bounds__(256) k_clear(SceneDev S, int64_t nsb, int nfields, unsigned long long* active) {
     const int lane = threadIdx.x & 63;
     const int64_t b = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
@@ -1678,9 +1682,26 @@ __global__ void __launch_bounds__(256) k_clear(SceneDev S, int64_t nsb, int nfie
     const int64_t wave0 = b - lane;
     unsigned long long cnt = 0;
     while (todo) {
-        const int l = __builtin_ctzll(todo);
-        todo &= todo - 1;
-        cnt += clear_block(S.grid, S.plane, S.touched, wave0 + l, nfields, lane);
+        int64_t blk[kClearBatch];
+        float m[kClearBatch];
+#pragma unroll
+        for (int u = 0; u < kClearBatch; ++u) {
+            blk[u] = -1;
+            if (todo) {
+                blk[u] = wave0 + __builtin_ctzll(todo);
+                todo &= todo - 1;
+            }
+        }
+#pragma unroll
+        for (int u = 0; u < kClearBatch; ++u) m[u] = blk[u] >= 0 ? S.grid[blk[u] * 64 + lane] : 0.0f;
+#pragma unroll
+        for (int u = 0; u < kClearBatch; ++u) {
+            if (blk[u] < 0) continue;
+            cnt += (unsigned long long)__popcll(__ballot(m[u] > 0.0f));
+            const int64_t ni = blk[u] * 64 + lane;
+            for (int f = 0; f < nfields; ++f) S.grid[(int64_t)f * S.plane + ni] = 0.0f;
+            if (lane == 0) S.touched[blk[u]] = 0u;
+        }
     }
     if (active) wg_count_add(active, lane == 0 ? cnt : 0ull);
 }
@@ -1847,12 +1868,15 @@ __global__ void k_pack(HostLayoutDev H, ParticlesDev P) {
         P.x[a * N + s] = H.pos[3 * id + a];
         P.v[a * N + s] = H.vel[3 * id + a];
     }
+    float Fp[3][3];
 #pragma unroll
     for (int k = 0; k < 9; ++k) {
         P.Fe[k * N + s] = H.Fe[9 * id + k];
-        P.Fp[k * N + s] = H.Fp[9 * id + k];
+        Fp[k / 3][k % 3] = H.Fp[9 * id + k];
+        P.Fp[k * N + s] = Fp[k / 3][k % 3];
         if (P.B) P.B[k * N + s] = H.B ? H.B[9 * id + k] : 0.0f;
     }
+    P.Jp[s] = m3_det(Fp);
 }
 void launch_pack(const HostLayoutDev& H, const ParticlesDev& P, hipStream_t st) {
     if (P.n <= 0) return;
@@ -1908,6 +1932,7 @@ __global__ void k_permute(ParticlesDev S, ParticlesDev D, const uint32_t* __rest
     D.m[i] = S.m[s];
     D.mu0[i] = S.mu0[s];
     D.lam0[i] = S.lam0[s];
+    D.Jp[i] = S.Jp[s];
     D.mat[i] = S.mat[s];
     D.pid[i] = S.pid[s];
 #pragma unroll
@@ -1938,8 +1963,9 @@ void launch_permute(const ParticlesDev& src, const ParticlesDev& dst, const uint
 // into its range of the global sorted order.
 
 // Field f of a store (planar components): 0 x, 1 v, 2 B, 3 Fe, 4 Fp,
-// 5 tau, 6 m, 7 V0, 8 mu0, 9 lambda0, 10 pid (moved as 32-bit words);
-// 11 mat is 16-bit.  Null for B without APIC.
+// 5 tau, 6 m, 7 V0, 8 mu0, 9 lambda0, 10 Jp, 11 pid (moved as 32-bit
+// words); kStoreFields - 1 = mat is 16-bit.  Null for B without APIC.
+constexpr int kStoreFields = 13;
 __device__ __forceinline__ uint32_t MPM_GLOBAL* store_field32(const ParticlesDev& P, int f, int& w) {
     float MPM_GLOBAL* p = nullptr;
     w = 1;
@@ -1954,6 +1980,7 @@ __device__ __forceinline__ uint32_t MPM_GLOBAL* store_field32(const ParticlesDev
         case 7: p = P.V0; break;
         case 8: p = P.mu0; break;
         case 9: p = P.lam0; break;
+        case 10: p = P.Jp; break;
         default: return P.pid;
     }
     return reinterpret_cast<uint32_t MPM_GLOBAL*>(p);
@@ -1999,7 +2026,7 @@ __global__ void __launch_bounds__(256) k_reorder_field(ParticlesDev P0, const Sc
         ps = tab[sc].pstart;
     }
     const int64_t l = g - ps, n = P.n;
-    if (f == 11) {
+    if (f == kStoreFields - 1) {
         uint16_t* s16 = reinterpret_cast<uint16_t*>(scratch);
         if (BACK) P.mat[l] = s16[i]; else s16[i] = P.mat[l];
         return;
@@ -2018,7 +2045,7 @@ void launch_reorder_store(const ParticlesDev& P, const SceneEntry* tab, int nsce
     if (ntot <= 0) return;
     const dim3 grid((unsigned)((ntot + 255) / 256)), block(256);
     if (tab) hipLaunchKernelGGL(k_slot_scenes, grid, block, 0, st, tab, nscenes, ntot, slot_sc);
-    for (int f = 0; f < 12; ++f) {
+    for (int f = 0; f < kStoreFields; ++f) {
         if (f == 2 && !apic) continue;
         if (tab) {
             hipLaunchKernelGGL((k_reorder_field<true, false>), grid, block, 0, st, P, tab, slot_sc, perm, ntot, f,

@@ -79,7 +79,7 @@ struct Solver {
     std::vector<void*> allocs;
     int64_t device_bytes = 0;
     // profiling
-    int prof = 0;  // 0 off, 1 every kernel, 2 the item kernels (p2g, g2p) and the store reorder only
+    int prof = 0;  // 0 off, 1 every kernel, 2 the item kernels (p2g, g2p, g2p2g) and the store reorder only
     std::vector<Pending> pending;
     std::vector<hipEvent_t> event_pool;
     std::map<std::string, std::pair<int64_t, double>> prof_tot;
@@ -109,6 +109,10 @@ struct Solver {
     // the device, copied to the host on a second stream
     float* pos4 = nullptr;
     hipStream_t copy_st = nullptr;
+    // the fused step clears the second grid buffer on aux_st while the sort
+    // runs on st (fork/join events)
+    hipStream_t aux_st = nullptr;
+    hipEvent_t aux_fork = nullptr, aux_join = nullptr;
     hipEvent_t pos4_packed = nullptr, pos4_copied = nullptr;
     bool pos4_pending = false;
     // store reorder: every reorder_interval steps (0 = never) the particle

@@ -364,8 +364,9 @@ class Solver:
         return {f[0]: getattr(s, f[0]) for f in N.MpmStats._fields_}
 
     def profile(self, enable=True):
-        """enable: False/0 off, True/1 every kernel group, 2 only p2g, g2p and
-        the store reorder (fewer events in the stream)."""
+        """enable: False/0 off, True/1 every kernel group, 2 only the item
+        kernels (p2g, g2p, g2p2g) and the store reorder (fewer events in
+        the stream)."""
         N.check(self._lib.mpm_profile_enable(self._h, int(enable)))
 
     def profile_reset(self):

@@ -24,6 +24,8 @@ def test_byte_model():
     assert bench.BYTES[1]["p2g"] + bench.BYTES[1]["g2p"] == 364
     assert bench.BYTES[0]["p2g"] + bench.BYTES[0]["g2p"] == 304
     assert bench.GRID_BYTES["p2g"] + bench.GRID_BYTES["g2p"] == 56
+    # the fused kernel is credited with both transfers
+    assert bench.BYTES[1]["g2p2g"] == 364 and bench.GRID_BYTES["g2p2g"] == 56
 
 
 def test_cpu_baseline_legs_small_scene():

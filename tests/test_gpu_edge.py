@@ -103,24 +103,28 @@ def test_cfl_monitor():
     assert np.isnan(s.stats()["max_speed"])
 
 
-def test_profile_levels():
+@pytest.mark.parametrize("fused", [False, True], ids=["two-pass", "fused"])
+def test_profile_levels(fused):
     # level 1 times every launch group, level 2 only the item kernels and the
-    # store reorder; other levels are rejected
+    # store reorder; other levels are rejected.  With transfer fusion the
+    # first step after creation runs P2G alone, every step the fused kernel.
     sc = scenes.snow_cube(seed=9, n=5000, res=32, transfer=N.APIC)
     s = Solver(sc.desc, sc.particles)
+    s.set_transfer_fusion(fused)
+    item = "g2p2g" if fused else "g2p"
     s.profile(1)
     s.profile_reset()
     s.step(3)
     full = s.profile_read()
-    assert {"p2g", "g2p", "sort_hist", "sort_scatter", "items", "grid_clear"} <= set(full), full
-    assert full["g2p"][0] == 3 and full["p2g"][0] == 3
+    assert {"p2g", item, "sort_hist", "sort_scatter", "items", "grid_clear"} <= set(full), full
+    assert full[item][0] == 3 and full["p2g"][0] == (1 if fused else 3)
     s.profile(2)
     s.profile_reset()
     s.set_reorder_interval(2)
     s.step(4)
     items = s.profile_read()
-    assert set(items) == {"p2g", "g2p", "reorder"}, items
-    assert items["g2p"][0] == 4 and items["g2p"][1] > 0.0
+    assert set(items) == ({"g2p2g", "reorder"} if fused else {"p2g", "g2p", "reorder"}), items
+    assert items[item][0] == 4 and items[item][1] > 0.0
     with pytest.raises(Exception):
         s.profile(3)
     s.profile(False)
@@ -136,5 +140,5 @@ def test_profile_levels():
     mixed = s.profile_read()
     s.profile(False)
     one = full["sort_hist"][1] / full["sort_hist"][0]
-    assert mixed["g2p"][0] == 4 and mixed["sort_hist"][0] >= 1  # the level-1 step's passes only
+    assert mixed[item][0] == 4 and mixed["sort_hist"][0] >= 1  # the level-1 step's passes only
     assert mixed["sort_hist"][1] / mixed["sort_hist"][0] < 20 * one + 0.05, (mixed["sort_hist"], one)
