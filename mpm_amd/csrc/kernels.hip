@@ -1264,7 +1264,14 @@ __device__ __forceinline__ void apic_gather(const float* tq, const float (&Nx)[W
 //   force    vs grad w             = dNx[a] (vs_0 yz) + Nx[a] (vs_1 dyz + vs_2 ydz)
 // Each node's four values are summed over the lanes of the particle's cell
 // (segmented scan) and the cell's last lane adds them to the tile at tq
-// (the stencil's first node).
+// (the stencil's first node).  The read-add-write of the tile is pipelined
+// by one node: a node's tile values are read before the next node's sums
+// are computed, and added and written after, so the LDS latency overlaps
+// arithmetic.  Writer lanes of different cells can reach one tile word
+// through different nodes, so every lane's write of node k must precede any
+// lane's read of node k + 1: the LDS accesses stay in node order (a wave's
+// LDS instructions execute in order; the compiler barriers keep the
+// compiler from reordering them).
 template <int W, int T, int TN>
 __device__ __forceinline__ void folded_cell_sums(float* tq, const SegPlan& sp, float m, const float (&vel)[3],
                                                  const float (&Cm)[3][3], const float (&vs)[3][3],
@@ -1278,6 +1285,8 @@ __device__ __forceinline__ void folded_cell_sums(float* tq, const SegPlan& sp, f
 #pragma unroll
         for (int k = 0; k < 3; ++k) cx[a][k] = vel[k] + Cm[k][0] * dxn[0][a];
     }
+    float pv[4], po[4];  // the pending node: its sums and the tile values read for it
+    int pq = 0;
 #pragma unroll
     for (int b = 0; b < W; ++b)
 #pragma unroll
@@ -1299,15 +1308,28 @@ __device__ __forceinline__ void folded_cell_sums(float* tq, const SegPlan& sp, f
                 for (int k = 0; k < 3; ++k) v[1 + k] = fmaf(mip, cx[a][k] + cyz[k], dNx[a] * g0[k] + Nx[a] * g12[k]);
                 seg_scan<4>(v, sp);
                 const int q = (c * T + b) * T + a;
+                const bool first = b == 0 && c == 0 && a == 0;
+                asm volatile("" ::: "memory");
                 if (sp.writer) {
-                    float o[4];
+                    if (!first) {
 #pragma unroll
-                    for (int f = 0; f < 4; ++f) o[f] = tq[f * TN + q];
+                        for (int f = 0; f < 4; ++f) tq[f * TN + pq] = po[f] + pv[f];
+                    }
+                    asm volatile("" ::: "memory");
 #pragma unroll
-                    for (int f = 0; f < 4; ++f) tq[f * TN + q] = o[f] + v[f];
+                    for (int f = 0; f < 4; ++f) po[f] = tq[f * TN + q];
                 }
+#pragma unroll
+                for (int f = 0; f < 4; ++f) pv[f] = v[f];
+                pq = q;
             }
+        }
+    asm volatile("" ::: "memory");
+    if (sp.writer) {
+#pragma unroll
+        for (int f = 0; f < 4; ++f) tq[f * TN + pq] = po[f] + pv[f];
     }
+    asm volatile("" ::: "memory");
 }
 
 // A particle that changed cell: its folded contributions straight to the
