@@ -355,12 +355,37 @@ void ensure_stress(Solver& s) {
     s.stress_valid = true;
 }
 
-// The fused step's sort of a large scene: the plan word chooses between the
+// Every 8 steps the count of incremental sorts is copied into pinned host
+// memory on the stream; once the copy has landed (an event query, no wait)
+// the share of updates since the previous reading sets few_passes: mostly
+// updates (>= 3/4) -> the full sort's kernels, which then mostly return at
+// once, use 3 passes instead of 4.  Only the speed depends on it.
+static void sample_inc_history(Solver& s) {
+    if (s.capturing || !s.inc_host || !s.inc_ev) return;
+    if (s.inc_pending && hipEventQuery(s.inc_ev) == hipSuccess) {
+        const uint32_t now = *reinterpret_cast<volatile uint32_t*>(s.inc_host);
+        if (s.inc_prev_step >= 0 && s.inc_copy_step > s.inc_prev_step)
+            s.few_passes = 4.0 * (double)(now - s.inc_prev) >= 3.0 * (double)(s.inc_copy_step - s.inc_prev_step);
+        s.inc_prev = now;
+        s.inc_prev_step = s.inc_copy_step;
+        s.inc_pending = false;
+    }
+    if (!s.inc_pending && s.steps - s.inc_copy_step >= 8) {
+        if (hipMemcpyAsync(s.inc_host, s.plan + kPlanIncSteps, 4, hipMemcpyDeviceToHost, s.st) != hipSuccess ||
+            hipEventRecord(s.inc_ev, s.st) != hipSuccess)
+            return;
+        s.inc_copy_step = s.steps;
+        s.inc_pending = true;
+    }
+}
+
+// The fused step's sort of a large scene: the plan words choose between the
 // incremental update of the last step's order (few moved particles) and the
 // full radix sort; both write the same buffers, the other's launches return
 // at once.  Bit-identical to run_sort.
 void run_sort_incremental(Solver& s) {
     ProfileHook h = hook(s);
+    sample_inc_history(s);
     if (!s.keys_valid) {
         Scope sc(h, "keys");
         launch_keys(s.S, s.P, s.interp, s.kin, s.st);
@@ -368,12 +393,8 @@ void run_sort_incremental(Solver& s) {
         s.snap_ok = false;
     }
     if (!s.snap_ok) launch_inc_set(s.plan, 0u, s.st);  // the full sort this time
-    {
-        Scope sc(h, "sort_plan");
-        launch_inc_plan(s.plan, s.st);
-    }
     const int sel = radix_sort_pairs(s.kin, s.iota, s.ka, s.va, s.kb, s.vb, s.N, s.key_bits, s.hist, s.scan_temp,
-                                     s.st, h, s.plan);
+                                     s.st, h, s.plan, s.few_passes);
     uint32_t* out_k = sel ? s.kb : s.ka;
     uint32_t* out_p = sel ? s.vb : s.va;
     s.I.sorted_keys = out_k;
@@ -387,8 +408,9 @@ void run_sort_incremental(Solver& s) {
         launch_inc_sort(s.plan, s.moves, s.snap_k, s.snap_p, s.N, out_k, out_p, s.st);
     }
     {
-        Scope sc(h, "items");
-        launch_items(s.I.sorted_keys, s.N, s.flags, s.scan_temp, s.items, s.counters, s.st);
+        Scope sc(h, "items");  // also resets the plan words for this step's G2P
+        launch_items(s.I.sorted_keys, s.N, s.flags, s.scan_temp, s.items, s.counters, s.st, 0, nullptr, 0, nullptr,
+                     s.plan);
     }
     s.sort_valid = true;
 }
@@ -580,11 +602,14 @@ int build(Solver& s, HostState& H, int run_init) {
         if ((rc = dalloc(s, (void**)&s.snap_p, (size_t)n * 4))) return rc;
         if ((rc = dalloc(s, (void**)&s.plan, kPlanWords * 4))) return rc;
         HIP_TRY(hipMemsetAsync(s.plan, 0, kPlanWords * 4, s.st));  // snapshot not valid yet
-        uint32_t** lists[] = {&s.moves.key, &s.moves.slot, &s.moves.pos, &s.moves.lb, &s.moves.rank,
-                              &s.moves.drank, &s.moves.dbelow, &s.moves.del_sorted, &s.moves.lb_sorted};
+        uint32_t** lists[] = {&s.moves.key, &s.moves.slot, &s.moves.pos, &s.moves.lb, &s.moves.del_sorted,
+                              &s.moves.lb_sorted};
         for (auto pp : lists)
             if ((rc = dalloc(s, (void**)pp, (size_t)kMoveMax * 4))) return rc;
+        if ((rc = dalloc(s, (void**)&s.moves.part, (size_t)3 * kIncSlices * kMoveMax * 4))) return rc;
         if ((rc = dalloc(s, (void**)&s.moves_dev, sizeof(MoveList)))) return rc;
+        HIP_TRY(hipHostMalloc((void**)&s.inc_host, 64, hipHostMallocDefault));
+        HIP_TRY(hipEventCreateWithFlags(&s.inc_ev, hipEventDisableTiming));
         HIP_TRY(hipMemcpyAsync(s.moves_dev, &s.moves, sizeof(MoveList), hipMemcpyHostToDevice, s.st));
         HIP_TRY(hipStreamSynchronize(s.st));  // the host copy is a member: it outlives the copy anyway
     }
@@ -641,6 +666,8 @@ void destroy(Solver* s) {
         (void)hipStreamDestroy(s->aux_st);
     }
     if (s->aux_fork) (void)hipEventDestroy(s->aux_fork);
+    if (s->inc_ev) (void)hipEventDestroy(s->inc_ev);
+    if (s->inc_host) (void)hipHostFree(s->inc_host);
     if (s->aux_join) (void)hipEventDestroy(s->aux_join);
     if (s->pos4_packed) (void)hipEventDestroy(s->pos4_packed);
     if (s->pos4_copied) (void)hipEventDestroy(s->pos4_copied);

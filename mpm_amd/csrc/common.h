@@ -102,23 +102,28 @@ struct SceneEntry {
 
 // Incremental cell sort of one large scene (incsort.hip): G2P appends each
 // particle whose key changed -- new key, slot, old sorted position -- to a
-// list of at most kMoveMax entries; the next step's plan word picks the
-// incremental update or the full radix sort.
+// list of at most kMoveMax entries; the next step's sort is the incremental
+// update when the plan words allow it, else the full radix sort.
 constexpr int kMoveMax = 16384;
+constexpr int kIncSlices = 16;  // the update's counting: slices of the list per entry block
 enum PlanWord {
-    kPlanMoved = 0,     // moves appended by this step's G2P (past kMoveMax: counted, not stored)
-    kPlanMode = 1,      // this step's choice: kPlanFull or kPlanIncremental
-    kPlanCount = 2,     // list entries the update consumes
-    kPlanSnapOk = 3,    // the snapshot is the previous step's order
-    kPlanIncSteps = 4,  // steps sorted by the update since creation
+    kPlanMoved = 0,     // moves appended by the last G2P (past kMoveMax: counted, not stored)
+    kPlanSnapOk = 1,    // the snapshot is the previous step's order
+    kPlanIncSteps = 2,  // steps sorted by the update since creation
     kPlanWords = 16
 };
-constexpr uint32_t kPlanFull = 0u;
-constexpr uint32_t kPlanIncremental = 1u;
+// The step's choice, read by every sort kernel: the update when the
+// snapshot is the previous step's order and the list holds every move.
+// The words change only between sorts (G2P appends; the item count of the
+// step resets them).
+__host__ __device__ inline bool plan_incremental(const uint32_t* plan) {
+    return plan[kPlanSnapOk] != 0u && plan[kPlanMoved] <= (uint32_t)kMoveMax;
+}
 struct MoveList {
-    uint32_t *key, *slot, *pos;            // appended by G2P
-    uint32_t *lb, *rank, *drank, *dbelow;  // per entry, during the update
-    uint32_t *del_sorted, *lb_sorted;      // removed positions, insertion places (ascending)
+    uint32_t *key, *slot, *pos;        // appended by G2P
+    uint32_t* lb;                      // per entry: its place among the snapshot
+    uint32_t* part;                    // [3][kIncSlices][kMoveMax] partial counts
+    uint32_t *del_sorted, *lb_sorted;  // removed positions, insertion places (ascending)
 };
 
 // Work items of the block-tiled kernels: a contiguous run of sorted

@@ -5,19 +5,20 @@
 // slab per step).  The sorted order of step n+1 -- the stable sort of the
 // new keys, i.e. ascending (key, slot) -- is the sorted order of step n
 // with the moved particles taken out and put back at their new places:
-//   * P2G copies step n's sorted keys and permutation (the snapshot) as it
-//     reads them;
+//   * the transfer kernel copies step n's sorted keys and permutation (the
+//     snapshot) as it reads them;
 //   * G2P appends each particle whose new key differs from its old one to a
 //     short list (new key, slot, old sorted position);
 //   * here the list is ranked by (key, slot) (by counting: it is short), each
-//     entry finds its place among the snapshot by binary search, and one
+//     entry finds its place among the snapshot by a 64-ary search, and one
 //     pass over the snapshot moves every unmoved entry by
 //     (#list entries placed before it) - (#moved entries before it).
-// The result is bit-identical to a stable radix sort of the new keys.  A
-// device-side plan word picks this path or the full radix sort each step
+// The result is bit-identical to a stable radix sort of the new keys.  The
+// device-side plan words pick this path or the full radix sort each step
 // (the other path's launches return at once): the full sort runs when the
 // list overflowed (more than kMoveMax moves) or the snapshot is not the
-// previous step's (first step, state written, store reordered).
+// previous step's (first step, state written, store reordered).  The step's
+// item count resets the words afterwards (launch_items with plan).
 #include <hip/hip_runtime.h>
 #include <stdint.h>
 
@@ -26,26 +27,12 @@
 namespace mpm {
 
 constexpr int kIncWG = 256;
-constexpr int kIncTile = 2048;  // snapshot entries per merge workgroup
-constexpr int kIncSlices = 16;  // ranking: slices of the list per entry block
+constexpr int kIncTile = 2048;   // snapshot entries per merge workgroup
 constexpr int kIncStage = 1024;  // list entries staged per merge workgroup
 
 // (key, slot) order of two sort entries.
 __device__ __forceinline__ bool entry_less(uint32_t ka, uint32_t sa, uint32_t kb, uint32_t sb) {
     return ka < kb || (ka == kb && sa < sb);
-}
-
-// plan word: mode, moves consumed, snapshot valid; the move counter is
-// reset for this step's G2P.
-__global__ void k_inc_plan(uint32_t* plan) {
-    if (threadIdx.x != 0) return;
-    const uint32_t moved = plan[kPlanMoved];
-    const bool incr = plan[kPlanSnapOk] != 0u && moved <= (uint32_t)kMoveMax;
-    plan[kPlanMode] = incr ? kPlanIncremental : kPlanFull;
-    plan[kPlanCount] = incr ? moved : 0u;
-    plan[kPlanIncSteps] += incr ? 1u : 0u;
-    plan[kPlanMoved] = 0u;
-    plan[kPlanSnapOk] = 1u;
 }
 
 __global__ void k_inc_set(uint32_t* plan, uint32_t snap_ok) {
@@ -55,39 +42,46 @@ __global__ void k_inc_set(uint32_t* plan, uint32_t snap_ok) {
 }
 
 // Each list entry's place among the snapshot (the number of snapshot
-// entries before it in (key, slot) order); clears the entry's counters.
+// entries before it in (key, slot) order), one wave per entry: every round
+// the 64 lanes probe evenly spaced entries of the current range and the
+// ballot of "probe < (key, slot)" (a prefix of the lanes: the snapshot is
+// sorted) narrows the range ~65-fold -- 4 rounds of dependent loads for 8.4M
+// entries instead of 23 for a binary search.
 __global__ void __launch_bounds__(kIncWG) k_inc_locate(const uint32_t* __restrict__ plan, MoveList L,
                                                        const uint32_t* __restrict__ snap_k,
                                                        const uint32_t* __restrict__ snap_p, int64_t n) {
-    if (plan[kPlanMode] != kPlanIncremental) return;
-    const uint32_t c = plan[kPlanCount];
-    const uint32_t j = blockIdx.x * kIncWG + threadIdx.x;
-    if (j >= c) return;
-    const uint32_t k = L.key[j], s = L.slot[j];
-    int64_t lo = 0, hi = n;  // first index whose entry is not less than (k, s)
-    while (lo < hi) {
-        const int64_t mid = (lo + hi) >> 1;
-        if (entry_less(snap_k[mid], snap_p[mid], k, s)) lo = mid + 1;
-        else hi = mid;
+    if (!plan_incremental(plan)) return;
+    const uint32_t c = plan[kPlanMoved];
+    const int lane = threadIdx.x & 63;
+    const uint32_t waves = gridDim.x * (kIncWG / 64);
+    for (uint32_t j = blockIdx.x * (kIncWG / 64) + (threadIdx.x >> 6); j < c; j += waves) {  // wave-uniform
+        const uint32_t k = L.key[j], s = L.slot[j];
+        int64_t lo = 0, hi = n;  // the place lies in [lo, hi]
+        for (int round = 0; round < 64 && lo < hi; ++round) {
+            const int64_t span = hi - lo;
+            const int64_t p = lo + span * (lane + 1) / 65;  // in [lo, hi)
+            const int below = __popcll(__ballot(entry_less(snap_k[p], snap_p[p], k, s)));
+            // lanes 0 .. below-1 probed entries before the place, lane `below` one at or after it
+            const int64_t nlo = below > 0 ? lo + span * below / 65 + 1 : lo;
+            const int64_t nhi = below < 64 ? lo + span * (below + 1) / 65 : hi;
+            lo = nlo;
+            hi = nhi;
+        }
+        if (lane == 0) L.lb[j] = (uint32_t)lo;
     }
-    L.lb[j] = (uint32_t)lo;
-    L.rank[j] = 0u;
-    L.drank[j] = 0u;
-    L.dbelow[j] = 0u;
 }
 
 // Counting over the list: rank = entries before this one in (key, slot)
 // order; drank = entries whose old position is smaller; dbelow = old
 // positions below this entry's place.  blockIdx.y takes one slice of the
-// other entries; partial counts are added atomically.
+// other entries and stores its partial counts (summed by k_inc_place).
 __global__ void __launch_bounds__(kIncWG) k_inc_count(const uint32_t* __restrict__ plan, MoveList L) {
-    if (plan[kPlanMode] != kPlanIncremental) return;
-    const uint32_t c = plan[kPlanCount];
+    if (!plan_incremental(plan)) return;
+    const uint32_t c = plan[kPlanMoved];
     const uint32_t j0 = blockIdx.x * kIncWG;
     if (j0 >= c) return;
     const uint32_t per = (c + kIncSlices - 1) / kIncSlices;
     const uint32_t e0 = blockIdx.y * per, e1 = e0 + per < c ? e0 + per : c;
-    if (e0 >= e1) return;
     __shared__ uint32_t sk[kIncWG], ss[kIncWG], sp[kIncWG];
     const uint32_t j = j0 + threadIdx.x;
     const bool own = j < c;
@@ -103,6 +97,7 @@ __global__ void __launch_bounds__(kIncWG) k_inc_count(const uint32_t* __restrict
         }
         __syncthreads();
         const uint32_t m = e1 - b < (uint32_t)kIncWG ? e1 - b : (uint32_t)kIncWG;
+#pragma unroll 8
         for (uint32_t q = 0; q < m; ++q) {
             rank += entry_less(sk[q], ss[q], k, s) ? 1u : 0u;
             drank += sp[q] < pos ? 1u : 0u;
@@ -111,9 +106,10 @@ __global__ void __launch_bounds__(kIncWG) k_inc_count(const uint32_t* __restrict
         __syncthreads();
     }
     if (own) {
-        if (rank) atomicAdd(&L.rank[j], rank);
-        if (drank) atomicAdd(&L.drank[j], drank);
-        if (dbelow) atomicAdd(&L.dbelow[j], dbelow);
+        uint32_t* const part = L.part + (size_t)blockIdx.y * kMoveMax + j;
+        part[0] = rank;
+        part[(size_t)kIncSlices * kMoveMax] = drank;
+        part[(size_t)2 * kIncSlices * kMoveMax] = dbelow;
     }
 }
 
@@ -121,12 +117,20 @@ __global__ void __launch_bounds__(kIncWG) k_inc_count(const uint32_t* __restrict
 // positions and of insertion places for the merge.
 __global__ void __launch_bounds__(kIncWG) k_inc_place(const uint32_t* __restrict__ plan, MoveList L, int64_t n,
                                                       uint32_t* __restrict__ out_k, uint32_t* __restrict__ out_p) {
-    if (plan[kPlanMode] != kPlanIncremental) return;
-    const uint32_t c = plan[kPlanCount];
+    if (!plan_incremental(plan)) return;
+    const uint32_t c = plan[kPlanMoved];
     const uint32_t j = blockIdx.x * kIncWG + threadIdx.x;
     if (j >= c) return;
-    const uint32_t rank = L.rank[j], lb = L.lb[j], drank = L.drank[j];
-    const uint32_t dst = rank + lb - L.dbelow[j];  // < n for a list that matches the snapshot
+    uint32_t rank = 0, drank = 0, dbelow = 0;
+#pragma unroll
+    for (int y = 0; y < kIncSlices; ++y) {
+        const uint32_t* const part = L.part + (size_t)y * kMoveMax + j;
+        rank += part[0];
+        drank += part[(size_t)kIncSlices * kMoveMax];
+        dbelow += part[(size_t)2 * kIncSlices * kMoveMax];
+    }
+    const uint32_t lb = L.lb[j];
+    const uint32_t dst = rank + lb - dbelow;  // < n for a list that matches the snapshot
     if ((int64_t)dst < n) {
         out_k[dst] = L.key[j];
         out_p[dst] = L.slot[j];
@@ -150,12 +154,14 @@ __device__ __forceinline__ uint32_t count_below(const uint32_t* a, uint32_t n, u
 // i - (moved entries before i) + (list entries placed at or before i).
 // The few removed positions and insertion places of the tile are staged
 // in LDS (global binary search when a tile holds more than kIncStage).
+// Consecutive lanes take consecutive entries, so that both the loads and
+// the (shifted) stores of a wave stay contiguous.
 __global__ void __launch_bounds__(kIncWG) k_inc_merge(const uint32_t* __restrict__ plan, MoveList L,
                                                       const uint32_t* __restrict__ snap_k,
                                                       const uint32_t* __restrict__ snap_p, int64_t n,
                                                       uint32_t* __restrict__ out_k, uint32_t* __restrict__ out_p) {
-    if (plan[kPlanMode] != kPlanIncremental) return;
-    const uint32_t c = plan[kPlanCount];
+    if (!plan_incremental(plan)) return;
+    const uint32_t c = plan[kPlanMoved];
     __shared__ uint32_t bounds[4];
     __shared__ uint32_t sdel[kIncStage], slb[kIncStage];
     const int64_t i0 = (int64_t)blockIdx.x * kIncTile;
@@ -200,16 +206,14 @@ void launch_inc_set(uint32_t* plan, uint32_t snap_ok, hipStream_t st) {
     hipLaunchKernelGGL(k_inc_set, dim3(1), dim3(64), 0, st, plan, snap_ok);
 }
 
-void launch_inc_plan(uint32_t* plan, hipStream_t st) {
-    hipLaunchKernelGGL(k_inc_plan, dim3(1), dim3(64), 0, st, plan);
-}
-
 void launch_inc_sort(const uint32_t* plan, const MoveList& L, const uint32_t* snap_k, const uint32_t* snap_p,
                      int64_t n, uint32_t* out_k, uint32_t* out_p, hipStream_t st) {
-    const unsigned blocks = (unsigned)((kMoveMax + kIncWG - 1) / kIncWG);
-    hipLaunchKernelGGL(k_inc_locate, dim3(blocks), dim3(kIncWG), 0, st, plan, L, snap_k, snap_p, n);
-    hipLaunchKernelGGL(k_inc_count, dim3(blocks, kIncSlices), dim3(kIncWG), 0, st, plan, L);
-    hipLaunchKernelGGL(k_inc_place, dim3(blocks), dim3(kIncWG), 0, st, plan, L, n, out_k, out_p);
+    const unsigned entry_blocks = (unsigned)((kMoveMax + kIncWG - 1) / kIncWG);
+    // 4096 waves: one entry each up to 4096 moves (the slab moves ~4000
+    // particles a step), a few each beyond
+    hipLaunchKernelGGL(k_inc_locate, dim3(1024), dim3(kIncWG), 0, st, plan, L, snap_k, snap_p, n);
+    hipLaunchKernelGGL(k_inc_count, dim3(entry_blocks, kIncSlices), dim3(kIncWG), 0, st, plan, L);
+    hipLaunchKernelGGL(k_inc_place, dim3(entry_blocks), dim3(kIncWG), 0, st, plan, L, n, out_k, out_p);
     const unsigned tiles = (unsigned)((n + kIncTile - 1) / kIncTile);
     hipLaunchKernelGGL(k_inc_merge, dim3(tiles), dim3(kIncWG), 0, st, plan, L, snap_k, snap_p, n, out_k, out_p);
 }

@@ -142,27 +142,58 @@ void launch_iota(uint32_t* p, int64_t n, hipStream_t st) {
 
 // ----------------------------------------------------------------- items
 // A work item starts where the key block changes (in a batch this includes
-// every scene boundary, since scene key ranges are 64-aligned) and every
-// item_max particles within a block.
-__device__ __forceinline__ bool item_head(const uint32_t* sk, int64_t i, uint32_t item_max) {
-    return i == 0 || (sk[i] >> 6) != (sk[i - 1] >> 6) || ((uint64_t)i % item_max) == 0;
+// every scene boundary, since scene key ranges are 64-aligned) and at
+// every multiple of item_max within a block.
+// The head flags of a thread's IPT consecutive sorted keys from base: the
+// keys in 16-byte loads and the key before base.
+template <int IPT>
+__device__ __forceinline__ void item_heads(const uint32_t* __restrict__ sk, int64_t n, int64_t base,
+                                           uint32_t item_max, uint32_t (&key)[IPT], bool (&head)[IPT]) {
+    static_assert(IPT % 4 == 0, "keys are read four at a time");
+    if (base + IPT <= n) {
+#pragma unroll
+        for (int j = 0; j < IPT; j += 4) {
+            const uint4 k4 = *reinterpret_cast<const uint4*>(sk + base + j);  // base is a multiple of IPT
+            key[j] = k4.x; key[j + 1] = k4.y; key[j + 2] = k4.z; key[j + 3] = k4.w;
+        }
+    } else {
+#pragma unroll
+        for (int j = 0; j < IPT; ++j) key[j] = base + j < n ? sk[base + j] : 0u;
+    }
+    uint32_t prev = base > 0 && base <= n ? sk[base - 1] : 0u;
+#pragma unroll
+    for (int j = 0; j < IPT; ++j) {
+        const int64_t i = base + j;
+        head[j] = i < n && (i == 0 || (key[j] >> 6) != (prev >> 6) || ((uint64_t)i % item_max) == 0);
+        prev = key[j];
+    }
 }
 // Work items in two passes over the sorted keys, with the head flags
 // recomputed in each pass instead of stored: per-tile head counts, a scan
 // of the tile counts, then every head written at its rank.  items[k] is the
 // start of item k, items[n + k] its scene-local key block and, for a batch,
 // item_scene[k] its scene (the last scene whose key offset is <= the item's
-// first key).
+// first key).  plan (the incremental sort's words, or null): the sort of the
+// step is done, so the first workgroup counts an incremental sort and
+// resets the move counter for the step's G2P (and marks the snapshot that
+// the step's transfer takes as valid).
+template <int IPT>
 __global__ void __launch_bounds__(kScanWG) k_item_count(const uint32_t* __restrict__ sk, int64_t n,
-                                                        uint32_t item_max, uint32_t* __restrict__ partial) {
+                                                        uint32_t item_max, uint32_t* __restrict__ partial,
+                                                        uint32_t* __restrict__ plan) {
     __shared__ uint32_t ws[kScanWG / 64];
-    const int64_t base = (int64_t)blockIdx.x * kScanTile + (int64_t)threadIdx.x * kScanIPT;
+    if (plan && blockIdx.x == 0 && threadIdx.x == 0) {
+        plan[kPlanIncSteps] += plan_incremental(plan) ? 1u : 0u;
+        plan[kPlanMoved] = 0u;
+        plan[kPlanSnapOk] = 1u;
+    }
+    const int64_t base = ((int64_t)blockIdx.x * kScanWG + threadIdx.x) * IPT;
+    uint32_t key[IPT];
+    bool head[IPT];
+    item_heads<IPT>(sk, n, base, item_max, key, head);
     uint32_t c = 0;
 #pragma unroll
-    for (int j = 0; j < kScanIPT; ++j) {
-        const int64_t i = base + j;
-        if (i < n && item_head(sk, i, item_max)) ++c;
-    }
+    for (int j = 0; j < IPT; ++j) c += head[j] ? 1u : 0u;
     uint32_t tot;
     block_exclusive_scan(c, ws, tot);
     if (threadIdx.x == 0) partial[blockIdx.x] = tot;
@@ -170,7 +201,7 @@ __global__ void __launch_bounds__(kScanWG) k_item_count(const uint32_t* __restri
 // tile_start: the scanned tile counts, or with RAW the raw counts, which
 // each workgroup sums over the tiles before it (and the last one writes the
 // item count): few tiles, one launch fewer.
-template <bool RAW>
+template <bool RAW, int IPT>
 __global__ void __launch_bounds__(kScanWG) k_item_emit(const uint32_t* __restrict__ sk, int64_t n, uint32_t item_max,
                                                        const uint32_t* __restrict__ tile_start,
                                                        uint32_t* __restrict__ items,
@@ -186,23 +217,21 @@ __global__ void __launch_bounds__(kScanWG) k_item_emit(const uint32_t* __restric
     } else {
         first = tile_start[blockIdx.x];
     }
-    const int64_t base = (int64_t)blockIdx.x * kScanTile + (int64_t)threadIdx.x * kScanIPT;
-    bool head[kScanIPT];
+    const int64_t base = ((int64_t)blockIdx.x * kScanWG + threadIdx.x) * IPT;
+    uint32_t keys[IPT];
+    bool head[IPT];
+    item_heads<IPT>(sk, n, base, item_max, keys, head);
     uint32_t c = 0;
 #pragma unroll
-    for (int j = 0; j < kScanIPT; ++j) {
-        const int64_t i = base + j;
-        head[j] = i < n && item_head(sk, i, item_max);
-        c += head[j] ? 1u : 0u;
-    }
+    for (int j = 0; j < IPT; ++j) c += head[j] ? 1u : 0u;
     uint32_t tot;
     uint32_t k = first + block_exclusive_scan(c, ws, tot);
     if (RAW && blockIdx.x == gridDim.x - 1 && threadIdx.x == 0) *item_count = first + tot;
 #pragma unroll
-    for (int j = 0; j < kScanIPT; ++j) {
+    for (int j = 0; j < IPT; ++j) {
         if (!head[j]) continue;
         const int64_t i = base + j;
-        const uint32_t key = sk[i];
+        const uint32_t key = keys[j];
         items[k] = (uint32_t)i;
         uint32_t koff = 0;
         if (item_scene) {
@@ -220,9 +249,28 @@ __global__ void __launch_bounds__(kScanWG) k_item_emit(const uint32_t* __restric
     }
 }
 
+template <int IPT>
+static void launch_items_ipt(const uint32_t* sorted_keys, int64_t n, uint32_t* flags, uint32_t* scan_temp,
+                             uint32_t* items, uint32_t* item_count, hipStream_t st, uint32_t item_max,
+                             const uint32_t* key_offsets, int nscenes, uint32_t* item_scene, uint32_t* plan) {
+    const int64_t tiles = (n + kScanWG * IPT - 1) / (kScanWG * IPT);
+    // flags[0..tiles) holds the tile head counts (then, for many tiles,
+    // their exclusive scan)
+    hipLaunchKernelGGL(k_item_count<IPT>, dim3((unsigned)tiles), dim3(kScanWG), 0, st, sorted_keys, n, item_max,
+                       flags, plan);
+    if (tiles <= 2048) {
+        hipLaunchKernelGGL((k_item_emit<true, IPT>), dim3((unsigned)tiles), dim3(kScanWG), 0, st, sorted_keys, n,
+                           item_max, flags, items, key_offsets, nscenes, item_scene, item_count);
+        return;
+    }
+    exclusive_scan(flags, tiles, scan_temp, item_count, st);
+    hipLaunchKernelGGL((k_item_emit<false, IPT>), dim3((unsigned)tiles), dim3(kScanWG), 0, st, sorted_keys, n,
+                       item_max, flags, items, key_offsets, nscenes, item_scene, item_count);
+}
+
 void launch_items(const uint32_t* sorted_keys, int64_t n, uint32_t* flags, uint32_t* scan_temp, uint32_t* items,
                   uint32_t* item_count, hipStream_t st, uint32_t item_max, const uint32_t* key_offsets,
-                  int nscenes, uint32_t* item_scene) {
+                  int nscenes, uint32_t* item_scene, uint32_t* plan) {
     if (n <= 0) return;
     if (item_max == 0) {
         // a multiple of 64 up to kItemMax that still gives about four items
@@ -231,18 +279,14 @@ void launch_items(const uint32_t* sorted_keys, int64_t n, uint32_t* flags, uint3
         const int64_t per = (n + 4 * 64 * (int64_t)device_cus() - 1) / (4 * 64 * (int64_t)device_cus());
         item_max = (uint32_t)(64 * (per < 1 ? 1 : (per > kItemMax / 64 ? kItemMax / 64 : per)));
     }
-    const int64_t tiles = (n + kScanTile - 1) / kScanTile;
-    // flags[0..tiles) holds the tile head counts (then, for many tiles,
-    // their exclusive scan)
-    hipLaunchKernelGGL(k_item_count, dim3((unsigned)tiles), dim3(kScanWG), 0, st, sorted_keys, n, item_max, flags);
-    if (tiles <= 2048) {
-        hipLaunchKernelGGL(k_item_emit<true>, dim3((unsigned)tiles), dim3(kScanWG), 0, st, sorted_keys, n, item_max,
-                           flags, items, key_offsets, nscenes, item_scene, item_count);
-        return;
-    }
-    exclusive_scan(flags, tiles, scan_temp, item_count, st);
-    hipLaunchKernelGGL(k_item_emit<false>, dim3((unsigned)tiles), dim3(kScanWG), 0, st, sorted_keys, n, item_max,
-                       flags, items, key_offsets, nscenes, item_scene, item_count);
+    // up to 2048 tiles the emit pass sums the tile counts itself (no scan
+    // launches): 4 keys per thread up to 2M keys, then 16
+    if (n <= (int64_t)2048 * kScanWG * 4)
+        launch_items_ipt<4>(sorted_keys, n, flags, scan_temp, items, item_count, st, item_max, key_offsets, nscenes,
+                            item_scene, plan);
+    else
+        launch_items_ipt<16>(sorted_keys, n, flags, scan_temp, items, item_count, st, item_max, key_offsets, nscenes,
+                             item_scene, plan);
 }
 
 // ------------------------------------------------------- item decoding

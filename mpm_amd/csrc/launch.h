@@ -21,23 +21,21 @@ struct ProfileHook {
 
 // ---- sort.hip
 int64_t scan_temp_words(int64_t n);
-// plan (device, may be null): every launch returns at once unless
-// plan[kPlanMode] == kPlanFull.
+// plan (device, may be null): every launch returns at once when the plan
+// words choose the incremental update (plan_incremental).
 void exclusive_scan(uint32_t* data, int64_t n, uint32_t* temp, uint32_t* total, hipStream_t st,
-                    const uint32_t* plan = nullptr);
+                    const uint32_t* plan = nullptr, int max_wgs = 0);
 int64_t radix_tiles(int64_t n);
 int64_t radix_hist_words(int64_t n);
-int radix_passes(int key_bits, int64_t n);
+int radix_passes(int key_bits, int64_t n, bool few_passes = false);
 int radix_sort_pairs(const uint32_t* kin, const uint32_t* vin, uint32_t* ka, uint32_t* va, uint32_t* kb, uint32_t* vb,
                      int64_t n, int key_bits, uint32_t* hist, uint32_t* scan_temp, hipStream_t st,
-                     const ProfileHook& prof, const uint32_t* plan = nullptr);
+                     const ProfileHook& prof, const uint32_t* plan = nullptr, bool few_passes = false);
 
 // ---- incsort.hip
 // plan words: reset the move counter and set whether the snapshot is the
 // previous step's (0 forces the full sort next).
 void launch_inc_set(uint32_t* plan, uint32_t snap_ok, hipStream_t st);
-// The step's choice between the incremental update and the full sort.
-void launch_inc_plan(uint32_t* plan, hipStream_t st);
 // The incremental update (returns at once unless the plan chose it): the
 // snapshot (sorted by the previous keys) and the move list -> out_k/out_p.
 void launch_inc_sort(const uint32_t* plan, const MoveList& L, const uint32_t* snap_k, const uint32_t* snap_p,
@@ -58,7 +56,7 @@ void launch_iota(uint32_t* p, int64_t n, hipStream_t st);
 // each item's scene in item_scene.
 void launch_items(const uint32_t* sorted_keys, int64_t n, uint32_t* flags, uint32_t* scan_temp, uint32_t* items,
                   uint32_t* item_count, hipStream_t st, uint32_t item_max = 0, const uint32_t* key_offsets = nullptr,
-                  int nscenes = 0, uint32_t* item_scene = nullptr);
+                  int nscenes = 0, uint32_t* item_scene = nullptr, uint32_t* plan = nullptr);
 
 // grid_wgs: persistent workgroups; 0 = one resident round (occupancy of the
 // kernel instance x compute units).

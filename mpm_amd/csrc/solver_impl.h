@@ -130,6 +130,15 @@ struct Solver {
     uint32_t* plan = nullptr;
     MoveList moves{};
     MoveList* moves_dev = nullptr;  // device copy of `moves` (the kernels' view)
+    // how often the update ran lately: the incremental-sort count read back
+    // every few steps through pinned memory (never waited for); mostly
+    // updates -> the full sort's launches use fewer passes (few_passes)
+    uint32_t* inc_host = nullptr;
+    hipEvent_t inc_ev = nullptr;
+    bool inc_pending = false;
+    int64_t inc_copy_step = 0, inc_prev_step = -1;
+    uint32_t inc_prev = 0;
+    bool few_passes = false;
     // fused transfer (APIC folded step, linear or quadratic): G2P scatters
     // the next step's P2G into the second grid buffer (same size as the
     // first: the two alternate), which then becomes the grid.  prescat:

@@ -10,6 +10,10 @@
 //             written out so that runs of one digit are contiguous stores.
 // Stability follows from ranking by (tile, chunk, lane) = input order.
 // No look-back spinning: every wave runs to completion independently.
+// Every kernel loops over its tiles (workgroup b takes tiles b, b + grid,
+// ...), so a sort can be launched on fewer workgroups than tiles: a sort
+// whose launches mostly return at once (the plan words chose the
+// incremental update) then costs fewer workgroup dispatches.
 #include <hip/hip_runtime.h>
 #include <stdint.h>
 
@@ -25,6 +29,7 @@ constexpr int kSortIPT = 8;
 constexpr int kSortTile = kSortWG * kSortIPT;  // 2048 keys per tile
 constexpr int kSortChunks = kSortTile / 64;    // 32 chunks of 64 keys
 constexpr int kChunksPerWave = kSortChunks / (kSortWG / 64);
+constexpr int kLightGrid = 512;  // workgroups of a sort launched with few_passes
 
 template <int BITS>
 __device__ __forceinline__ uint64_t digit_match(uint32_t digit, uint64_t active) {
@@ -42,29 +47,31 @@ template <int BITS>
 __global__ void __launch_bounds__(kSortWG) k_radix_hist(const uint32_t* __restrict__ keys, int64_t n, int shift,
                                                         uint32_t mask, uint32_t* __restrict__ hist, int64_t tiles,
                                                         const uint32_t* __restrict__ plan) {
-    if (plan && plan[kPlanMode] != kPlanFull) return;
+    if (plan && plan_incremental(plan)) return;
     constexpr int kRadix = 1 << BITS;
     __shared__ uint32_t h[kRadix];
     const int tid = threadIdx.x;
     const int lane = tid & 63;
-    for (int i = tid; i < kRadix; i += kSortWG) h[i] = 0;
-    __syncthreads();
-    const int64_t tile = blockIdx.x;
-    const int64_t base = tile * kSortTile;
     const int wave = tid >> 6;
+    for (int64_t tile = blockIdx.x; tile < tiles; tile += gridDim.x) {
+        for (int i = tid; i < kRadix; i += kSortWG) h[i] = 0;
+        __syncthreads();
+        const int64_t base = tile * kSortTile;
 #pragma unroll
-    for (int c = 0; c < kChunksPerWave; ++c) {
-        int64_t idx = base + (int64_t)(wave * kChunksPerWave + c) * 64 + lane;
-        bool valid = idx < n;
-        uint32_t digit = valid ? ((keys[idx] >> shift) & mask) : 0u;
-        uint64_t active = __ballot(valid);
-        if (active == 0) break;
-        uint64_t m = digit_match<BITS>(digit, active);
-        int leader = __ffsll((unsigned long long)m) - 1;
-        if (valid && lane == leader) atomicAdd(&h[digit], (uint32_t)__popcll(m));
+        for (int c = 0; c < kChunksPerWave; ++c) {
+            int64_t idx = base + (int64_t)(wave * kChunksPerWave + c) * 64 + lane;
+            bool valid = idx < n;
+            uint32_t digit = valid ? ((keys[idx] >> shift) & mask) : 0u;
+            uint64_t active = __ballot(valid);
+            if (active == 0) break;
+            uint64_t m = digit_match<BITS>(digit, active);
+            int leader = __ffsll((unsigned long long)m) - 1;
+            if (valid && lane == leader) atomicAdd(&h[digit], (uint32_t)__popcll(m));
+        }
+        __syncthreads();
+        for (int d = tid; d < kRadix; d += kSortWG) hist[(int64_t)d * tiles + tile] = h[d];
+        __syncthreads();
     }
-    __syncthreads();
-    for (int d = tid; d < kRadix; d += kSortWG) hist[(int64_t)d * tiles + tile] = h[d];
 }
 
 template <int BITS, bool RAW>
@@ -74,7 +81,7 @@ __global__ void __launch_bounds__(kSortWG) k_radix_scatter(const uint32_t* __res
                                                            uint32_t* __restrict__ vals_out, int64_t n, int shift,
                                                            uint32_t mask, const uint32_t* __restrict__ hist_scan,
                                                            int64_t tiles, const uint32_t* __restrict__ plan) {
-    if (plan && plan[kPlanMode] != kPlanFull) return;
+    if (plan && plan_incremental(plan)) return;
     constexpr int kRadix = 1 << BITS;
     // digits per thread: 2 (9 bits), 1 (8 bits), or 1 for the first kRadix
     // threads and none for the rest (narrower digits)
@@ -91,7 +98,7 @@ __global__ void __launch_bounds__(kSortWG) k_radix_scatter(const uint32_t* __res
     const int tid = threadIdx.x;
     const int lane = tid & 63;
     const int wave = tid >> 6;
-    const int64_t tile = blockIdx.x;
+    for (int64_t tile = blockIdx.x; tile < tiles; tile += gridDim.x) {
     const int64_t base = tile * kSortTile;
     const int tile_count = (int)((n - base) < kSortTile ? (n - base) : kSortTile);
 
@@ -209,6 +216,8 @@ __global__ void __launch_bounds__(kSortWG) k_radix_scatter(const uint32_t* __res
         keys_out[out] = k;
         vals_out[out] = stage_v[i];
     }
+    __syncthreads();  // the next tile reuses every LDS array
+    }
 }
 
 // ------------------------------------------------------------------ scan
@@ -217,18 +226,21 @@ __global__ void __launch_bounds__(kSortWG) k_radix_scatter(const uint32_t* __res
 __global__ void __launch_bounds__(kScanWG) k_scan_reduce(const uint32_t* __restrict__ in, int64_t n,
                                                          uint32_t* __restrict__ partial,
                                                          const uint32_t* __restrict__ plan) {
-    if (plan && plan[kPlanMode] != kPlanFull) return;
+    if (plan && plan_incremental(plan)) return;
     __shared__ uint32_t ws[kScanWG / 64];
-    int64_t base = (int64_t)blockIdx.x * kScanTile;
-    uint32_t s = 0;
+    const int64_t tiles = (n + kScanTile - 1) / kScanTile;
+    for (int64_t tile = blockIdx.x; tile < tiles; tile += gridDim.x) {
+        int64_t base = tile * kScanTile;
+        uint32_t s = 0;
 #pragma unroll
-    for (int i = 0; i < kScanIPT; ++i) {
-        int64_t idx = base + (int64_t)threadIdx.x * kScanIPT + i;
-        s += idx < n ? in[idx] : 0u;
+        for (int i = 0; i < kScanIPT; ++i) {
+            int64_t idx = base + (int64_t)threadIdx.x * kScanIPT + i;
+            s += idx < n ? in[idx] : 0u;
+        }
+        uint32_t tot;
+        block_exclusive_scan(s, ws, tot);  // ends with a barrier: ws is free again
+        if (threadIdx.x == 0) partial[tile] = tot;
     }
-    uint32_t tot;
-    block_exclusive_scan(s, ws, tot);
-    if (threadIdx.x == 0) partial[blockIdx.x] = tot;
 }
 
 // Exclusive scan of each tile, offset by the tile prefix: partial_scan[b]
@@ -239,37 +251,40 @@ __global__ void __launch_bounds__(kScanWG) k_scan_reduce(const uint32_t* __restr
 template <bool RAW>
 __global__ void __launch_bounds__(kScanWG) k_scan_down(const uint32_t* in, int64_t n, const uint32_t* partial_scan,
                                                        uint32_t* out, uint32_t* total, const uint32_t* plan) {
-    if (plan && plan[kPlanMode] != kPlanFull) return;
+    if (plan && plan_incremental(plan)) return;
     __shared__ uint32_t ws[kScanWG / 64];
-    uint32_t tile_prefix = 0;
-    if (partial_scan) {
-        if (RAW) {
-            uint32_t part = 0;
-            for (uint32_t t = threadIdx.x; t < blockIdx.x; t += kScanWG) part += partial_scan[t];
-            block_exclusive_scan(part, ws, tile_prefix);
-        } else {
-            tile_prefix = partial_scan[blockIdx.x];
+    const int64_t tiles = (n + kScanTile - 1) / kScanTile;
+    for (int64_t tile = blockIdx.x; tile < tiles; tile += gridDim.x) {
+        uint32_t tile_prefix = 0;
+        if (partial_scan) {
+            if (RAW) {
+                uint32_t part = 0;
+                for (int64_t t = threadIdx.x; t < tile; t += kScanWG) part += partial_scan[t];
+                block_exclusive_scan(part, ws, tile_prefix);
+            } else {
+                tile_prefix = partial_scan[tile];
+            }
         }
-    }
-    int64_t base = (int64_t)blockIdx.x * kScanTile;
-    uint32_t v[kScanIPT];
-    uint32_t s = 0;
+        int64_t base = tile * kScanTile;
+        uint32_t v[kScanIPT];
+        uint32_t s = 0;
 #pragma unroll
-    for (int i = 0; i < kScanIPT; ++i) {
-        int64_t idx = base + (int64_t)threadIdx.x * kScanIPT + i;
-        v[i] = idx < n ? in[idx] : 0u;
-        s += v[i];
-    }
-    uint32_t tot;
-    uint32_t pre = block_exclusive_scan(s, ws, tot);
-    uint32_t off = tile_prefix + pre;
+        for (int i = 0; i < kScanIPT; ++i) {
+            int64_t idx = base + (int64_t)threadIdx.x * kScanIPT + i;
+            v[i] = idx < n ? in[idx] : 0u;
+            s += v[i];
+        }
+        uint32_t tot;
+        uint32_t pre = block_exclusive_scan(s, ws, tot);
+        uint32_t off = tile_prefix + pre;
 #pragma unroll
-    for (int i = 0; i < kScanIPT; ++i) {
-        int64_t idx = base + (int64_t)threadIdx.x * kScanIPT + i;
-        if (idx < n) out[idx] = off;
-        off += v[i];
+        for (int i = 0; i < kScanIPT; ++i) {
+            int64_t idx = base + (int64_t)threadIdx.x * kScanIPT + i;
+            if (idx < n) out[idx] = off;
+            off += v[i];
+        }
+        if (total && tile == tiles - 1 && threadIdx.x == kScanWG - 1) *total = off;
     }
-    if (total && blockIdx.x == gridDim.x - 1 && threadIdx.x == kScanWG - 1) *total = off;
 }
 
 int64_t scan_temp_words(int64_t n) {
@@ -282,10 +297,16 @@ int64_t scan_temp_words(int64_t n) {
     return words + 64;
 }
 
+// Workgroups of a launch over `tiles` tiles: one per tile, or at most
+// max_wgs (> 0), each looping over its tiles.
+static unsigned tile_grid(int64_t tiles, int max_wgs) {
+    return (unsigned)(max_wgs > 0 && tiles > max_wgs ? max_wgs : tiles);
+}
+
 // Exclusive scan in place over n words; temp holds scan_temp_words(n).
 // total (device, may be null) receives the sum.
 void exclusive_scan(uint32_t* data, int64_t n, uint32_t* temp, uint32_t* total, hipStream_t st,
-                    const uint32_t* plan) {
+                    const uint32_t* plan, int max_wgs) {
     if (n <= 0) return;
     if (n <= kScanTile) {
         hipLaunchKernelGGL(k_scan_down<false>, dim3(1), dim3(kScanWG), 0, st, data, n, (const uint32_t*)nullptr, data,
@@ -294,15 +315,16 @@ void exclusive_scan(uint32_t* data, int64_t n, uint32_t* temp, uint32_t* total, 
     }
     int64_t tiles = (n + kScanTile - 1) / kScanTile;
     uint32_t* partial = temp;
-    hipLaunchKernelGGL(k_scan_reduce, dim3((unsigned)tiles), dim3(kScanWG), 0, st, data, n, partial, plan);
+    const unsigned grid = tile_grid(tiles, max_wgs);
+    hipLaunchKernelGGL(k_scan_reduce, dim3(grid), dim3(kScanWG), 0, st, data, n, partial, plan);
     if (tiles <= 2048) {  // each tile sums at most 2047 tile totals itself
-        hipLaunchKernelGGL(k_scan_down<true>, dim3((unsigned)tiles), dim3(kScanWG), 0, st, data, n,
-                           (const uint32_t*)partial, data, total, plan);
+        hipLaunchKernelGGL(k_scan_down<true>, dim3(grid), dim3(kScanWG), 0, st, data, n, (const uint32_t*)partial,
+                           data, total, plan);
         return;
     }
-    exclusive_scan(partial, tiles, temp + tiles, nullptr, st, plan);
-    hipLaunchKernelGGL(k_scan_down<false>, dim3((unsigned)tiles), dim3(kScanWG), 0, st, data, n,
-                       (const uint32_t*)partial, data, total, plan);
+    exclusive_scan(partial, tiles, temp + tiles, nullptr, st, plan, max_wgs);
+    hipLaunchKernelGGL(k_scan_down<false>, dim3(grid), dim3(kScanWG), 0, st, data, n, (const uint32_t*)partial, data,
+                       total, plan);
 }
 
 int64_t radix_tiles(int64_t n) { return (n + kSortTile - 1) / kSortTile; }
@@ -315,14 +337,17 @@ int64_t radix_hist_words(int64_t n) { return radix_tiles(n) * kMaxRadix; }
 // and on the 8.4M-particle slab (25-bit keys) 4 passes of 7/6/6/6 bits take
 // 0.31 ms against 0.37 ms for 3 passes of 9/8/8 and 0.35 ms for 5 passes of
 // 5-6 bits.  Small sorts are launch-bound and use as few passes as 8- or
-// 9-bit digits allow.  Bits are split as evenly as possible, widest first.
-int radix_passes(int key_bits, int64_t n) {
-    if (n >= (int64_t(1) << 20)) return (key_bits + 6) / 7;
+// 9-bit digits allow, and so do large ones with few_passes (the caller
+// expects the sort's launches to return at once most of the time: fewer
+// launches, a slower sort when it does run).  Bits are split as evenly as
+// possible, widest first.
+int radix_passes(int key_bits, int64_t n, bool few_passes) {
+    if (n >= (int64_t(1) << 20) && !few_passes) return (key_bits + 6) / 7;
     const int p8 = (key_bits + 7) / 8, p9 = (key_bits + 8) / 9;
     return p9 < p8 ? p9 : p8;
 }
-static int digit_bits(int key_bits, int64_t n, int pass) {
-    const int passes = radix_passes(key_bits, n);
+static int digit_bits(int key_bits, int64_t n, int pass, bool few_passes) {
+    const int passes = radix_passes(key_bits, n, few_passes);
     int done = 0;
     for (int p = 0; p < pass; ++p) done += (key_bits - done + (passes - p) - 1) / (passes - p);
     return (key_bits - done + (passes - pass) - 1) / (passes - pass);
@@ -332,22 +357,25 @@ static int digit_bits(int key_bits, int64_t n, int pass) {
 // returns 0 if the result is in (ka,va), 1 if in (kb,vb).
 int radix_sort_pairs(const uint32_t* kin, const uint32_t* vin, uint32_t* ka, uint32_t* va, uint32_t* kb, uint32_t* vb,
                      int64_t n, int key_bits, uint32_t* hist, uint32_t* scan_temp, hipStream_t st,
-                     const ProfileHook& prof, const uint32_t* plan) {
+                     const ProfileHook& prof, const uint32_t* plan, bool few_passes) {
     if (n <= 0) return 0;
     int64_t tiles = radix_tiles(n);
-    int passes = radix_passes(key_bits, n);
+    int passes = radix_passes(key_bits, n, few_passes);
+    // with few_passes the launches mostly return at once: fewer workgroups
+    const int max_wgs = few_passes ? kLightGrid : 0;
+    const unsigned grid = tile_grid(tiles, max_wgs);
     const uint32_t *ki = kin, *vi = vin;
     uint32_t *ko = ka, *vo = va;
     int shift = 0;
     for (int p = 0; p < passes; ++p) {
-        const int bits = digit_bits(key_bits, n, p);
+        const int bits = digit_bits(key_bits, n, p, few_passes);
         const uint32_t mask = (1u << bits) - 1u;
         // digit widths of the plan: 9 or 8 bits, or 6-7 under a narrower plan
         const int dw = bits > 8 ? 9 : (bits > 7 ? 8 : (bits > 6 ? 7 : 6));
         const int radix = 1 << dw;
         prof.begin("sort_hist");
-#define MPM_HIST(B) hipLaunchKernelGGL(k_radix_hist<B>, dim3((unsigned)tiles), dim3(kSortWG), 0, st, ki, n, shift, \
-                                       mask, hist, tiles, plan)
+#define MPM_HIST(B) hipLaunchKernelGGL(k_radix_hist<B>, dim3(grid), dim3(kSortWG), 0, st, ki, n, shift, mask, hist, \
+                                       tiles, plan)
         if (dw == 9) MPM_HIST(9);
         else if (dw == 8) MPM_HIST(8);
         else if (dw == 7) MPM_HIST(7);
@@ -358,11 +386,11 @@ int radix_sort_pairs(const uint32_t* kin, const uint32_t* vin, uint32_t* ka, uin
         // (one launch fewer per pass for small sorts)
         const bool raw = tiles <= kRawTiles;
         prof.begin("sort_scan");
-        if (!raw) exclusive_scan(hist, tiles * radix, scan_temp, nullptr, st, plan);
+        if (!raw) exclusive_scan(hist, tiles * radix, scan_temp, nullptr, st, plan, max_wgs);
         prof.end("sort_scan");
         prof.begin("sort_scatter");
-#define MPM_SCATTER(B, R) hipLaunchKernelGGL((k_radix_scatter<B, R>), dim3((unsigned)tiles), dim3(kSortWG), 0, st, ki, \
-                                             vi, ko, vo, n, shift, mask, hist, tiles, plan)
+#define MPM_SCATTER(B, R) hipLaunchKernelGGL((k_radix_scatter<B, R>), dim3(grid), dim3(kSortWG), 0, st, ki, vi, ko, \
+                                             vo, n, shift, mask, hist, tiles, plan)
         if (raw) {
             if (dw == 9) MPM_SCATTER(9, true);
             else if (dw == 8) MPM_SCATTER(8, true);

@@ -50,6 +50,11 @@ def test_incremental_sort_bit_exact_slab():
         st = _step_and_check(gpu, sc.desc, "slab eager")
     # first step and the steps after the reorders (6, 12) sort in full
     assert st["incremental_sorts"] >= 9
+    # a run of updates: the full sort's launches switch to fewer passes on
+    # fewer workgroups, which the full sort after the state write uses
+    gpu.set_reorder_interval(0)
+    for k in range(24):
+        st = _step_and_check(gpu, sc.desc, "slab updates")
     # a state write: keys recomputed, the next sort in full, then updates again
     s0 = st["incremental_sorts"]
     state = gpu.read_particles()
