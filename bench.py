@@ -406,6 +406,8 @@ def main(argv=None):
     ap.add_argument("--warmup", type=int, default=10)
     ap.add_argument("--workload", default="apic_slab", choices=["apic_slab", "two_balls", "snow_cube"])
     ap.add_argument("--graph", action="store_true", help="replay the step as a captured hipGraph")
+    ap.add_argument("--no-fusion", action="store_true",
+                    help="APIC quadratic: G2P and the next P2G as two kernels instead of the fused one")
     ap.add_argument("--no-cpu-baseline", action="store_true")
     ap.add_argument("--no-small", action="store_true", help="skip the 16k-particle snow cube timing")
     ap.add_argument("--seed", type=int, default=None)
@@ -448,7 +450,8 @@ def main(argv=None):
 
         outdir = tempfile.mkdtemp(prefix="mpm_bench_pmc_")
         try:
-            traffic = pmc_traffic(a.workload, [] if a.seed is None else ["--seed", str(a.seed)], outdir)
+            extra = ([] if a.seed is None else ["--seed", str(a.seed)]) + (["--no-fusion"] if a.no_fusion else [])
+            traffic = pmc_traffic(a.workload, extra, outdir)
         finally:
             shutil.rmtree(outdir, ignore_errors=True)
         log(f"pmc traffic passes in {time.time() - t0:.1f}s: "
@@ -463,6 +466,8 @@ def main(argv=None):
     log(f"workload {a.workload}: {n} particles, grid {sc.desc.resolution}, generated in {time.time() - t0:.1f}s")
     t0 = time.time()
     solver = Solver(sc.desc, sc.particles)
+    if a.no_fusion:
+        solver.set_transfer_fusion(False)
     solver.synchronize()
     log(f"created + initialized in {time.time() - t0:.1f}s")
 
@@ -568,7 +573,7 @@ def main(argv=None):
         "config": {"workload": sc.name, "particles": n, "grid": list(sc.desc.resolution),
                    "interp": "quadratic" if sc.desc.interp == 1 else ("cubic" if sc.desc.interp == 2 else "linear"),
                    "transfer": "APIC" if apic else "PIC-FLIP", "dt": sc.desc.dt, "seed": sc.seed,
-                   "graph": bool(a.graph), "parallelism": "single GPU"},
+                   "graph": bool(a.graph), "transfer_fusion": not a.no_fusion, "parallelism": "single GPU"},
         "hbm_roofline_frac_step": step_frac,
         "algorithmic_bytes_per_step": step_bytes,
         "active_nodes_mean": act,

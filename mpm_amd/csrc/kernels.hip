@@ -1424,16 +1424,19 @@ __device__ __forceinline__ void scatter_direct(const SceneDev& S, float* grid, u
 // 10^4) keeps its lane in the cell's segment with zero contribution and
 // scatters itself with global atomics.  The stress of the new state is used
 // at once, so it is not cached.
-template <int INTERP>
 #ifndef MPM_G2P2G_WAVES
 #define MPM_G2P2G_WAVES 3
 #endif
-// 3 waves per SIMD: the kernel needs ~168 VGPRs (4 waves would spill ~34)
-// and its 24 KB of LDS per workgroup allow 6 workgroups (12 waves) per CU.
-__global__ void __launch_bounds__(128) __attribute__((amdgpu_waves_per_eu(MPM_G2P2G_WAVES)))
+#ifndef MPM_G2P2G_WG
+#define MPM_G2P2G_WG 128
+#endif
+// 3 waves per SIMD: the kernel needs ~162 VGPRs (at 128, 4 waves, ~24
+// spill); its LDS (19.6 KB per 128-thread workgroup) would allow 4.
+template <int INTERP, int WG>
+__global__ void __launch_bounds__(WG) __attribute__((amdgpu_waves_per_eu(MPM_G2P2G_WAVES)))
 k_g2p2g(SceneDev S, ParticlesDev P, ItemsDev I, float* __restrict__ ogrid, uint32_t* __restrict__ otouched,
         int pcollide, uint32_t* __restrict__ next_keys) {
-    constexpr int WG = 128;
+    constexpr int NW = WG / 64;  // waves: each has its own output tile
     constexpr int W = Stencil<INTERP>::W;
     constexpr int T = 4 + W - 1;
     constexpr int TN = T * T * T;
@@ -1442,7 +1445,7 @@ k_g2p2g(SceneDev S, ParticlesDev P, ItemsDev I, float* __restrict__ ogrid, uint3
     __shared__ float tile[3 * TN];       // v* of this step
     __shared__ float stage[4 * TN];      // the next item's folded tile in flight
     __shared__ float pstage[PW * WG];    // Fe, Jp, mu0, lam0, V0 of each lane's particle
-    __shared__ float otile[2 * 4 * TN];  // next step's sums, one tile per wave
+    __shared__ float otile[NW * 4 * TN];  // next step's sums, one tile per wave
     const int wave = threadIdx.x >> 6, lane = threadIdx.x & 63;
     const int64_t N = P.n;
     const uint32_t nitems = *I.count;
@@ -1455,7 +1458,7 @@ k_g2p2g(SceneDev S, ParticlesDev P, ItemsDev I, float* __restrict__ ogrid, uint3
     for (uint32_t it = blockIdx.x; it < nitems; it += gridDim.x) {
         const ItemGeom g = gnext;
         if (!g.escape) commit_folded_tile<T, TR, WG>(S, g, stage, tile);
-        for (int q = threadIdx.x; q < 2 * 4 * TN; q += WG) otile[q] = 0.0f;
+        for (int q = threadIdx.x; q < NW * 4 * TN; q += WG) otile[q] = 0.0f;
         __syncthreads();
         const uint32_t itn = it + gridDim.x;
         if (itn < nitems) {
@@ -1615,7 +1618,9 @@ k_g2p2g(SceneDev S, ParticlesDev P, ItemsDev I, float* __restrict__ ogrid, uint3
                 bool any = false;
 #pragma unroll
                 for (int f = 0; f < 4; ++f) {
-                    const float val = otile[f * TN + q] + otile[4 * TN + f * TN + q];
+                    float val = otile[f * TN + q];
+#pragma unroll
+                    for (int w = 1; w < NW; ++w) val += otile[(4 * w + f) * TN + q];
                     if (val != 0.0f) {
                         unsafeAtomicAdd(&ogrid[(int64_t)f * S.plane + ni], val);
                         any = true;
@@ -1631,7 +1636,8 @@ k_g2p2g(SceneDev S, ParticlesDev P, ItemsDev I, float* __restrict__ ogrid, uint3
 void launch_g2p2g(const SceneDev& S, const ParticlesDev& P, const ItemsDev& I, int pcollide, float* ogrid,
                   uint32_t* otouched, uint32_t* next_keys, hipStream_t st) {
     if (I.n <= 0) return;
-    MPM_PERSISTENT(k_g2p2g<MPM_INTERP_QUADRATIC_BSPLINE>, 128, 0, st, S, P, I, ogrid, otouched, pcollide, next_keys);
+    MPM_PERSISTENT((k_g2p2g<MPM_INTERP_QUADRATIC_BSPLINE, MPM_G2P2G_WG>), MPM_G2P2G_WG, 0, st, S, P, I, ogrid,
+                   otouched, pcollide, next_keys);
 }
 
 template <int INTERP, int APIC, int SRC, bool BATCH>
