@@ -82,11 +82,23 @@ struct Batch {
     bool tune_fixed = false;  // set explicitly: the tuning table is not consulted
     bool fold = false;        // APIC with no member recording its grid: 4-plane P2G
     // every reorder_interval steps (0 = never) each member's store is
-    // permuted into its range of the global cell order; scratch 9 * cap words
+    // permuted into its range of the global cell order, a chunk of members
+    // per step (chunks: about ntot / reorder_interval particles each, bounds
+    // in global positions, the current chunk index on the device); the
+    // autotuner's full reorders use reorder_scratch (9 * cap words)
     int reorder_interval = 16;
     int64_t last_reorder = 0;
     uint32_t* reorder_scratch = nullptr;
     int64_t reorder_cap = 0;
+    int nchunks = 0;
+    int64_t chunk_max = 0;
+    int64_t* chunk_bounds = nullptr;  // nchunks + 1
+    int chunk_bounds_cap = 0;
+    uint32_t* chunk_rr = nullptr;
+    uint32_t* slot_scene = nullptr;  // cap words: each global slot's member
+    int64_t slot_scene_cap = 0;
+    uint32_t* chunk_scratch = nullptr;
+    int64_t chunk_scratch_cap = 0;
     // per-member health words (3 per member), allocated on first use
     uint32_t* health = nullptr;
     int health_cap = 0;
@@ -168,6 +180,56 @@ SceneEntry make_entry(const Solver& s, int64_t pstart, uint32_t koff, uint32_t i
     return e;
 }
 
+// The reorder chunks of the current layout: consecutive members, closed
+// once they hold ntot / reorder_interval particles (a larger member is a
+// chunk of its own); their position bounds, each slot's member, the scratch
+// of the largest chunk, and the chunk index reset to 0.  No chunks when the
+// reorder is off.
+std::vector<int64_t> g_chunk_host;  // host staging of the bounds (the upload is waited for)
+int layout_chunks(Batch& b) {
+    b.nchunks = 0;
+    b.chunk_max = 0;
+    const int ns = (int)b.members.size();
+    if (b.reorder_interval <= 0 || ns == 0 || b.ntot == 0) return MPM_OK;
+    const int64_t target = (b.ntot + b.reorder_interval - 1) / b.reorder_interval;
+    std::vector<int64_t>& bounds = g_chunk_host;
+    bounds.assign(1, 0);
+    int64_t acc = 0;
+    for (int i = 0; i < ns; ++i) {
+        acc += b.members[(size_t)i]->N;
+        if (acc - bounds.back() >= target || i == ns - 1) bounds.push_back(acc);
+    }
+    const int k = (int)bounds.size() - 1;
+    int64_t mx = 0;
+    for (int c = 0; c < k; ++c) mx = std::max(mx, bounds[(size_t)c + 1] - bounds[(size_t)c]);
+    int rc;
+    if (k + 1 > b.chunk_bounds_cap) {
+        (void)hipStreamSynchronize(b.st);
+        bfree(b, *(void**)&b.chunk_bounds);
+        if ((rc = balloc(b, (void**)&b.chunk_bounds, (size_t)(k + 1) * 8))) return rc;
+        b.chunk_bounds_cap = k + 1;
+    }
+    if (!b.chunk_rr && (rc = balloc(b, (void**)&b.chunk_rr, 64))) return rc;
+    if (b.slot_scene_cap < b.cap) {
+        (void)hipStreamSynchronize(b.st);
+        bfree(b, *(void**)&b.slot_scene);
+        if ((rc = balloc(b, (void**)&b.slot_scene, (size_t)b.cap * 4))) return rc;
+        b.slot_scene_cap = b.cap;
+    }
+    if (b.chunk_scratch_cap < mx) {
+        (void)hipStreamSynchronize(b.st);
+        bfree(b, *(void**)&b.chunk_scratch);
+        if ((rc = balloc(b, (void**)&b.chunk_scratch, (size_t)mx * kChunkScratchWords * 4))) return rc;
+        b.chunk_scratch_cap = mx;
+    }
+    HIP_TRY(hipMemcpyAsync(b.chunk_bounds, bounds.data(), (size_t)(k + 1) * 8, hipMemcpyHostToDevice, b.st));
+    HIP_TRY(hipMemsetAsync(b.chunk_rr, 0, 4, b.st));
+    launch_slot_scenes(b.tab, ns, b.ntot, b.slot_scene, b.st);
+    b.nchunks = k;
+    b.chunk_max = mx;
+    return MPM_OK;
+}
+
 // Recompute the global layout of the members, (re)upload the tables and
 // the sort input (keys with offsets, identity values).
 int relayout(Batch& b) {
@@ -230,6 +292,7 @@ int relayout(Batch& b) {
         auto it = g_tune.find({b.interp, b.apic, size_bucket(n)});
         b.tune = it != g_tune.end() ? it->second.t : Tune{};
     }
+    if ((rc = layout_chunks(b))) return rc;
     HIP_TRY(hipStreamSynchronize(b.st));  // host tables must outlive the copies
     b.layout_valid = true;
     return MPM_OK;
@@ -271,19 +334,16 @@ int sync_members(Batch& b) {
 // reorder: permute every member's store by this step's own sort (eager
 // steps; reorder_scratch allocated), after which the step runs on the
 // identity permutation.
-void enqueue_step(Batch& b, bool reorder = false) {
+void enqueue_step(Batch& b) {
     const int ns = (int)b.members.size();
     launch_clear_batch(b.tab, b.block_start, ns, b.total_blocks, b.fold ? GF_FOLDED : GF_PRIMARY, b.st);
     const int sel = radix_sort_pairs(b.kin, b.iota, b.ka, b.va, b.kb, b.vb, b.ntot, b.key_bits, b.hist, b.scan_temp,
                                      b.st, ProfileHook{});
     b.I.sorted_keys = sel ? b.kb : b.ka;
     b.I.perm = sel ? b.vb : b.va;
-    if (reorder) {
-        // flags (ntot words) holds each slot's scene until launch_items reuses it
-        launch_reorder_store(ParticlesDev{}, b.tab, ns, b.I.perm, b.ntot, b.reorder_scratch, b.apic != 0, b.st,
-                             b.flags);
-        b.I.perm = b.iota;  // slot i now holds the i-th particle in cell order
-    }
+    if (b.reorder_interval > 0 && b.nchunks > 0)  // one chunk of members into this step's order
+        launch_reorder_chunk(b.tab, b.slot_scene, sel ? b.vb : b.va, b.chunk_bounds, b.chunk_rr, b.nchunks,
+                             b.chunk_max, b.chunk_scratch, b.apic != 0, b.st);
     launch_items(b.I.sorted_keys, b.ntot, b.flags, b.scan_temp, b.items, b.counters, b.st, (uint32_t)b.tune.item_max,
                  b.key_offsets, ns, b.item_scene);
     const SceneDev S0{};
@@ -391,16 +451,8 @@ int run_steps(Batch& b, int n, bool graph) {
             e1 = take_event(b);
             if (e0 && e1) HIP_TRY(hipEventRecord(e0, b.st));
         }
-        const bool due = b.reorder_interval > 0 && b.steps + i - b.last_reorder >= b.reorder_interval;
-        if (due) {
-            // a replayed graph sorts by itself, so the reorder runs its own
-            // sort between replays; an eager step reorders on its own sort
-            int rc2 = graph ? reorder_members(b) : ensure_reorder_scratch(b);
-            if (rc2) return rc2;
-            b.last_reorder = b.steps + i;
-        }
         if (graph) HIP_TRY(hipGraphLaunch(b.gexec, b.st));
-        else enqueue_step(b, due);
+        else enqueue_step(b);
         if (e0 && e1) {
             HIP_TRY(hipEventRecord(e1, b.st));
             b.pending.push_back({e0, e1});
@@ -581,7 +633,11 @@ MPM_API int mpm_batch_step(void* batch, int n) {
 MPM_API int mpm_batch_set_reorder_interval(void* batch, int steps) {
     GET_BATCH(b, batch);
     if (steps < 0) return fail(MPM_ERR_INVALID, "steps < 0");
-    b->reorder_interval = steps;
+    if (b->reorder_interval != steps) {
+        b->reorder_interval = steps;
+        b->layout_valid = false;  // new chunks; the captured step changes
+        drop_graph(*b);
+    }
     return MPM_OK;
 }
 

@@ -2133,6 +2133,84 @@ void launch_reorder_store(const ParticlesDev& P, const SceneEntry* tab, int nsce
     }
 }
 
+// ---------------------------------------------------- batch reorder by chunk
+// A batch spreads its store reorder over steps: its members are grouped
+// into chunks of consecutive members (about ntot / chunks particles each,
+// positions [bounds[k], bounds[k + 1]) of the global order), and each step
+// permutes the members of chunk *rr into the step's sorted order, then
+// advances *rr.  The index lives on the device, so a captured step does the
+// same on every replay.  A chunk's positions are exactly its members' slots
+// (a scene's keys are one contiguous range), so the gather pass copies every
+// word of slot perm[pos] into a scratch row (kStoreWords rows of stride
+// words, and the 16-bit material index beside them) and the scatter pass
+// writes row i back to slot pos and sets perm[pos] = pos: the rest of the
+// step then reads the reordered store through the identity.
+constexpr int kStoreWords = 45;  // x 3, v 3, B 9, Fe 9, Fp 9, Jp, tau 6, m, V0, mu0, lam0, pid
+template <bool SCATTER>
+__global__ void __launch_bounds__(256) k_reorder_chunk(const SceneEntry* __restrict__ tab,
+                                                       const uint32_t* __restrict__ slot_sc, uint32_t* perm,
+                                                       const int64_t* __restrict__ bounds, const uint32_t* rr,
+                                                       int64_t stride, uint32_t* __restrict__ scratch, int apic) {
+    const uint32_t k = *rr;
+    const int64_t lo = bounds[k], hi = bounds[k + 1];
+    uint16_t* const s16 = reinterpret_cast<uint16_t*>(scratch + (int64_t)kStoreWords * stride);
+    for (int64_t i = (int64_t)blockIdx.x * blockDim.x + threadIdx.x; lo + i < hi;
+         i += (int64_t)gridDim.x * blockDim.x) {
+        const int64_t pos = lo + i;
+        const uint32_t sc = slot_sc[pos];
+        const int64_t ps = tab[sc].pstart;
+        const ParticlesDev& P = tab[sc].P;
+        const int64_t n = P.n;
+        const int64_t l = SCATTER ? pos - ps : (int64_t)perm[pos] - ps;
+        int w = 0;
+        auto move = [&](float MPM_GLOBAL* f, int comps) {
+            for (int c = 0; c < comps; ++c, ++w) {
+                uint32_t MPM_GLOBAL* const a = reinterpret_cast<uint32_t MPM_GLOBAL*>(f) + (int64_t)c * n + l;
+                if (SCATTER) *a = scratch[(int64_t)w * stride + i];
+                else scratch[(int64_t)w * stride + i] = *a;
+            }
+        };
+        move(P.x, 3);
+        move(P.v, 3);
+        if (apic) move(P.B, 9);
+        else w += 9;
+        move(P.Fe, 9);
+        move(P.Fp, 9);
+        move(P.Jp, 1);
+        move(P.tau, 6);
+        move(P.m, 1);
+        move(P.V0, 1);
+        move(P.mu0, 1);
+        move(P.lam0, 1);
+        move(reinterpret_cast<float MPM_GLOBAL*>(P.pid), 1);
+        if (SCATTER) {
+            P.mat[l] = s16[i];
+            perm[pos] = (uint32_t)pos;
+        } else {
+            s16[i] = P.mat[l];
+        }
+    }
+}
+__global__ void k_chunk_advance(uint32_t* rr, uint32_t nchunks) {
+    if (threadIdx.x == 0) *rr = *rr + 1u < nchunks ? *rr + 1u : 0u;
+}
+void launch_reorder_chunk(const SceneEntry* tab, const uint32_t* slot_sc, uint32_t* perm, const int64_t* bounds,
+                          uint32_t* rr, int nchunks, int64_t chunk_max, uint32_t* scratch, bool apic, hipStream_t st) {
+    if (nchunks <= 0 || chunk_max <= 0) return;
+    const int64_t want = (chunk_max + 255) / 256;
+    const unsigned blocks = (unsigned)(want < 4096 ? want : 4096);
+    hipLaunchKernelGGL(k_reorder_chunk<false>, dim3(blocks), dim3(256), 0, st, tab, slot_sc, perm, bounds, rr,
+                       chunk_max, scratch, apic ? 1 : 0);
+    hipLaunchKernelGGL(k_reorder_chunk<true>, dim3(blocks), dim3(256), 0, st, tab, slot_sc, perm, bounds, rr,
+                       chunk_max, scratch, apic ? 1 : 0);
+    hipLaunchKernelGGL(k_chunk_advance, dim3(1), dim3(64), 0, st, rr, (uint32_t)nchunks);
+}
+void launch_slot_scenes(const SceneEntry* tab, int nscenes, int64_t ntot, uint32_t* slot_sc, hipStream_t st) {
+    if (ntot <= 0) return;
+    hipLaunchKernelGGL(k_slot_scenes, dim3((unsigned)((ntot + 255) / 256)), dim3(256), 0, st, tab, nscenes, ntot,
+                       slot_sc);
+}
+
 __global__ void k_gather_u32(const uint32_t* __restrict__ src, const uint32_t* __restrict__ idx, uint32_t* dst,
                              int64_t n) {
     int64_t i = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;

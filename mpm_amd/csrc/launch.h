@@ -113,6 +113,15 @@ void launch_gather_u32(const uint32_t* src, const uint32_t* idx, uint32_t* dst, 
 // one store, ntot = P.n and slot_sc is unused.
 void launch_reorder_store(const ParticlesDev& P, const SceneEntry* tab, int nscenes, const uint32_t* perm,
                           int64_t ntot, uint32_t* scratch, bool apic, hipStream_t st, uint32_t* slot_sc = nullptr);
+// A batch's reorder of one chunk of members per step: the members of chunk
+// *rr (global positions [bounds[*rr], bounds[*rr + 1]), at most chunk_max)
+// permuted into the step's sorted order (perm, set to the identity there),
+// then *rr advanced modulo nchunks.  scratch: kStoreWords * chunk_max words
+// plus chunk_max 16-bit words; slot_sc: each global slot's scene.
+constexpr int kChunkScratchWords = 46;  // per chunk position (45 words + the 16-bit index, rounded up)
+void launch_reorder_chunk(const SceneEntry* tab, const uint32_t* slot_sc, uint32_t* perm, const int64_t* bounds,
+                          uint32_t* rr, int nchunks, int64_t chunk_max, uint32_t* scratch, bool apic, hipStream_t st);
+void launch_slot_scenes(const SceneEntry* tab, int nscenes, int64_t ntot, uint32_t* slot_sc, hipStream_t st);
 void launch_grid_dense(const SceneDev& S, float* mass, float* vel, float* vstar, float* force, int have_vstar,
                        hipStream_t st);
 // Density raster readout from a one-plane mass grid D (D.grid = sum m w):
