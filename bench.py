@@ -313,16 +313,21 @@ def batch_bench(count, steps, warmup, seed=3):
         run = b.step if mode == "eager" else b.step_graph
         run(warmup)
         b.synchronize()
-        b.timing(True)
+        # throughput: back-to-back steps with no events in the stream
         q0 = time.perf_counter()
         run(steps)
         b.synchronize()
         el = time.perf_counter() - q0
+        # per-step latency: a second pass with a hipEvent pair around each step
+        b.timing(True)
+        run(steps)
+        b.synchronize()
         t = b.read_times()
         b.timing(False)
         out[mode] = {"value": ntot * steps / el, "ms_per_step": 1e3 * el / steps,
                      "step_ms_p50": float(np.percentile(t, 50)), "step_ms_p99": float(np.percentile(t, 99))}
-        log(f"batched ({mode}): {1e3 * el / steps:.2f} ms/step, {ntot * steps / el:.3e} p-u/s")
+        log(f"batched ({mode}): {1e3 * el / steps:.2f} ms/step, {ntot * steps / el:.3e} p-u/s, "
+            f"step p50/p99 {out[mode]['step_ms_p50']:.2f}/{out[mode]['step_ms_p99']:.2f} ms")
     info = b.info()
     b.close()
     for s in solvers:
@@ -377,9 +382,33 @@ def small_scene_bench(steps=200, cpu=True):
     return out
 
 
-def serve_bench(window_s, seed=4, rate=20.0):
+def serve_plain(reqs):
+    """The same requests one at a time, in arrival order, as a plain caller
+    runs them: create the scene, step it to the end, read the result back,
+    free it (no batching, no overlap).  Returns (wall s, particle-updates)."""
+    from mpm_amd.solver import Solver
+
+    t0 = time.perf_counter()
+    pu = 0
+    for i, r in enumerate(reqs):
+        s = Solver(r.desc, r.particles)
+        s.step(r.steps)
+        s.read_particles()
+        s.close()
+        pu += r.particles.count * r.steps
+        if (i + 1) % 100 == 0:
+            log(f"plain: {i + 1}/{len(reqs)} requests in {time.perf_counter() - t0:.1f}s")
+    return time.perf_counter() - t0, pu
+
+
+def serve_bench(window_s, seed=4, rate=20.0, plain=False, saturate=0):
     """Poisson request stream (seeded) over a window of window_s seconds,
-    served to completion under a 200 GB HBM budget."""
+    served to completion under a 200 GB HBM budget.  plain: the same stream
+    also run one request at a time (served / plain).  saturate: the first
+    `saturate` requests of the stream also served arriving all at once (the
+    server's capacity)."""
+    import dataclasses
+
     from mpm_amd import scenes
     from mpm_amd.serve import Server, requests_from_scenes
 
@@ -396,6 +425,24 @@ def serve_bench(window_s, seed=4, rate=20.0):
         f"request p50/p99 {rep.request_s_p50:.2f}/{rep.request_s_p99:.2f}s")
     d["workload"] = (f"request_stream: Poisson {rate}/s for {window_s}s, N log-uniform [5k,500k], "
                      f"steps U[200,2000], PIC-FLIP/APIC 50/50, 64^3-256^3, 200 GB budget")
+    if saturate > 0:
+        burst = [dataclasses.replace(r, arrival=0.0) for r in reqs[:saturate]]
+        srv = Server(budget_bytes=200e9, tick_steps=16)
+        try:
+            sat = srv.run(burst)
+        finally:
+            srv.close()
+        d["saturated"] = {"requests": sat.completed, "wall_s": sat.wall_s,
+                          "particle_updates_per_s": sat.particle_updates_per_s,
+                          "step_ms_p50": sat.step_ms_p50, "step_ms_p99": sat.step_ms_p99,
+                          "peak_scenes": sat.peak_scenes, "peak_bytes": sat.peak_bytes}
+        log(f"saturated: {sat.completed} requests arriving at once, {sat.wall_s:.1f}s, "
+            f"{sat.particle_updates_per_s:.3e} p-u/s, peak {sat.peak_scenes} scenes")
+    if plain:
+        wall, pu = serve_plain(reqs)
+        d["plain"] = {"wall_s": wall, "particle_updates": pu, "particle_updates_per_s": pu / wall}
+        d["served_vs_plain"] = rep.particle_updates_per_s / (pu / wall)
+        log(f"plain one-at-a-time: {wall:.1f}s, {pu / wall:.3e} p-u/s; served/plain {d['served_vs_plain']:.2f}x")
     return d
 
 
@@ -413,6 +460,11 @@ def main(argv=None):
     ap.add_argument("--seed", type=int, default=None)
     ap.add_argument("--batch-scenes", type=int, default=256, help="scenes of the batched run (0: skip)")
     ap.add_argument("--batch-steps", type=int, default=20)
+    ap.add_argument("--serve-rate", type=float, default=20.0, help="request arrivals per second")
+    ap.add_argument("--serve-plain", action="store_true",
+                    help="also run the served stream one request at a time (served / plain)")
+    ap.add_argument("--serve-saturate", type=int, default=0, metavar="N",
+                    help="also serve the stream's first N requests arriving at once (capacity)")
     ap.add_argument("--serve-seconds", type=float, default=10.0,
                     help="arrival window of the served request stream (0: skip)")
     ap.add_argument("--no-traffic", action="store_true",
@@ -548,7 +600,8 @@ def main(argv=None):
     density = density_bench(solver, sc, cpu=not a.no_cpu_baseline) if not a.graph else None
     solver.close()
     batched = batch_bench(a.batch_scenes, a.batch_steps, 3) if a.batch_scenes > 0 else None
-    served = serve_bench(a.serve_seconds) if a.serve_seconds > 0 else None
+    served = serve_bench(a.serve_seconds, rate=a.serve_rate, plain=a.serve_plain,
+                         saturate=a.serve_saturate) if a.serve_seconds > 0 else None
     small = small_scene_bench(cpu=not a.no_cpu_baseline) if a.workload == "apic_slab" and not a.no_small else None
 
     cpu = None

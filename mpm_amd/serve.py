@@ -154,7 +154,10 @@ class ServeReport:
     records: list = field(default_factory=list, repr=False)
 
     def summary(self) -> dict:
-        return {k: v for k, v in self.__dict__.items() if k != "records"}
+        d = {k: v for k, v in self.__dict__.items() if k != "records"}
+        d["errors"] = [{"rid": r.rid, "particles": r.particles, "steps": r.steps, "steps_done": r.steps_done,
+                        "error": r.error} for r in self.records if r.error][:20]
+        return d
 
 
 class Server:
