@@ -181,6 +181,27 @@ MPM_API void mpm_destroy(void* handle);
 MPM_API const char* mpm_last_error(void);
 MPM_API const char* mpm_version(void);
 
+/* ----------------------------------------------------------- device memory */
+/* No counterpart in the reference (one scene per process, its buffers
+ * freed by the destructor, src/mpm_solver3d.cu:93-98).  Solvers and
+ * batches take device memory from a per-device cache of released blocks,
+ * because hipFree waits for every stream of the device: a scene built or
+ * destroyed while batches run would otherwise stall behind all their
+ * queued steps.  A released block stays cached (up to the limit, per
+ * device; 16 GiB by default) for the next allocation of a similar size
+ * (a block at most 25% larger than the request's size class).  Cached
+ * bytes are not part of any handle's device_bytes.  Thread-safe. */
+typedef struct mpm_cache_info {
+    int64_t cached_bytes; /* released blocks kept, current device */
+    int64_t live_bytes;   /* blocks held by solvers and batches, current device */
+    int64_t hits;         /* allocations served from the cache (all devices) */
+    int64_t misses;       /* allocations that called hipMalloc (all devices) */
+} mpm_cache_info;
+MPM_API int mpm_set_cache_limit(int64_t bytes); /* trims every device to the new limit */
+MPM_API int mpm_get_cache_info(mpm_cache_info* out);
+/* Hands every cached block back to the driver (hipFree: waits for the device). */
+MPM_API int mpm_release_cache(void);
+
 /* -------------------------------------------------------------- time steps */
 
 /* simulateOneStep() repeated n times (src/mpm_solver3d.cu:100-125,

@@ -28,6 +28,7 @@
 #include <tuple>
 #include <vector>
 
+#include "devcache.h"
 #include "launch.h"
 #include "solver_impl.h"
 
@@ -117,24 +118,28 @@ struct Batch {
     int64_t device_bytes = 0;
 };
 
+// Device memory from the block cache (devcache.h), as for scenes.
 int balloc(Batch& b, void** p, size_t bytes) {
-    if (bytes == 0) bytes = 256;
-    hipError_t e = hipMalloc(p, bytes);
+    size_t got = 0;
+    hipError_t e = devcache::alloc(p, bytes, &got);
     if (e != hipSuccess) {
         *p = nullptr;
         (void)hipGetLastError();
         return fail(MPM_ERR_OOM, "hipMalloc(" + std::to_string(bytes) + " B) failed: " + hipGetErrorString(e));
     }
     b.allocs.push_back(*p);
-    b.device_bytes += (int64_t)bytes;
+    b.device_bytes += (int64_t)got;
     return MPM_OK;
 }
 
+// The batch stream is synchronised first: the block may be handed to
+// another stream at once.
 void bfree(Batch& b, void*& p) {
     if (!p) return;
     for (size_t i = 0; i < b.allocs.size(); ++i)
         if (b.allocs[i] == p) { b.allocs.erase(b.allocs.begin() + (long)i); break; }
-    (void)hipFree(p);
+    if (b.st) (void)hipStreamSynchronize(b.st);
+    b.device_bytes -= (int64_t)devcache::release(p);
     p = nullptr;
 }
 
@@ -488,7 +493,7 @@ void batch_destroy(Batch* b) {
     drop_graph(*b);
     for (auto& pr : b->pending) b->event_pool.push_back(pr.first), b->event_pool.push_back(pr.second);
     for (auto e : b->event_pool) (void)hipEventDestroy(e);
-    for (void* p : b->allocs) (void)hipFree(p);
+    for (void* p : b->allocs) (void)devcache::release(p);  // the stream is synchronised above
     if (b->st) (void)hipStreamDestroy(b->st);
     delete b;
 }

@@ -20,6 +20,7 @@
 #include <tuple>
 #include <vector>
 
+#include "devcache.h"
 #include "devmath.h"
 #include "launch.h"
 #include "solver_impl.h"
@@ -35,24 +36,29 @@ using namespace mpm;
 namespace mpm {
 namespace host {
 
+// Device memory comes from the block cache (devcache.h), so that building
+// and destroying scenes never waits for other streams' work.
 int dalloc(Solver& s, void** p, size_t bytes) {
-    if (bytes == 0) bytes = 256;
-    hipError_t e = hipMalloc(p, bytes);
+    size_t got = 0;
+    hipError_t e = devcache::alloc(p, bytes, &got);
     if (e != hipSuccess) {
         *p = nullptr;
         (void)hipGetLastError();
         return fail(MPM_ERR_OOM, "hipMalloc(" + std::to_string(bytes) + " B) failed: " + hipGetErrorString(e));
     }
     s.allocs.push_back(*p);
-    s.device_bytes += (int64_t)bytes;
+    s.device_bytes += (int64_t)got;
     return MPM_OK;
 }
 
+// The solver's stream (the batch's while it is a member) is synchronised
+// first: the block may be handed to another stream at once.
 void dfree(Solver& s, void* p) {
     if (!p) return;
     for (size_t i = 0; i < s.allocs.size(); ++i)
         if (s.allocs[i] == p) { s.allocs.erase(s.allocs.begin() + (long)i); break; }
-    (void)hipFree(p);
+    if (s.st) (void)hipStreamSynchronize(s.st);
+    s.device_bytes -= (int64_t)devcache::release(p);
 }
 
 // ---- profiling hook (hipEvent pairs on the solver stream)
@@ -675,7 +681,7 @@ void destroy(Solver* s) {
     if (s->graph) (void)hipGraphDestroy(s->graph);
     for (auto& p : s->pending) s->event_pool.push_back(p.a), s->event_pool.push_back(p.b);
     for (auto e : s->event_pool) (void)hipEventDestroy(e);
-    for (void* p : s->allocs) (void)hipFree(p);
+    for (void* p : s->allocs) (void)devcache::release(p);  // every stream of the scene is synchronised above
     if (s->st) (void)hipStreamDestroy(s->st);
     delete s;
 }
@@ -721,6 +727,31 @@ extern "C" {
 
 MPM_API const char* mpm_last_error(void) { return g_err.c_str(); }
 MPM_API const char* mpm_version(void) { return "mpm_hip 1.0 (gfx950)"; }
+
+MPM_API int mpm_set_cache_limit(int64_t bytes) {
+    if (bytes < 0) return fail(MPM_ERR_INVALID, "cache limit < 0");
+    devcache::set_limit((size_t)bytes);
+    return MPM_OK;
+}
+
+MPM_API int mpm_get_cache_info(mpm_cache_info* out) {
+    if (!out) return fail(MPM_ERR_INVALID, "out is NULL");
+    int dev = 0;
+    HIP_TRY(hipGetDevice(&dev));
+    if (dev < 0 || dev >= devcache::kMaxDevices) return fail(MPM_ERR_INVALID, "device index beyond the cache's table");
+    devcache::State& st = devcache::state();
+    std::lock_guard<std::mutex> g(st.mu);
+    out->cached_bytes = (int64_t)st.cached[dev];
+    out->live_bytes = (int64_t)st.live_bytes[dev];
+    out->hits = (int64_t)st.hits;
+    out->misses = (int64_t)st.misses;
+    return MPM_OK;
+}
+
+MPM_API int mpm_release_cache(void) {
+    devcache::release_all();
+    return MPM_OK;
+}
 
 MPM_API int mpm_create(const mpm_desc* desc, const mpm_particle_input* in, void** out_handle) {
     if (!out_handle) return fail(MPM_ERR_INVALID, "out_handle is NULL");

@@ -141,6 +141,10 @@ class MpmSceneHealth(C.Structure):
     _fields_ = [("max_speed", C.c_double), ("cfl", C.c_double), ("nonfinite", C.c_int64), ("escaped", C.c_int64)]
 
 
+class MpmCacheInfo(C.Structure):
+    _fields_ = [("cached_bytes", C.c_int64), ("live_bytes", C.c_int64), ("hits", C.c_int64), ("misses", C.c_int64)]
+
+
 # Every exported entry point and its signature (also used by the CPU test
 # that checks the library exports what the header declares).
 SIGNATURES = {
@@ -149,6 +153,9 @@ SIGNATURES = {
     "mpm_destroy": (None, [C.c_void_p]),
     "mpm_last_error": (C.c_char_p, []),
     "mpm_version": (C.c_char_p, []),
+    "mpm_set_cache_limit": (C.c_int, [C.c_int64]),
+    "mpm_get_cache_info": (C.c_int, [C.POINTER(MpmCacheInfo)]),
+    "mpm_release_cache": (C.c_int, []),
     "mpm_step": (C.c_int, [C.c_void_p, C.c_int]),
     "mpm_synchronize": (C.c_int, [C.c_void_p]),
     "mpm_step_graph": (C.c_int, [C.c_void_p, C.c_int]),

@@ -427,3 +427,40 @@ class Server:
 def requests_from_scenes(stream) -> list[ServeRequest]:
     """ServeRequests from scenes.request_stream() output."""
     return [ServeRequest(i, r.arrival, r.scene.desc, r.scene.particles, r.scene.steps) for i, r in enumerate(stream)]
+
+
+def main(argv=None) -> int:
+    """Serve a seeded request stream and print the report as one JSON line:
+    ``python3 -m mpm_amd.serve --seconds 60 --rate 20`` (``--burst``: every
+    request arrives at t=0, the server's capacity)."""
+    import argparse
+    import dataclasses
+    import json
+
+    from . import scenes
+
+    ap = argparse.ArgumentParser(description=main.__doc__)
+    ap.add_argument("--seed", type=int, default=4)
+    ap.add_argument("--rate", type=float, default=20.0, help="arrivals per second")
+    ap.add_argument("--seconds", type=float, default=60.0, help="arrival window")
+    ap.add_argument("--max-requests", type=int, default=None)
+    ap.add_argument("--burst", action="store_true", help="all requests arrive at t=0")
+    ap.add_argument("--budget-gb", type=float, default=200.0)
+    ap.add_argument("--tick-steps", type=int, default=16)
+    a = ap.parse_args(argv)
+    reqs = requests_from_scenes(scenes.request_stream(seed=a.seed, rate=a.rate, duration=a.seconds,
+                                                      max_requests=a.max_requests))
+    if a.burst:
+        reqs = [dataclasses.replace(r, arrival=0.0) for r in reqs]
+    print(f"[serve] {len(reqs)} requests", flush=True)
+    srv = Server(budget_bytes=a.budget_gb * 1e9, tick_steps=a.tick_steps)
+    try:
+        rep = srv.run(reqs)
+    finally:
+        srv.close()
+    print(json.dumps(rep.summary()), flush=True)
+    return 0 if rep.failed == 0 else 1
+
+
+if __name__ == "__main__":
+    raise SystemExit(main())

@@ -410,3 +410,22 @@ def debug_svd3(A: np.ndarray):
     s = np.zeros((n, 3), np.float32)
     N.check(lib.mpm_debug_svd3(N.fptr(A), n, N.fptr(U), N.fptr(s), N.fptr(V)))
     return U, s, V
+
+
+def cache_info() -> dict:
+    """Device block cache of the current device: bytes cached and held, and
+    allocations served from the cache (hits) or from hipMalloc (misses)."""
+    lib = N.load()
+    out = N.MpmCacheInfo()
+    N.check(lib.mpm_get_cache_info(C.byref(out)))
+    return {"cached_bytes": out.cached_bytes, "live_bytes": out.live_bytes, "hits": out.hits, "misses": out.misses}
+
+
+def set_cache_limit(nbytes: int):
+    """Bytes of released device blocks kept per device for reuse."""
+    N.check(N.load().mpm_set_cache_limit(int(nbytes)))
+
+
+def release_cache():
+    """Return every cached device block to the driver (waits for the device)."""
+    N.check(N.load().mpm_release_cache())

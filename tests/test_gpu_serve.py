@@ -5,6 +5,8 @@ tens of steps, so it uses the many-step tolerance: mass exact, position
 difference <= 1e-3 * dx, and Fe within 1e-3 relative."""
 from __future__ import annotations
 
+import json
+
 import numpy as np
 import pytest
 
@@ -103,3 +105,12 @@ def test_served_nan_scene_is_evicted_others_equal_alone():
              "Fe": rel_err(pm.Fe, pa.Fe)}
         report(f"beside a NaN scene: rid={r.rid} N={r.particles.count}", e)
         assert np.all(np.isfinite(pm.position)) and e["pos/dx"] <= 1e-3 and e["Fe"] <= 1e-3, e
+
+
+def test_command_line_driver(capsys):
+    from mpm_amd import serve
+
+    assert serve.main(["--seconds", "2", "--rate", "4", "--max-requests", "4", "--burst"]) == 0
+    rep = json.loads(capsys.readouterr().out.strip().splitlines()[-1])
+    print("driver report:", rep, flush=True)
+    assert rep["completed"] == rep["requests"] > 0 and rep["failed"] == 0 and rep["errors"] == []
