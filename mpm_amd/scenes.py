@@ -226,9 +226,15 @@ def request_stream(seed: int = 4, rate: float = 20.0, duration: float = 600.0, n
     """Poisson arrivals; each request a seeded scene (1-4 primitives,
     N log-uniform [n_lo, n_hi], grid 64^3-256^3 by size, PIC-FLIP or APIC
     50/50, steps uniform in [steps_lo, steps_hi])."""
+    return list(iter_request_stream(seed, rate, duration, n_lo, n_hi, steps_lo, steps_hi, max_requests))
+
+
+def iter_request_stream(seed: int = 4, rate: float = 20.0, duration: float = 600.0, n_lo: int = 5000,
+                        n_hi: int = 500000, steps_lo: int = 200, steps_hi: int = 2000,
+                        max_requests: int | None = None):
+    """The requests of request_stream() one at a time, in arrival order."""
     rng = Rng(seed)
     t = 0.0
-    out = []
     i = 0
     while True:
         t += -math.log1p(-rng.uniform(1)[0]) / rate
@@ -243,9 +249,8 @@ def request_stream(seed: int = 4, rate: float = 20.0, duration: float = 600.0, n
         vel = r.uniform(3, -1.0, 1.0)
         desc = SolverDesc((0.0, 0.0, 0.0), (res, res, res), 1.0 / res, FRICTION, 0.95, N.QUADRATIC_BSPLINE, transfer,
                           1e-4, GRAVITY)
-        out.append(Request(t, Scene(f"req_{i}", desc, snow_particles(pos, vel, RHO * vol / n), steps, r.seed)))
+        yield Request(t, Scene(f"req_{i}", desc, snow_particles(pos, vel, RHO * vol / n), steps, r.seed))
         i += 1
-    return out
 
 
 def workload(name: str, **kw) -> Scene:

@@ -163,8 +163,16 @@ class ServeReport:
 class Server:
     def __init__(self, budget_bytes: float = 200e9, tick_steps: int = 16, max_scenes: int | None = None,
                  backend=None, clock=time.perf_counter, keep_results: bool = False, sleep=time.sleep,
-                 cfl_limit: float = 1.0, evict_escaped: bool = True):
+                 cfl_limit: float = 1.0, evict_escaped: bool = True, builds_per_tick: int = 8,
+                 progress_s: float = 0.0, log=print):
         self.budget = int(budget_bytes)
+        # a progress line (arrived, resident, done, failed, particle-updates/s) every progress_s seconds
+        self.progress_s = float(progress_s)
+        self.log = log
+        # scenes built per admission pass while others run: a burst of
+        # arrivals is built a few at a time between and during ticks, so the
+        # GPU steps the first ones while the host builds the rest
+        self.builds_per_tick = max(1, int(builds_per_tick))
         self.tick_steps = int(tick_steps)
         self.max_scenes = max_scenes
         self.backend = backend or DeviceBackend()
@@ -307,6 +315,14 @@ class Server:
                 a.rec.cfl = float(r["cfl"])
                 self._drop(a, t0, f"evicted after {a.rec.steps_done} steps: {why}")
 
+    def _progress(self, t0):
+        now = self.clock() - t0
+        done = [r for r in self.records if r.finished >= 0 and not r.error]
+        pu = sum(r.particles * r.steps for r in done)
+        self.log(f"[serve] t={now:.0f}s arrived {len(self.records)} resident {len(self.active)} "
+                 f"done {len(done)} failed {sum(1 for r in self.records if r.error)} "
+                 f"{pu / max(now, 1e-9):.3e} particle-updates/s, {self.used / 1e9:.1f} GB committed")
+
     # -- main loop
     def run(self, requests) -> ServeReport:
         reqs = sorted(requests, key=lambda r: (r.arrival, r.rid))
@@ -336,16 +352,24 @@ class Server:
                 queue.append((r, rec))
 
         def admissions():
+            built = 0
             while queue and self._fits(queue[0][0], queue[0][1]):
+                if built >= self.builds_per_tick:  # the next tick runs what is built
+                    break
                 r, rec = queue.popleft()
                 self._build(r, rec, t0)
+                built += 1
 
         def joins():
             staged, self.staged = self.staged, []
             for a in staged:
                 self._join(a, t0)
 
+        next_report = self.progress_s
         while pending or queue or self.staged or self.active:
+            if self.progress_s > 0 and self.clock() - t0 >= next_report:
+                next_report += self.progress_s
+                self._progress(t0)
             arrivals()
             admissions()
             joins()
@@ -447,13 +471,19 @@ def main(argv=None) -> int:
     ap.add_argument("--burst", action="store_true", help="all requests arrive at t=0")
     ap.add_argument("--budget-gb", type=float, default=200.0)
     ap.add_argument("--tick-steps", type=int, default=16)
+    ap.add_argument("--progress", type=float, default=30.0, help="seconds between progress lines (0: none)")
     a = ap.parse_args(argv)
-    reqs = requests_from_scenes(scenes.request_stream(seed=a.seed, rate=a.rate, duration=a.seconds,
-                                                      max_requests=a.max_requests))
+    stream = []
+    for r in scenes.iter_request_stream(seed=a.seed, rate=a.rate, duration=a.seconds, max_requests=a.max_requests):
+        stream.append(r)
+        if len(stream) % 1000 == 0:
+            print(f"[serve] generated {len(stream)} requests", flush=True)
+    reqs = requests_from_scenes(stream)
     if a.burst:
         reqs = [dataclasses.replace(r, arrival=0.0) for r in reqs]
     print(f"[serve] {len(reqs)} requests", flush=True)
-    srv = Server(budget_bytes=a.budget_gb * 1e9, tick_steps=a.tick_steps)
+    srv = Server(budget_bytes=a.budget_gb * 1e9, tick_steps=a.tick_steps, progress_s=a.progress,
+                 log=lambda m: print(m, flush=True))
     try:
         rep = srv.run(reqs)
     finally:

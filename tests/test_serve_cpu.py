@@ -103,6 +103,38 @@ def test_every_request_gets_exactly_its_steps():
     assert rep.particle_updates == sum(r.particles * r.steps for r in rep.records)
 
 
+def test_burst_is_built_a_few_scenes_per_tick():
+    """20 requests arriving at once: the first tick runs after 4 builds, and
+    the rest are built between and during ticks, still in arrival order."""
+    reqs = [_req(i, 0.0, 2000, 300) for i in range(20)]
+    srv, be, _ = _server(1e12, tick_steps=16, builds_per_tick=4)
+    events = []
+    create = be.create_solver
+
+    def logged_create(desc, particles):
+        events.append("build")
+        return create(desc, particles)
+
+    be.create_solver = logged_create
+    step = FakeBatch.step
+
+    def logged_step(self, k):
+        events.append("step")
+        step(self, k)
+
+    FakeBatch.step = logged_step
+    try:
+        rep = srv.run(reqs)
+    finally:
+        FakeBatch.step = step
+    assert events[:5] == ["build"] * 4 + ["step"], events[:8]
+    assert events.count("build") == 20 and rep.completed == 20
+    adm = [r.admitted for r in sorted(rep.records, key=lambda r: r.rid)]
+    assert adm == sorted(adm)
+    for r in rep.records:
+        assert r.result == {"steps": r.steps}
+
+
 def test_budget_limits_resident_scenes_and_keeps_arrival_order():
     reqs = [_req(i, 0.0, 2000, 50) for i in range(6)]
     one = estimate_bytes(reqs[0].desc, 2000)
