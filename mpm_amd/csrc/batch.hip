@@ -95,6 +95,7 @@ struct Batch {
     int64_t chunk_max = 0;
     int64_t* chunk_bounds = nullptr;  // nchunks + 1
     int chunk_bounds_cap = 0;
+    std::vector<int64_t> chunk_host;  // host copy of the bounds (the source of their upload)
     uint32_t* chunk_rr = nullptr;
     uint32_t* slot_scene = nullptr;  // cap words: each global slot's member
     int64_t slot_scene_cap = 0;
@@ -190,14 +191,13 @@ SceneEntry make_entry(const Solver& s, int64_t pstart, uint32_t koff, uint32_t i
 // chunk of its own); their position bounds, each slot's member, the scratch
 // of the largest chunk, and the chunk index reset to 0.  No chunks when the
 // reorder is off.
-std::vector<int64_t> g_chunk_host;  // host staging of the bounds (the upload is waited for)
 int layout_chunks(Batch& b) {
     b.nchunks = 0;
     b.chunk_max = 0;
     const int ns = (int)b.members.size();
     if (b.reorder_interval <= 0 || ns == 0 || b.ntot == 0) return MPM_OK;
     const int64_t target = (b.ntot + b.reorder_interval - 1) / b.reorder_interval;
-    std::vector<int64_t>& bounds = g_chunk_host;
+    std::vector<int64_t>& bounds = b.chunk_host;
     bounds.assign(1, 0);
     int64_t acc = 0;
     for (int i = 0; i < ns; ++i) {

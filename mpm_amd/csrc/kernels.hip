@@ -62,27 +62,26 @@ static unsigned resident_grid(F* kernel, int block, int requested) {
 #define MPM_PERSISTENT(K, BLOCK, REQ, ST, ...) \
     hipLaunchKernelGGL(K, dim3(resident_grid(K, BLOCK, REQ)), dim3(BLOCK), 0, ST, __VA_ARGS__)
 
-// Minimum waves per SIMD asked of the register allocator for G2P.  APIC
-// G2P with 128-thread workgroups fits 128 VGPRs for the linear and
-// quadratic stencils (4 waves per SIMD, no spill; 64 and 256 threads would
-// spill 8-10 registers under that cap), and the other APIC variants fit the
-// 168 of 3 waves.  Measured on the 8.4M-particle slab: G2P 0.833 ms (64
-// threads, 3 waves) -> 0.821 ms (128 threads, 4 waves).  PIC-FLIP G2P keeps
-// the compiler's allocation for the cubic stencil and 256 threads
-// (MPM_PIC_G2P_WAVES; cubic 156-179 VGPRs) and is capped at 4 waves for the
-// linear and quadratic stencils at 64 and 128 threads, which fit 128 VGPRs
-// without spilling.
+// Minimum waves per SIMD asked of the register allocator for G2P.  The
+// linear and quadratic G2P need 147-161 VGPRs; capped at 128 (4 waves per
+// SIMD) they spill 60-90 bytes per lane, which costs more than the fourth
+// wave gains.  Measured with 128-thread workgroups, per step: the 256-scene
+// APIC batch 2.54 -> 2.39 ms, the same scenes as PIC-FLIP 2.86 -> 2.51 ms,
+// the 8.4M-particle slab's two-kernel step 1.52 -> 1.44 ms, going from 4
+// waves to 3 (168 VGPRs, no spill).  PIC-FLIP G2P keeps the compiler's
+// allocation for the cubic stencil and 256 threads (MPM_PIC_G2P_WAVES; cubic
+// 156-179 VGPRs).
 #ifndef MPM_APIC_G2P_WAVES
 #define MPM_APIC_G2P_WAVES 3
 #endif
 #ifndef MPM_APIC_G2P_WAVES_WG128
-#define MPM_APIC_G2P_WAVES_WG128 4
+#define MPM_APIC_G2P_WAVES_WG128 3
 #endif
 #ifndef MPM_PIC_G2P_WAVES
 #define MPM_PIC_G2P_WAVES 1
 #endif
 #ifndef MPM_PIC_G2P_WAVES_LQ
-#define MPM_PIC_G2P_WAVES_LQ 4
+#define MPM_PIC_G2P_WAVES_LQ 3
 #endif
 template <int INTERP, int APIC, int WG>
 struct G2PWaves {
@@ -941,11 +940,12 @@ __global__ void __launch_bounds__(WG) __attribute__((amdgpu_waves_per_eu(G2PWave
     // folded grid: the next item's tile loads overlap this item
     constexpr bool PREFETCH = SRC == 3;
     constexpr int TR = PREFETCH ? (TN + WG - 1) / WG : 1;
-    // and each particle's Fe, Fp, mu0, lam0, V0 go straight into LDS when
+    // and each particle's Fe, Jp, mu0, lam0, V0 go straight into LDS when
     // the particle is picked, to be read after the gather (not for cubic:
-    // its larger tile would cut the workgroups per CU)
+    // its larger tile would cut the workgroups per CU); Fp is read only by
+    // a wave with a plastic update
     constexpr bool PSTAGE = PREFETCH && INTERP != MPM_INTERP_CUBIC_BSPLINE;
-    constexpr int PW = 21;  // staged words per particle
+    constexpr int PW = 13;  // staged words per particle
     __shared__ float tile[NF * TN];
     __shared__ float stage[PREFETCH ? 4 * TN : 1];
     __shared__ float pstage[PSTAGE ? PW * WG : 1];
@@ -1027,15 +1027,13 @@ __global__ void __launch_bounds__(WG) __attribute__((amdgpu_waves_per_eu(G2PWave
             float* const pst = pstage + (threadIdx.x >> 6) * (PW * 64);  // the wave's region
             if constexpr (PSTAGE) {
 #pragma unroll 1
-                for (int f = 0; f < 9; ++f) {
+                for (int f = 0; f < 9; ++f)
                     __builtin_amdgcn_global_load_lds((const void*)(P.Fe + (int64_t)f * N + p), (lds_void*)(pst + f * 64),
                                                      4, 0, 0);
-                    __builtin_amdgcn_global_load_lds((const void*)(P.Fp + (int64_t)f * N + p),
-                                                     (lds_void*)(pst + (9 + f) * 64), 4, 0, 0);
-                }
-                __builtin_amdgcn_global_load_lds((const void*)(P.mu0 + p), (lds_void*)(pst + 18 * 64), 4, 0, 0);
-                __builtin_amdgcn_global_load_lds((const void*)(P.lam0 + p), (lds_void*)(pst + 19 * 64), 4, 0, 0);
-                __builtin_amdgcn_global_load_lds((const void*)(P.V0 + p), (lds_void*)(pst + 20 * 64), 4, 0, 0);
+                __builtin_amdgcn_global_load_lds((const void*)(P.Jp + p), (lds_void*)(pst + 9 * 64), 4, 0, 0);
+                __builtin_amdgcn_global_load_lds((const void*)(P.mu0 + p), (lds_void*)(pst + 10 * 64), 4, 0, 0);
+                __builtin_amdgcn_global_load_lds((const void*)(P.lam0 + p), (lds_void*)(pst + 11 * 64), 4, 0, 0);
+                __builtin_amdgcn_global_load_lds((const void*)(P.V0 + p), (lds_void*)(pst + 12 * 64), 4, 0, 0);
             }
             const float x0 = P.x[p], x1 = P.x[N + p], x2 = P.x[2 * N + p];
             int bx = 0, by = 0, bz = 0;
@@ -1173,20 +1171,18 @@ __global__ void __launch_bounds__(WG) __attribute__((amdgpu_waves_per_eu(G2PWave
                 n1 = S.blend * vf1 + ob * vpic1;
                 n2 = S.blend * vf2 + ob * vpic2;
             }
-            float Fe[3][3], Fp[3][3], mu0, lam0, V0;
+            float Fe[3][3], Jp0, mu0, lam0, V0;
             if constexpr (PSTAGE) {
                 // staged words have landed (waited for before this
                 // particle's stores are issued, so the wait covers loads only)
                 asm volatile("s_waitcnt vmcnt(0)" ::: "memory");
                 const float* q = pst + (threadIdx.x & 63);
 #pragma unroll
-                for (int f = 0; f < 9; ++f) {
-                    Fe[f / 3][f % 3] = q[f * 64];
-                    Fp[f / 3][f % 3] = q[(9 + f) * 64];
-                }
-                mu0 = q[18 * 64];
-                lam0 = q[19 * 64];
-                V0 = q[20 * 64];
+                for (int f = 0; f < 9; ++f) Fe[f / 3][f % 3] = q[f * 64];
+                Jp0 = q[9 * 64];
+                mu0 = q[10 * 64];
+                lam0 = q[11 * 64];
+                V0 = q[12 * 64];
             }
             // Velocity, B, position and next sort key first: they depend on
             // the gather only, and storing them before the F update frees
@@ -1211,7 +1207,7 @@ __global__ void __launch_bounds__(WG) __attribute__((amdgpu_waves_per_eu(G2PWave
             }
             if constexpr (!PSTAGE) {
                 load_m3(P.Fe, N, p, Fe);
-                load_m3(P.Fp, N, p, Fp);
+                Jp0 = P.Jp[p];
                 mu0 = P.mu0[p];
                 lam0 = P.lam0[p];
                 V0 = P.V0[p];
@@ -1222,11 +1218,14 @@ __global__ void __launch_bounds__(WG) __attribute__((amdgpu_waves_per_eu(G2PWave
             const bool elastic = mat.model == MPM_MODEL_FIXED_COROTATED && elastic_trial(Fn, mat, E, J);
             if (__ballot(!elastic) == 0ull) {
                 // every particle of the wave stays inside its thresholds:
-                // Fe = Fn, Fp unchanged (not stored), stress without an SVD
-                elastic_fc_stress(E, J, m3_det(Fp), mu0, lam0, V0, mat, tau);
+                // Fe = Fn, Fp unchanged (neither read nor stored), stress
+                // without an SVD from the cached det Fp
+                elastic_fc_stress(E, J, Jp0, mu0, lam0, V0, mat, tau);
                 store_m3(P.Fe, N, p, Fn);
             } else {
-                float U[3][3], sv[3], V[3][3];
+                // a plastic update somewhere in the wave: Fp from memory
+                float Fp[3][3], U[3][3], sv[3], V[3][3];
+                load_m3(P.Fp, N, p, Fp);
                 update_deformation(Fn, Fe, Fp, mat, U, sv, V);
                 // stress for the next P2G from the same decomposition
                 const float Jp = m3_det(Fp);
