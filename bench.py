@@ -307,7 +307,7 @@ def batch_bench(count, steps, warmup, seed=3):
         b.add(s)
     tune = b.autotune(trial_steps=4)
     log(f"autotune: item_max={tune.item_max} g2p_threads={tune.g2p_threads} workgroups={tune.workgroups} "
-        f"({tune.step_ms:.3f} ms/step)")
+        f"fuse_transfer={tune.fuse_transfer} ({tune.step_ms:.3f} ms/step)")
     out = {}
     for mode in ("eager", "graph"):
         run = b.step if mode == "eager" else b.step_graph

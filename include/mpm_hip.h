@@ -374,6 +374,14 @@ typedef struct mpm_tune {
                              round per kernel (its occupancy x CUs)              */
     int32_t bucket;       /* out: size bucket, floor(log2(particles))              */
     double step_ms;       /* out (autotune): measured ms per step of the choice    */
+    int32_t fuse_transfer; /* 1: G2P fused with the next step's P2G (APIC batches
+                              with the quadratic stencil whose members do not
+                              record their grids; ignored otherwise): one kernel
+                              scatters each member's next P2G into a second grid
+                              buffer of the member (one more grid of device
+                              memory), then the batch table's two grids of every
+                              member are swapped.  0 by default: it pays when few
+                              particles change cell per step (see the autotuner) */
 } mpm_tune;
 
 typedef struct mpm_batch_stats {
@@ -413,8 +421,10 @@ MPM_API int mpm_batch_synchronize(void* batch);
 /* Fixes the launch parameters; NULL returns to the tuning table. */
 MPM_API int mpm_batch_set_tune(void* batch, const mpm_tune* tune);
 /* Times candidate parameters on the batch's own scenes (every trial step
- * is a real step of every member: 1 + trial_steps per candidate, at most 7
- * candidates), keeps the fastest for the batch and records it in the
+ * is a real step of every member: 1 + trial_steps per candidate, at most 9
+ * candidates: work-item size, G2P workgroup, persistent workgroups, then
+ * the fused transfer where the batch allows it), keeps the fastest for the
+ * batch and records it in the
  * process-wide table for this (interp, transfer, size bucket). */
 MPM_API int mpm_batch_autotune(void* batch, int trial_steps, mpm_tune* out);
 MPM_API int mpm_tune_lookup(int32_t interp, int32_t transfer, int64_t particles, mpm_tune* out);

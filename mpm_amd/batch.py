@@ -27,13 +27,15 @@ class Tune:
     workgroups: int = 0
     bucket: int = 0
     step_ms: float = 0.0
+    fuse_transfer: int = 0  # 1: G2P fused with the next P2G (APIC quadratic batches)
 
     def to_c(self) -> N.MpmTune:
-        return N.MpmTune(self.item_max, self.g2p_threads, self.workgroups, self.bucket, self.step_ms)
+        return N.MpmTune(self.item_max, self.g2p_threads, self.workgroups, self.bucket, self.step_ms,
+                         self.fuse_transfer)
 
     @staticmethod
     def from_c(t: N.MpmTune) -> "Tune":
-        return Tune(t.item_max, t.g2p_threads, t.workgroups, t.bucket, t.step_ms)
+        return Tune(t.item_max, t.g2p_threads, t.workgroups, t.bucket, t.step_ms, t.fuse_transfer)
 
 
 class Batch:

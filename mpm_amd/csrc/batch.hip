@@ -37,9 +37,12 @@ namespace host {
 
 struct Tune {
     int item_max = 256;  // what the autotuner picks for the 256-scene uneven batch
-    int g2p_wg = 128;  // as the single-scene G2P (4 waves per SIMD for linear/quadratic APIC)
+    int g2p_wg = 128;  // as the single-scene G2P
     int wgs = 0;  // 0: one resident round per kernel (occupancy x CUs)
-    bool operator==(const Tune& o) const { return item_max == o.item_max && g2p_wg == o.g2p_wg && wgs == o.wgs; }
+    int fuse = 0;  // the fused transfer (APIC, folded grid, quadratic stencil)
+    bool operator==(const Tune& o) const {
+        return item_max == o.item_max && g2p_wg == o.g2p_wg && wgs == o.wgs && fuse == o.fuse;
+    }
 };
 
 struct TuneEntry {
@@ -82,6 +85,13 @@ struct Batch {
     Tune tune;
     bool tune_fixed = false;  // set explicitly: the tuning table is not consulted
     bool fold = false;        // APIC with no member recording its grid: 4-plane P2G
+    // transfer fusion (tune.fuse; APIC, folded grid, quadratic stencil): one
+    // kernel runs the step's G2P and the next step's P2G into each member's
+    // second grid, then a small kernel swaps every entry's grid and grid2 on
+    // the device.  parity: swaps since the members' host pointers last
+    // matched the table (odd: each member's S.grid / grid2 are the other way
+    // round there).
+    int parity = 0;
     // every reorder_interval steps (0 = never) each member's store is
     // permuted into its range of the global cell order, a chunk of members
     // per step (chunks: about ntot / reorder_interval particles each, bounds
@@ -183,7 +193,31 @@ SceneEntry make_entry(const Solver& s, int64_t pstart, uint32_t koff, uint32_t i
     e.active = (decltype(e.active))s.acc;  // acc[0]: active-node sum over steps
     e.key_offset = koff;
     e.index = index;
+    e.grid2 = s.grid2;
+    e.touched2 = s.touched2;
+    e.prescat = s.bprescat ? 1u : 0u;
     return e;
+}
+
+bool can_fuse(const Batch& b) { return b.apic && b.fold && b.interp == MPM_INTERP_QUADRATIC_BSPLINE; }
+bool fuses(const Batch& b) { return b.tune.fuse != 0 && can_fuse(b); }
+
+// The members' host pointers and the host table made equal to the device
+// table again (after an odd number of fused steps each entry has its grid
+// buffers the other way round).
+void normalize(Batch& b) {
+    if (!b.parity) return;
+    for (size_t i = 0; i < b.members.size(); ++i) {
+        Solver& s = *b.members[i];
+        std::swap(s.S.grid, s.grid2);
+        std::swap(s.S.touched, s.touched2);
+        if (i < b.tab_host.size()) {
+            SceneEntry& e = b.tab_host[i];
+            std::swap(e.S.grid, e.grid2);
+            std::swap(e.S.touched, e.touched2);
+        }
+    }
+    b.parity = 0;
 }
 
 // The reorder chunks of the current layout: consecutive members, closed
@@ -239,6 +273,7 @@ int layout_chunks(Batch& b) {
 // the sort input (keys with offsets, identity values).
 int relayout(Batch& b) {
     drop_graph(b);
+    normalize(b);  // the table is rebuilt from the members' host pointers
     const int ns = (int)b.members.size();
     b.fold = b.apic != 0;
     for (const Solver* m : b.members) b.fold = b.fold && !m->record_grid;
@@ -313,15 +348,23 @@ int sync_members(Batch& b) {
         if (rc) return rc;
     }
     bool upload = false;
+    const bool fuse = fuses(b);
     for (size_t i = 0; i < b.members.size(); ++i) {
         Solver& s = *b.members[i];
+        if (!fuse) batch_unfuse_member(&b, s);  // fusion turned off (tune) or no longer possible
+        if (fuse) {
+            int rc = ensure_grid2_buffers(s);
+            if (rc) return rc;
+        }
         SceneEntry e = make_entry(s, b.tab_host[i].pstart, b.tab_host[i].key_offset, (uint32_t)i);
         if (std::memcmp(&e, &b.tab_host[i], sizeof(SceneEntry)) != 0) {
             b.tab_host[i] = e;
             upload = true;
         }
-        if (s.grid_dirty && (!s.count_on_clear || s.vstar_dirty || (b.fold && s.force_dirty))) run_clear(s);
-        ensure_stress(s);
+        if (!s.bprescat) {  // a pre-scattered member's grid holds its next sums; the fused step needs no stress cache
+            if (s.grid_dirty && (!s.count_on_clear || s.vstar_dirty || (b.fold && s.force_dirty))) run_clear(s);
+            ensure_stress(s);
+        }
         if (b.synced_version[i] != s.version) {
             launch_keys(s.S, s.P, b.interp, b.kin + b.tab_host[i].pstart, b.st, b.tab_host[i].key_offset);
             b.synced_version[i] = s.version;
@@ -339,9 +382,16 @@ int sync_members(Batch& b) {
 // reorder: permute every member's store by this step's own sort (eager
 // steps; reorder_scratch allocated), after which the step runs on the
 // identity permutation.
-void enqueue_step(Batch& b) {
+// fused: the fused transfer; scattered: every member's grid already holds
+// this step's P2G (else the step clears and scatters the others first).
+void enqueue_step(Batch& b, bool fused, bool scattered) {
     const int ns = (int)b.members.size();
-    launch_clear_batch(b.tab, b.block_start, ns, b.total_blocks, b.fold ? GF_FOLDED : GF_PRIMARY, b.st);
+    if (!fused) {
+        launch_clear_batch(b.tab, b.block_start, ns, b.total_blocks, b.fold ? GF_FOLDED : GF_PRIMARY, b.st);
+    } else {
+        if (!scattered) launch_clear_batch(b.tab, b.block_start, ns, b.total_blocks, GF_FOLDED, b.st, 1);
+        launch_clear_batch(b.tab, b.block_start, ns, b.total_blocks, GF_FOLDED, b.st, 2);
+    }
     const int sel = radix_sort_pairs(b.kin, b.iota, b.ka, b.va, b.kb, b.vb, b.ntot, b.key_bits, b.hist, b.scan_temp,
                                      b.st, ProfileHook{});
     b.I.sorted_keys = sel ? b.kb : b.ka;
@@ -354,14 +404,28 @@ void enqueue_step(Batch& b) {
     const SceneDev S0{};
     const ParticlesDev P0{};
     const int wgs = effective_wgs(b);
+    if (fused) {
+        // P2G only for members whose grid is not pre-scattered (the kernel
+        // skips the others' items), then G2P fused with the next P2G
+        if (!scattered) launch_p2g(S0, P0, b.I, b.interp, b.apic, 2, wgs, b.st, b.tab);
+        launch_g2p2g_batch(b.tab, b.I, wgs, b.kin, b.st);
+        launch_swap_grids(b.tab, ns, b.st);
+        return;
+    }
     launch_p2g(S0, P0, b.I, b.interp, b.apic, b.fold ? 2 : 0, wgs, b.st, b.tab);
     launch_g2p(S0, P0, b.I, b.interp, b.apic, b.fold ? 3 : 0, 1, 0, wgs, b.st, b.kin, b.tab, b.tune.g2p_wg);
 }
 
 // Host-side bookkeeping of the members after n batch steps.
-void after_steps(Batch& b, int n) {
+void after_steps(Batch& b, int n, bool fused) {
+    if (fused) b.parity ^= (n & 1);
     for (size_t i = 0; i < b.members.size(); ++i) {
         Solver& s = *b.members[i];
+        if (fused) {
+            s.bprescat = true;
+            b.tab_host[i].prescat = 1u;
+            s.stress_valid = false;  // the fused kernel uses the new stress at once
+        }
         s.steps += n;
         s.version += (uint64_t)n;
         b.synced_version[i] = s.version;  // the batch G2P wrote the keys of the new positions
@@ -432,40 +496,53 @@ int reorder_members(Batch& b) {
     return MPM_OK;
 }
 
+int capture_step(Batch& b, bool fused) {
+    drop_graph(b);
+    HIP_TRY(hipStreamBeginCapture(b.st, hipStreamCaptureModeThreadLocal));
+    enqueue_step(b, fused, fused);
+    hipError_t e = hipStreamEndCapture(b.st, &b.graph);
+    if (e != hipSuccess) return fail(MPM_ERR_HIP, std::string("batch graph capture: ") + hipGetErrorString(e));
+    HIP_TRY(hipGraphInstantiate(&b.gexec, b.graph, nullptr, nullptr, 0));
+    b.graph_tune = b.tune;
+    return MPM_OK;
+}
+
 int run_steps(Batch& b, int n, bool graph) {
     int rc = sync_members(b);
     if (rc) return rc;
+    const bool fused = fuses(b);
     if (b.members.empty() || b.ntot == 0 || n <= 0) {
-        if (!b.members.empty() && n > 0) after_steps(b, n);
+        if (!b.members.empty() && n > 0) after_steps(b, n, false);
         return MPM_OK;
     }
-    if (graph && (!b.gexec || !(b.graph_tune == b.tune))) {
-        drop_graph(b);
-        HIP_TRY(hipStreamBeginCapture(b.st, hipStreamCaptureModeThreadLocal));
-        enqueue_step(b);
-        hipError_t e = hipStreamEndCapture(b.st, &b.graph);
-        if (e != hipSuccess) return fail(MPM_ERR_HIP, std::string("batch graph capture: ") + hipGetErrorString(e));
-        HIP_TRY(hipGraphInstantiate(&b.gexec, b.graph, nullptr, nullptr, 0));
-        b.graph_tune = b.tune;
-    }
+    // A fused step whose members are not all pre-scattered also scatters
+    // the others' P2G first; it runs eagerly, and the steps after it (and a
+    // captured graph) start from pre-scattered grids.
+    bool scattered = fused;
+    for (const Solver* m : b.members) scattered = scattered && m->bprescat;
     auto t0 = std::chrono::steady_clock::now();
     for (int i = 0; i < n; ++i) {
+        const bool first_scatter = fused && !scattered;
+        if (graph && !first_scatter && (!b.gexec || !(b.graph_tune == b.tune))) {
+            if ((rc = capture_step(b, fused))) return rc;
+        }
         hipEvent_t e0 = nullptr, e1 = nullptr;
         if (b.timing) {
             e0 = take_event(b);
             e1 = take_event(b);
             if (e0 && e1) HIP_TRY(hipEventRecord(e0, b.st));
         }
-        if (graph) HIP_TRY(hipGraphLaunch(b.gexec, b.st));
-        else enqueue_step(b);
+        if (graph && !first_scatter) HIP_TRY(hipGraphLaunch(b.gexec, b.st));
+        else enqueue_step(b, fused, scattered);
         if (e0 && e1) {
             HIP_TRY(hipEventRecord(e1, b.st));
             b.pending.push_back({e0, e1});
         }
+        scattered = fused;
     }
     auto t1 = std::chrono::steady_clock::now();
     b.last_step_ms = std::chrono::duration<double, std::milli>(t1 - t0).count();
-    after_steps(b, n);
+    after_steps(b, n, fused);
     hipError_t e = hipGetLastError();
     if (e != hipSuccess) return fail(MPM_ERR_HIP, std::string("batch step: ") + hipGetErrorString(e));
     return MPM_OK;
@@ -475,6 +552,8 @@ int detach(Batch& b, Solver& s) {
     auto it = std::find(b.members.begin(), b.members.end(), &s);
     if (it == b.members.end()) return fail(MPM_ERR_NOT_FOUND, "scene is not a member of this batch");
     HIP_TRY(hipStreamSynchronize(b.st));
+    normalize(b);  // while the member list still matches the table
+    batch_unfuse_member(&b, s);
     b.members.erase(it);
     s.st = s.own_st;
     s.own_st = nullptr;
@@ -503,6 +582,22 @@ void batch_forget(Batch* b, Solver* s) {
     if (b && s) (void)detach(*b, *s);
 }
 
+// A member leaves the fused state: its grid becomes the one its last step
+// read and the pre-scattered sums wait in its grid2 to be cleared, as the
+// two-kernel step leaves them; the next batch step scatters its P2G again
+// (the changed table entry is uploaded before that step).
+void batch_unfuse_member(Batch* b, Solver& s) {
+    if (!b || !s.bprescat) return;
+    normalize(*b);
+    std::swap(s.S.grid, s.grid2);
+    std::swap(s.S.touched, s.touched2);
+    s.grid2_dirty = true;
+    s.grid2_count = false;
+    s.bprescat = false;
+    s.stress_valid = false;
+    drop_graph(*b);
+}
+
 void batch_invalidate(Batch* b) {
     if (!b) return;
     b->layout_valid = false;
@@ -516,6 +611,7 @@ int check_tune(const mpm_tune* t) {
     if (t->g2p_threads != 64 && t->g2p_threads != 128 && t->g2p_threads != 256)
         return fail(MPM_ERR_INVALID, "g2p_threads must be 64, 128 or 256");
     if (t->workgroups < 0 || t->workgroups > (1 << 20)) return fail(MPM_ERR_INVALID, "workgroups out of range");
+    if (t->fuse_transfer != 0 && t->fuse_transfer != 1) return fail(MPM_ERR_INVALID, "fuse_transfer must be 0 or 1");
     return MPM_OK;
 }
 
@@ -525,6 +621,7 @@ void to_c(const Batch& b, const Tune& t, mpm_tune* out, double ms) {
     out->workgroups = t.wgs;
     out->bucket = size_bucket(b.ntot);
     out->step_ms = ms;
+    out->fuse_transfer = t.fuse;
 }
 
 Batch* as_batch(void* h) { return static_cast<Batch*>(h); }
@@ -590,6 +687,7 @@ MPM_API int mpm_batch_add(void* batch, void* scene) {
     ensure_stress(*s);
     HIP_TRY(hipStreamSynchronize(s->st));
     HIP_TRY(hipStreamSynchronize(b->st));
+    normalize(*b);  // before the member list changes: the new scene's pointers are its own
     s->own_st = s->st;
     s->st = b->st;
     s->batch = b;
@@ -672,6 +770,7 @@ MPM_API int mpm_batch_set_tune(void* batch, const mpm_tune* t) {
     b->tune.item_max = t->item_max;
     b->tune.g2p_wg = t->g2p_threads;
     b->tune.wgs = t->workgroups;
+    b->tune.fuse = t->fuse_transfer;
     b->tune_fixed = true;
     return MPM_OK;
 }
@@ -718,12 +817,17 @@ MPM_API int mpm_batch_autotune(void* batch, int trial_steps, mpm_tune* out) {
     Tune best;
     double best_ms = 0.0;
     rc = measure(best, best_ms);
-    for (int stage = 0; stage < 3 && !rc; ++stage) {
-        for (int k = 0; k < (stage == 0 ? 4 : 3) && !rc; ++k) {
+    // the last stage: the fused transfer, where the batch allows it (it wins
+    // when few particles change cell per step: each one that does scatters
+    // its 27 nodes with global atomics)
+    for (int stage = 0; stage < 4 && !rc; ++stage) {
+        if (stage == 3 && !can_fuse(*b)) break;
+        for (int k = 0; k < (stage == 0 ? 4 : stage == 3 ? 2 : 3) && !rc; ++k) {
             Tune t = best;
             if (stage == 0) t.item_max = item_opts[k];
             if (stage == 1) t.g2p_wg = wg_opts[k];
             if (stage == 2) t.wgs = wgs_opts[k];
+            if (stage == 3) t.fuse = k;
             if (t == best) continue;
             double ms = 0.0;
             rc = measure(t, ms);

@@ -432,19 +432,22 @@ static bool use_fused_transfer(const Solver& s) {
 
 // The second grid buffer of the fused transfer and the stream that clears
 // it (allocated on first use).
-static int ensure_grid2(Solver& s) {
-    if (s.grid2) return MPM_OK;
-    if (!s.aux_st) HIP_TRY(hipStreamCreateWithFlags(&s.aux_st, hipStreamNonBlocking));
-    if (!s.aux_fork) HIP_TRY(hipEventCreateWithFlags(&s.aux_fork, hipEventDisableTiming));
-    if (!s.aux_join) HIP_TRY(hipEventCreateWithFlags(&s.aux_join, hipEventDisableTiming));
+int ensure_grid2_buffers(Solver& s) {
+    if (s.grid2 && s.touched2) return MPM_OK;
     const size_t gbytes = (size_t)s.S.plane * GF_ALL * sizeof(float);
-    int rc = dalloc(s, (void**)&s.grid2, gbytes);
-    if (rc) return rc;
-    if ((rc = dalloc(s, (void**)&s.touched2, (size_t)s.nsb * 4))) return rc;
+    int rc;
+    if (!s.grid2 && (rc = dalloc(s, (void**)&s.grid2, gbytes))) return rc;
+    if (!s.touched2 && (rc = dalloc(s, (void**)&s.touched2, (size_t)s.nsb * 4))) return rc;
     HIP_TRY(hipMemsetAsync(s.grid2, 0, gbytes, s.st));
     HIP_TRY(hipMemsetAsync(s.touched2, 0, (size_t)s.nsb * 4, s.st));
     s.grid2_dirty = false;
     return MPM_OK;
+}
+static int ensure_grid2(Solver& s) {
+    if (!s.aux_st) HIP_TRY(hipStreamCreateWithFlags(&s.aux_st, hipStreamNonBlocking));
+    if (!s.aux_fork) HIP_TRY(hipEventCreateWithFlags(&s.aux_fork, hipEventDisableTiming));
+    if (!s.aux_join) HIP_TRY(hipEventCreateWithFlags(&s.aux_join, hipEventDisableTiming));
+    return ensure_grid2_buffers(s);
 }
 
 // Clear of the second grid buffer (the grid the last fused step read, its
@@ -688,9 +691,12 @@ void destroy(Solver* s) {
 
 Solver* as_solver(void* h) { return static_cast<Solver*>(h); }
 
+// A member of a batch leaves the batch's fused state before any call of
+// its own (its grid pointers and stress cache become its own again).
 #define GET_SOLVER(var, h)                                                         \
     Solver* var = as_solver(h);                                                    \
-    if (!var) return fail(MPM_ERR_INVALID, "handle is NULL")
+    if (!var) return fail(MPM_ERR_INVALID, "handle is NULL");                      \
+    if (var->batch) batch_unfuse_member(var->batch, *var)
 
 int finish(Solver& s) {
     hipError_t e = hipGetLastError();

@@ -98,6 +98,14 @@ struct SceneEntry {
     unsigned long long MPM_GLOBAL* active;  // the scene's active-node counter
     uint32_t key_offset;
     uint32_t index;
+    // the fused batch step: the second grid buffer, into which the step's
+    // G2P scatters the next step's P2G (swapped with S.grid after the
+    // step), and whether S.grid already holds the P2G of the current
+    // particles (prescat; the step then runs no P2G for this scene)
+    float MPM_GLOBAL* grid2;
+    uint32_t MPM_GLOBAL* touched2;
+    uint32_t prescat;
+    uint32_t pad_;
 };
 
 // Incremental cell sort of one large scene (incsort.hip): G2P appends each

@@ -649,6 +649,8 @@ __global__ void __launch_bounds__(64) __attribute__((amdgpu_waves_per_eu(P2GWave
     const uint32_t nitems = *I.count;
     const int lane = threadIdx.x;
     for (uint32_t it = blockIdx.x; it < nitems; it += gridDim.x) {
+        // a fused batch step: scenes whose last step already scattered this P2G
+        if (BATCH && tab[I.scene[it]].prescat != 0u) continue;
         SceneDev S;
         ParticlesDev P;
         uint32_t pbase, koff;
@@ -1431,10 +1433,13 @@ __device__ __forceinline__ void scatter_direct(const SceneDev& S, float* grid, u
 #endif
 // 3 waves per SIMD: the kernel needs ~162 VGPRs (at 128, 4 waves, ~24
 // spill); its LDS (19.6 KB per 128-thread workgroup) would allow 4.
-template <int INTERP, int WG>
+// BATCH: the items span the scenes of a batch; each item's scene, its
+// second grid (ogrid, otouched) and its particle collision flag come from
+// the table, and the next sort keys carry the scene's key offset.
+template <int INTERP, int WG, bool BATCH>
 __global__ void __launch_bounds__(WG) __attribute__((amdgpu_waves_per_eu(MPM_G2P2G_WAVES)))
-k_g2p2g(SceneDev S, ParticlesDev P, ItemsDev I, float* __restrict__ ogrid, uint32_t* __restrict__ otouched,
-        int pcollide, uint32_t* __restrict__ next_keys) {
+k_g2p2g(SceneDev S0, ParticlesDev P0, ItemsDev I, const SceneEntry* __restrict__ tab, float* __restrict__ ogrid0,
+        uint32_t* __restrict__ otouched0, int pcollide0, uint32_t* __restrict__ next_keys) {
     constexpr int NW = WG / 64;  // waves: each has its own output tile
     constexpr int W = Stencil<INTERP>::W;
     constexpr int T = 4 + W - 1;
@@ -1446,23 +1451,44 @@ k_g2p2g(SceneDev S, ParticlesDev P, ItemsDev I, float* __restrict__ ogrid, uint3
     __shared__ float pstage[PW * WG];    // Fe, Jp, mu0, lam0, V0 of each lane's particle
     __shared__ float otile[NW * 4 * TN];  // next step's sums, one tile per wave
     const int wave = threadIdx.x >> 6, lane = threadIdx.x & 63;
-    const int64_t N = P.n;
     const uint32_t nitems = *I.count;
     float* const ot = otile + wave * (4 * TN);
     ItemGeom gnext{};
     if (blockIdx.x < nitems) {
-        gnext = item_geom(S, I, blockIdx.x, nitems, 0);
-        fetch_folded_tile<T, TR, WG>(S, gnext, stage);
+        SceneDev Sn;
+        ParticlesDev Pn;
+        uint32_t pbn, kon;
+        item_scene<BATCH>(S0, P0, I, tab, blockIdx.x, Sn, Pn, pbn, kon);
+        gnext = item_geom(Sn, I, blockIdx.x, nitems, kon);
+        fetch_folded_tile<T, TR, WG>(Sn, gnext, stage);
     }
     for (uint32_t it = blockIdx.x; it < nitems; it += gridDim.x) {
+        SceneDev S;
+        ParticlesDev P;
+        uint32_t pbase, koff;
+        item_scene<BATCH>(S0, P0, I, tab, it, S, P, pbase, koff);
+        float MPM_GLOBAL* ogrid = (float MPM_GLOBAL*)ogrid0;
+        uint32_t MPM_GLOBAL* otouched = (uint32_t MPM_GLOBAL*)otouched0;
+        int pcollide = pcollide0;
+        if constexpr (BATCH) {
+            const uint32_t sc = I.scene[it];
+            ogrid = tab[sc].grid2;
+            otouched = tab[sc].touched2;
+            pcollide = S.particle_collision;
+        }
+        const int64_t N = P.n;
         const ItemGeom g = gnext;
         if (!g.escape) commit_folded_tile<T, TR, WG>(S, g, stage, tile);
         for (int q = threadIdx.x; q < NW * 4 * TN; q += WG) otile[q] = 0.0f;
         __syncthreads();
         const uint32_t itn = it + gridDim.x;
         if (itn < nitems) {
-            gnext = item_geom(S, I, itn, nitems, 0);
-            fetch_folded_tile<T, TR, WG>(S, gnext, stage);
+            SceneDev Sn;
+            ParticlesDev Pn;
+            uint32_t pbn, kon;
+            item_scene<BATCH>(S0, P0, I, tab, itn, Sn, Pn, pbn, kon);
+            gnext = item_geom(Sn, I, itn, nitems, kon);
+            fetch_folded_tile<T, TR, WG>(Sn, gnext, stage);
         }
         // each wave takes every other 64-particle chunk of the item (no
         // barrier inside: a wave's trip count is its own, and every pass
@@ -1479,7 +1505,7 @@ k_g2p2g(SceneDev S, ParticlesDev P, ItemsDev I, float* __restrict__ ogrid, uint3
                 for (int c = 0; c < 3; ++c) { Cn[r][c] = 0.0f; vs[r][c] = 0.0f; }
             if (active) {
                 // ---- step n: gridToParticles (src/apic_mpm_solver3d.cu:326-388)
-                const uint32_t p = I.perm[i];
+                const uint32_t p = I.perm[i] - pbase;
                 kold = I.sorted_keys[i];
                 if (I.snap_k) {  // the sorted order, kept for the next step's incremental sort
                     I.snap_k[i] = kold;
@@ -1527,9 +1553,9 @@ k_g2p2g(SceneDev S, ParticlesDev P, ItemsDev I, float* __restrict__ ogrid, uint3
                 P.x[p] = xn[0];
                 P.x[N + p] = xn[1];
                 P.x[2 * N + p] = xn[2];
-                knew = cell_key(S, xn[0], xn[1], xn[2], Stencil<INTERP>::SHIFT, Stencil<INTERP>::W);
-                next_keys[p] = knew;
-                if (I.plan) record_move(I, knew != kold, knew, p, i);
+                knew = koff + cell_key(S, xn[0], xn[1], xn[2], Stencil<INTERP>::SHIFT, Stencil<INTERP>::W);
+                next_keys[pbase + p] = knew;
+                if (I.plan) record_move(I, knew != kold, knew, p, i);  // one scene only: pbase is 0
                 // updateDeformationGradient and the stress of the new state
                 float Fe[3][3];
                 asm volatile("s_waitcnt vmcnt(0)" ::: "memory");  // staged words have landed
@@ -1635,8 +1661,16 @@ k_g2p2g(SceneDev S, ParticlesDev P, ItemsDev I, float* __restrict__ ogrid, uint3
 void launch_g2p2g(const SceneDev& S, const ParticlesDev& P, const ItemsDev& I, int pcollide, float* ogrid,
                   uint32_t* otouched, uint32_t* next_keys, hipStream_t st) {
     if (I.n <= 0) return;
-    MPM_PERSISTENT((k_g2p2g<MPM_INTERP_QUADRATIC_BSPLINE, MPM_G2P2G_WG>), MPM_G2P2G_WG, 0, st, S, P, I, ogrid,
-                   otouched, pcollide, next_keys);
+    MPM_PERSISTENT((k_g2p2g<MPM_INTERP_QUADRATIC_BSPLINE, MPM_G2P2G_WG, false>), MPM_G2P2G_WG, 0, st, S, P, I,
+                   (const SceneEntry*)nullptr, ogrid, otouched, pcollide, next_keys);
+}
+void launch_g2p2g_batch(const SceneEntry* tab, const ItemsDev& I, int grid_wgs, uint32_t* next_keys,
+                        hipStream_t st) {
+    if (I.n <= 0) return;
+    const SceneDev S0{};
+    const ParticlesDev P0{};
+    MPM_PERSISTENT((k_g2p2g<MPM_INTERP_QUADRATIC_BSPLINE, MPM_G2P2G_WG, true>), MPM_G2P2G_WG, grid_wgs, st, S0, P0,
+                   I, tab, (float*)nullptr, (uint32_t*)nullptr, 0, next_keys);
 }
 
 template <int INTERP, int APIC, int SRC, bool BATCH>
@@ -1794,9 +1828,21 @@ __global__ void __launch_bounds__(256) k_count_active(SceneDev S, int64_t nsb, u
 // Batched clear over the storage blocks of every scene: global block index
 // gb lies in scene s when block_start[s] <= gb < block_start[s + 1].  Each
 // scene's active-node count goes to its own counter.
+// mode 0: every scene's grid, its active nodes counted (the two-kernel
+// step); mode 1: the grid of the scenes without pre-scattered sums, counted
+// (the fused step's first step for them); mode 2: every scene's second
+// grid, counted for the scenes whose last step was fused (it is the grid
+// that step read) and not for the others (sums dropped when they left the
+// fused state).
+__device__ __forceinline__ float MPM_GLOBAL* clear_grid_of(const SceneEntry& e, int mode) {
+    return mode == 2 ? e.grid2 : e.S.grid;
+}
+__device__ __forceinline__ uint32_t MPM_GLOBAL* clear_touched_of(const SceneEntry& e, int mode) {
+    return mode == 2 ? e.touched2 : e.S.touched;
+}
 __global__ void __launch_bounds__(256) k_clear_batch(const SceneEntry* __restrict__ tab,
                                                      const int64_t* __restrict__ block_start, int nscenes,
-                                                     int nfields) {
+                                                     int nfields, int mode) {
     const int lane = threadIdx.x & 63;
     const int64_t gb = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
     const int64_t total = block_start[nscenes];
@@ -1810,7 +1856,8 @@ __global__ void __launch_bounds__(256) k_clear_batch(const SceneEntry* __restric
             else hi = mid - 1;
         }
         sc = lo;
-        t = tab[sc].S.touched[gb - block_start[sc]] != 0u;
+        const bool take = mode != 1 || tab[sc].prescat == 0u;
+        t = take && clear_touched_of(tab[sc], mode)[gb - block_start[sc]] != 0u;
     }
     uint64_t todo = __ballot(t);
     // Blocks are in scene order, so a wave's touched blocks form a few runs
@@ -1829,7 +1876,9 @@ __global__ void __launch_bounds__(256) k_clear_batch(const SceneEntry* __restric
             run = 0;
         }
         const int64_t b = (gb - lane + l) - block_start[s];
-        run += clear_block(tab[s].S.grid, tab[s].S.plane, tab[s].S.touched, b, nfields, lane);
+        const unsigned long long c = clear_block(clear_grid_of(tab[s], mode), tab[s].S.plane,
+                                                 clear_touched_of(tab[s], mode), b, nfields, lane);
+        if (mode != 2 || tab[s].prescat != 0u) run += c;
     }
     if (lane == 0 && run) atomicAdd(tab[cur].active, run);
 }
@@ -1846,10 +1895,28 @@ void launch_clear(const SceneDev& S, int64_t nsb, int nfields, unsigned long lon
     hipLaunchKernelGGL(k_clear, dim3((unsigned)((nsb + 255) / 256)), dim3(256), 0, st, S, nsb, nfields, active);
 }
 void launch_clear_batch(const SceneEntry* tab, const int64_t* block_start, int nscenes, int64_t total_blocks,
-                        int nfields, hipStream_t st) {
+                        int nfields, hipStream_t st, int mode) {
     if (total_blocks <= 0 || nscenes <= 0) return;
     hipLaunchKernelGGL(k_clear_batch, dim3((unsigned)((total_blocks + 255) / 256)), dim3(256), 0, st, tab,
-                       block_start, nscenes, nfields);
+                       block_start, nscenes, nfields, mode);
+}
+
+// After a fused batch step: every scene's second grid (just scattered)
+// becomes its grid, and the grid just read its second grid.
+__global__ void k_swap_grids(SceneEntry* __restrict__ tab, int nscenes) {
+    const int i = (int)(blockIdx.x * blockDim.x + threadIdx.x);
+    if (i >= nscenes) return;
+    float MPM_GLOBAL* g = tab[i].S.grid;
+    uint32_t MPM_GLOBAL* t = tab[i].S.touched;
+    tab[i].S.grid = tab[i].grid2;
+    tab[i].S.touched = tab[i].touched2;
+    tab[i].grid2 = g;
+    tab[i].touched2 = t;
+    tab[i].prescat = 1u;
+}
+void launch_swap_grids(SceneEntry* tab, int nscenes, hipStream_t st) {
+    if (nscenes <= 0) return;
+    hipLaunchKernelGGL(k_swap_grids, dim3((unsigned)((nscenes + 63) / 64)), dim3(64), 0, st, tab, nscenes);
 }
 void launch_count_active(const SceneDev& S, int64_t nsb, unsigned long long* active, hipStream_t st) {
     if (nsb <= 0) return;

@@ -81,13 +81,22 @@ void launch_g2p(const SceneDev& S, const ParticlesDev& P, const ItemsDev& I, int
 // scatters into ogrid / otouched (a second grid of the same layout, clear).
 void launch_g2p2g(const SceneDev& S, const ParticlesDev& P, const ItemsDev& I, int pcollide, float* ogrid,
                   uint32_t* otouched, uint32_t* next_keys, hipStream_t st);
+// The same over a batch's items (scenes from the table, each scattering into
+// its entry's grid2); next_keys are the global keys at the global slots.
+void launch_g2p2g_batch(const SceneEntry* tab, const ItemsDev& I, int grid_wgs, uint32_t* next_keys,
+                        hipStream_t st);
+// After a fused batch step: each entry's grid2 becomes its grid and the
+// reverse; every entry is marked pre-scattered.
+void launch_swap_grids(SceneEntry* tab, int nscenes, hipStream_t st);
 
 void launch_clear(const SceneDev& S, int64_t storage_blocks, int nfields, unsigned long long* active_nodes,
                   hipStream_t st);
 // Clear of every scene's touched blocks; block_start[nscenes + 1] is the
 // prefix sum of the scenes' storage block counts (on the device).
+// mode 0: every grid (counted); 1: the grids of entries not pre-scattered
+// (counted); 2: every grid2 (counted where the entry is pre-scattered).
 void launch_clear_batch(const SceneEntry* tab, const int64_t* block_start, int nscenes, int64_t total_blocks,
-                        int nfields, hipStream_t st);
+                        int nfields, hipStream_t st, int mode = 0);
 void launch_count_active(const SceneDev& S, int64_t storage_blocks, unsigned long long* active_nodes,
                          hipStream_t st);
 // op: 0 gravity, 1 update velocities, 2 grid collision (touched blocks only)

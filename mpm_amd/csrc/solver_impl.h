@@ -154,6 +154,11 @@ struct Solver {
     decltype(SceneDev::grid) grid2 = nullptr;
     decltype(SceneDev::touched) touched2 = nullptr;
     decltype(SceneDev::grid) graph_grid = nullptr;  // the grid buffer the captured step uses
+    // member of a batch that fuses its transfer: the batch table's grid of
+    // this scene holds the P2G of its current particles (the table's grid
+    // and grid2 may be swapped relative to S.grid / grid2 here, by the
+    // batch's parity)
+    bool bprescat = false;
 };
 
 // Below this many particles the fused step sorts in full every step: the
@@ -183,6 +188,12 @@ int finish(Solver& s);
 void batch_forget(struct Batch* b, Solver* s);
 // batch.hip: a member's settings changed; the batch recomputes its layout.
 void batch_invalidate(struct Batch* b);
+// batch.hip: before a member's own call touches its grid or particles, the
+// member leaves the batch's fused state (its grids as a two-kernel step
+// leaves them).
+void batch_unfuse_member(struct Batch* b, Solver& s);
+// The second grid buffer (all planes, cleared) of the fused transfer.
+int ensure_grid2_buffers(Solver& s);
 // Planes of the grid that P2G writes in the fused step.
 inline bool fused_folds(const Solver& s) { return s.apic && !s.record_grid; }
 // Before anything but a fused step touches the particles or the grid: the
@@ -190,6 +201,7 @@ inline bool fused_folds(const Solver& s) { return s.apic && !s.record_grid; }
 // leaves it) and the next step's pre-scattered sums wait in grid2 to be
 // cleared.
 inline void unfuse(Solver& s) {
+    if (s.batch) batch_unfuse_member(s.batch, s);
     if (!s.prescat) return;
     std::swap(s.S.grid, s.grid2);
     std::swap(s.S.touched, s.touched2);

@@ -120,6 +120,7 @@ class MpmTune(C.Structure):
         ("workgroups", C.c_int32),
         ("bucket", C.c_int32),
         ("step_ms", C.c_double),
+        ("fuse_transfer", C.c_int32),
     ]
 
 

@@ -15,7 +15,7 @@ import pytest
 
 from mpm_amd import native as N
 from mpm_amd import scenes
-from mpm_amd.batch import Batch
+from mpm_amd.batch import Batch, Tune
 from mpm_amd.solver import Solver
 from oracle import pyoracle
 from tests.helpers import particle_errors_apic, rel_err, report
@@ -113,3 +113,82 @@ def test_fused_state_transitions():
     _compare(a, b, sc.desc.stride, "transitions fused vs two-pass", pos_tol=1e-3)
     a.close()
     b.close()
+
+
+# ------------------------------------------------------------ batched fusion
+def _fuse(b, on):
+    b.set_tune(Tune(256, 128, 0, fuse_transfer=1 if on else 0))
+
+
+def _batch_runs(scs):
+    """The same scenes in two batches: fused transfer and two-pass."""
+    runs = []
+    for fuse in (True, False):
+        solvers = [Solver(s.desc, s.particles) for s in scs]
+        b = Batch(N.QUADRATIC_BSPLINE, N.APIC)
+        _fuse(b, fuse)
+        for s in solvers:
+            b.add(s)
+        runs.append((b, solvers))
+    return runs
+
+
+def _close_runs(runs):
+    for b, solvers in runs:
+        b.close()
+        for s in solvers:
+            s.close()
+
+
+def test_batched_fused_matches_oracle():
+    """Three steps of a fused batch (the first scatters every member, the
+    next two run on pre-scattered grids); the largest member against the
+    oracle, read only at the end (no member call in between)."""
+    rng = scenes.Rng(21)
+    scs = [scenes.batch_scene(rng, i, n_lo=3000, n_hi=20000) for i in range(5)]
+    runs = _batch_runs(scs)
+    b, solvers = runs[0]
+    b.step(3)
+    b.synchronize()
+    k = int(np.argmax([s.particles.count for s in scs]))
+    ref = pyoracle.OracleSolver(scs[k].desc, scs[k].particles, 32)
+    ref.step(3)
+    e = particle_errors_apic(solvers[k].read_particles(), ref.read_particles(), scs[k].desc.stride)
+    report(f"batched fused, member {k} (N={scs[k].particles.count}) vs oracle, 3 steps", e)
+    assert all(v <= TOL for v in e.values()), e
+    ref.close()
+    _close_runs(runs)
+
+
+def test_batched_fused_matches_two_pass_with_transitions():
+    """Fused and two-pass batches through the transitions: graph replays, a
+    member's own call (a state write), a member leaving and joining, and
+    fusion switched off and on; every member equal after 30 steps."""
+    rng = scenes.Rng(22)
+    scs = [scenes.batch_scene(rng, i, n_lo=2000, n_hi=25000) for i in range(6)]
+    runs = _batch_runs(scs)
+    for b, solvers in runs:
+        b.step(4)
+        b.step_graph(4)
+        st = solvers[2].read_particles()  # a member's own calls mid-run
+        st.velocity[::4] *= 0.5
+        solvers[2].write_particles(st)
+        b.step(3)
+        b.remove(solvers[4])  # leaves, steps alone, joins again
+        solvers[4].step(2)
+        b.step(2)
+        b.add(solvers[4])
+        b.step_graph(5)
+        fused_run = b is runs[0][0]
+        _fuse(b, False)  # off and on again (only the first batch fuses)
+        b.step(3)
+        _fuse(b, fused_run)
+        b.step(4)
+        solvers[4].step(2)  # the member's own step while a member
+        b.step_graph(3)
+        b.synchronize()
+    (bf, fused), (bp, plain) = runs
+    for k, (a, c) in enumerate(zip(fused, plain)):
+        assert a.stats()["steps"] == c.stats()["steps"]
+        _compare(a, c, scs[k].desc.stride, f"batched fused vs two-pass, member {k} (N={a.count})", pos_tol=1e-3)
+    _close_runs(runs)
