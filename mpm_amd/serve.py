@@ -36,6 +36,8 @@ from __future__ import annotations
 
 import time
 from collections import deque
+from concurrent.futures import ThreadPoolExecutor
+from concurrent.futures import wait as wait_futures
 from dataclasses import dataclass, field
 
 import numpy as np
@@ -164,8 +166,15 @@ class Server:
     def __init__(self, budget_bytes: float = 200e9, tick_steps: int = 16, max_scenes: int | None = None,
                  backend=None, clock=time.perf_counter, keep_results: bool = False, sleep=time.sleep,
                  cfl_limit: float = 1.0, evict_escaped: bool = True, builds_per_tick: int = 8,
-                 progress_s: float = 0.0, log=print):
+                 progress_s: float = 0.0, log=print, build_threads: int = 1):
         self.budget = int(budget_bytes)
+        # scenes are built on build_threads background threads (0: inline,
+        # between ticks): the library's calls on different handles may
+        # overlap and ctypes releases the GIL during them, so the main thread
+        # keeps the batches stepping while scenes upload and initialise
+        self.build_threads = max(0, int(build_threads))
+        self._pool = None
+        self.building: list = []  # (future, request, record) in submission order
         # a progress line (arrived, resident, done, failed, particle-updates/s) every progress_s seconds
         self.progress_s = float(progress_s)
         self.log = log
@@ -197,7 +206,7 @@ class Server:
         """Device bytes committed: measured bytes of the joined handles and
         of the batches, estimates of the scenes built but not yet joined."""
         return (sum(a.bytes for a in self.active.values()) + sum(self.batch_bytes.values()) +
-                sum(a.rec.est_bytes for a in self.staged))
+                sum(a.rec.est_bytes for a in self.staged) + sum(rec.est_bytes for _, _, rec in self.building))
 
     def _note_peak(self):
         self.peak_bytes = max(self.peak_bytes, self.used)
@@ -210,30 +219,64 @@ class Server:
 
     # -- admission
     def _fits(self, req: ServeRequest, rec: RequestRecord) -> bool:
-        if self.max_scenes is not None and len(self.active) + len(self.staged) >= self.max_scenes:
+        resident = len(self.active) + len(self.staged) + len(self.building)
+        if self.max_scenes is not None and resident >= self.max_scenes:
             return False
         key = (req.desc.interp, req.desc.transfer)
         keys = sum(a.span for a in list(self.active.values()) + self.staged if a.key == key)
+        keys += sum(key_span(r.desc) for _, r, _ in self.building if (r.desc.interp, r.desc.transfer) == key)
         if keys + key_span(req.desc) > (1 << 32):
             return False
         return self.used + rec.est_bytes <= self.budget
 
-    def _build(self, req: ServeRequest, rec: RequestRecord, t0: float):
-        """Creates the scene on the device (its own stream): safe while
-        batch steps are in flight.  A library error rejects the request."""
-        try:
-            s = self.backend.create_solver(req.desc, req.particles)
-        except N.MpmError as e:
-            rec.error = f"build failed: {e}"
-            rec.admitted = rec.finished = self.clock() - t0
-            return
-        rec.admitted = self.clock() - t0
-        a = _Active(s, int(req.steps), rec, (req.desc.interp, req.desc.transfer), key_span(req.desc))
+    def _create(self, req: ServeRequest):
+        """The scene on the device (its own stream; safe while batch steps
+        are in flight, and on a builder thread) and its measured bytes."""
+        s = self.backend.create_solver(req.desc, req.particles)
         stats = getattr(s, "stats", None)
+        return s, (int(stats()["device_bytes"]) if stats is not None else None)
+
+    def _stage(self, req: ServeRequest, rec: RequestRecord, s, nbytes):
+        a = _Active(s, int(req.steps), rec, (req.desc.interp, req.desc.transfer), key_span(req.desc))
         # measured where the backend reports it; else the estimate stands
-        a.bytes = int(stats()["device_bytes"]) if stats is not None else rec.est_bytes
+        a.bytes = nbytes if nbytes is not None else rec.est_bytes
         self.staged.append(a)
         self._note_peak()
+
+    def _build(self, req: ServeRequest, rec: RequestRecord, t0: float):
+        """Builds the scene, inline or on the builder thread.  A library
+        error rejects the request."""
+        rec.admitted = self.clock() - t0
+        if self.build_threads > 0:
+            if self._pool is None:
+                self._pool = ThreadPoolExecutor(self.build_threads, thread_name_prefix="mpm-build")
+            self.building.append((self._pool.submit(self._create, req), req, rec))
+            self._note_peak()
+            return
+        try:
+            s, nbytes = self._create(req)
+        except N.MpmError as e:
+            rec.error = f"build failed: {e}"
+            rec.finished = self.clock() - t0
+            return
+        self._stage(req, rec, s, nbytes)
+
+    def _collect(self, t0: float, wait: bool = False):
+        """Scenes finished on the builder thread are staged (in submission
+        order); wait: block until the first one is done."""
+        if wait and self.building:
+            wait_futures([self.building[0][0]], timeout=0.05)
+        while self.building and self.building[0][0].done():
+            f, req, rec = self.building.pop(0)
+            e = f.exception()
+            if isinstance(e, N.MpmError):
+                rec.error = f"build failed: {e}"
+                rec.finished = self.clock() - t0
+                continue
+            if e is not None:
+                raise e
+            s, nbytes = f.result()
+            self._stage(req, rec, s, nbytes)
 
     def _join(self, a: _Active, t0: float):
         """Adds a built scene to its batch (between ticks)."""
@@ -354,19 +397,20 @@ class Server:
         def admissions():
             built = 0
             while queue and self._fits(queue[0][0], queue[0][1]):
-                if built >= self.builds_per_tick:  # the next tick runs what is built
+                if self.build_threads == 0 and built >= self.builds_per_tick:  # the next tick runs what is built
                     break
                 r, rec = queue.popleft()
                 self._build(r, rec, t0)
                 built += 1
 
         def joins():
+            self._collect(t0)
             staged, self.staged = self.staged, []
             for a in staged:
                 self._join(a, t0)
 
         next_report = self.progress_s
-        while pending or queue or self.staged or self.active:
+        while pending or queue or self.staged or self.active or self.building:
             if self.progress_s > 0 and self.clock() - t0 >= next_report:
                 next_report += self.progress_s
                 self._progress(t0)
@@ -374,7 +418,9 @@ class Server:
             admissions()
             joins()
             if not self.active:
-                if pending and not queue and not self.staged:
+                if self.building:
+                    self._collect(t0, wait=True)
+                elif pending and not queue and not self.staged:
                     self.sleep(max(0.0, min(0.05, pending[0].arrival - (self.clock() - t0))))
                 continue
             # one tick: enqueue k steps on every batch, admit meanwhile, then retire
@@ -437,6 +483,15 @@ class Server:
             failed=sum(1 for r in self.records if r.error), records=self.records)
 
     def close(self):
+        for f, _, _ in self.building:  # scenes still being built are freed once built
+            try:
+                f.result()[0].close()
+            except Exception:
+                pass
+        self.building = []
+        if self._pool is not None:
+            self._pool.shutdown(wait=True)
+            self._pool = None
         for a in list(self.active.values()) + self.staged:
             try:
                 a.solver.close()

@@ -3,6 +3,8 @@ device backend and a fake clock: admission by HBM budget in arrival order,
 joins while other scenes run, exact step counts, budget accounting."""
 from __future__ import annotations
 
+import time
+
 import numpy as np
 import pytest
 
@@ -86,6 +88,7 @@ def _req(rid, arrival, n, steps, transfer=N.APIC, res=32):
 def _server(budget, **kw):
     c = Clock()
     be = FakeBackend(c)
+    kw.setdefault("build_threads", 0)  # inline builds: the fake clock stays deterministic
     return Server(budget_bytes=budget, backend=be, clock=c, sleep=c.sleep, keep_results=True, **kw), be, c
 
 
@@ -300,6 +303,7 @@ class FailingBackend(FakeBackend):
 def _failing(budget=1e12, **kw):
     c = Clock()
     be = FailingBackend(c, budget, **{k: kw.pop(k) for k in ("nan_after", "fail_keys", "fail_build") if k in kw})
+    kw.setdefault("build_threads", 0)
     return Server(budget_bytes=budget, backend=be, clock=c, sleep=c.sleep, keep_results=True, **kw), be
 
 
@@ -373,3 +377,53 @@ def test_budget_holds_real_bytes_under_churn():
     assert rep.completed == 40 and rep.failed == 0
     assert be.meter.peak <= budget and rep.peak_bytes <= budget
     assert srv.used == 0
+
+
+class SlowBuildBackend:
+    """Real-time fake device: a build takes 20 ms (off the main thread when
+    the server has a builder), a batch step 2 ms."""
+
+    def __init__(self):
+        self.log, self.solvers, self.threads = [], [], set()
+
+    def create_solver(self, desc, particles):
+        import threading
+
+        self.threads.add(threading.get_ident())
+        time.sleep(0.02)
+        s = FakeSolver(desc, particles, self.log)
+        self.solvers.append(s)
+        return s
+
+    def create_batch(self, interp, transfer):
+        b = FakeBatch((interp, transfer), Clock(), self.log)
+        step = b.step
+
+        def timed_step(k):
+            time.sleep(0.002 * k)
+            step(k)
+
+        b.step = timed_step
+        return b
+
+
+def test_builder_thread_overlaps_steps():
+    """Builds run on the builder thread while the main thread steps: the
+    requests complete with exact step counts, in less time than building
+    and stepping one after the other would take."""
+    import threading
+
+    reqs = [_req(i, 0.0, 2000, 40) for i in range(12)]
+    be = SlowBuildBackend()
+    srv = Server(budget_bytes=1e12, backend=be, keep_results=True, tick_steps=16, build_threads=1)
+    t0 = time.perf_counter()
+    rep = srv.run(reqs)
+    wall = time.perf_counter() - t0
+    srv.close()
+    assert rep.completed == 12 and rep.failed == 0
+    for r in rep.records:
+        assert r.result == {"steps": r.steps}
+    assert threading.get_ident() not in be.threads  # every build ran off the main thread
+    assert all(s.closed for s in be.solvers)
+    print(f"builder thread: 12 builds of 20 ms with 40 steps each in {wall:.3f} s", flush=True)
+    assert wall < 12 * 0.02 + 0.25  # the stepping overlapped the builds
