@@ -1381,18 +1381,22 @@ MPM_API int mpm_positions4_wait(void* handle) {
 // ---- density raster (writeToOpenVDB, src/mpm_solver3d.cu:510-572)
 
 // The raster's one-plane grid: the scene's geometry with the density
-// buffers in place of the simulation grid.
+// buffers in place of the simulation grid -- or, after a fused step, the
+// simulation grid itself: its first plane holds the next step's P2G mass,
+// sum m w over the current positions, which is the raster's sum.
 static SceneDev density_scene(const Solver& s) {
     SceneDev D = s.S;
+    if (s.prescat) return D;
     D.grid = decltype(D.grid)(s.dens);
     D.touched = decltype(D.touched)(s.dens_touched);
     return D;
 }
 
 // Mass-only P2G of the current positions into the density plane (skipped
-// when the particles have not changed since the last raster).
+// when the particles have not changed since the last raster, and after a
+// fused step, whose grid already holds the sums).
 static int run_density(Solver& s) {
-    if (s.dens && s.dens_version == s.version) return MPM_OK;
+    if (s.dens && (s.prescat || s.dens_version == s.version)) return MPM_OK;
     ProfileHook h = hook(s);
     Scope sc(h, "density");
     if (!s.dens) {
@@ -1414,6 +1418,7 @@ static int run_density(Solver& s) {
         s.dens_temp = (uint32_t*)p[3];
         s.dens_dirty = false;
     }
+    if (s.prescat) return MPM_OK;  // the buffers above serve the compaction
     const SceneDev D = density_scene(s);
     if (s.dens_dirty) launch_clear(D, s.nsb, 1, nullptr, s.st);
     if (s.N > 0) {
@@ -1453,7 +1458,8 @@ static int density_blocks(Solver& s, std::vector<int32_t>* origins, std::vector<
                           float* h_vals, int64_t max_blocks, int64_t* count) {
     int rc = run_density(s);
     if (rc) return rc;
-    launch_touched_flags(s.dens_touched, s.nsb, s.dens_rank, s.st);
+    const SceneDev D = density_scene(s);
+    launch_touched_flags((const uint32_t*)D.touched, s.nsb, s.dens_rank, s.st);
     exclusive_scan(s.dens_rank, s.nsb, s.dens_temp, s.dens_rank + s.nsb, s.st);
     uint32_t total = 0;
     HIP_TRY(hipMemcpyAsync(&total, s.dens_rank + s.nsb, 4, hipMemcpyDeviceToHost, s.st));
@@ -1470,7 +1476,7 @@ static int density_blocks(Solver& s, std::vector<int32_t>* origins, std::vector<
     float* d_vals = nullptr;
     if ((rc = dalloc(s, (void**)&d_orig, (size_t)total * 12))) return rc;
     if ((rc = dalloc(s, (void**)&d_vals, (size_t)total * 256))) { dfree(s, d_orig); return rc; }
-    launch_density_blocks(density_scene(s), s.dens_touched, s.dens_rank, s.nsb, d_orig, d_vals, total, s.st);
+    launch_density_blocks(D, (const uint32_t*)D.touched, s.dens_rank, s.nsb, d_orig, d_vals, total, s.st);
     hipError_t e = hipSuccess;
     if (h_orig) e = hipMemcpyAsync(h_orig, d_orig, (size_t)total * 12, hipMemcpyDeviceToHost, s.st);
     if (e == hipSuccess && h_vals) e = hipMemcpyAsync(h_vals, d_vals, (size_t)total * 256, hipMemcpyDeviceToHost, s.st);

@@ -91,13 +91,14 @@ struct G2PWaves {
 };
 
 // Minimum waves per SIMD for P2G.  The folded (4-value) and mass-only
-// P2G of the linear and quadratic stencils fit 128 VGPRs without spills, so
-// they run 4 waves per SIMD instead of 3 (measured: APIC P2G 0.674 ->
-// 0.616 ms on the 8.4M slab).  The 7-value P2G and the cubic stencil (64
-// nodes; 48 VGPRs of spill under the same cap) keep the compiler's
-// allocation.
+// P2G of the linear and quadratic stencils fit 96 VGPRs without spills
+// (the folded P2G since its node values are regrouped as yz R(a) + Nx[a]
+// Syz(b,c)), so they run 5 waves per SIMD (measured: the two-kernel slab
+// step 1.412 -> 1.398 ms and the 256-scene APIC batch 2.354 -> 2.313 ms
+// from 4 waves; 3 -> 4 waves had taken APIC P2G from 0.674 to 0.616 ms).
+// The 7-value P2G and the cubic stencil keep the compiler's allocation.
 #ifndef MPM_P2G_WAVES
-#define MPM_P2G_WAVES 4
+#define MPM_P2G_WAVES 5
 #endif
 template <int INTERP, int MODE>
 struct P2GWaves {
@@ -741,6 +742,48 @@ __global__ void __launch_bounds__(64) __attribute__((amdgpu_waves_per_eu(P2GWave
                     dxn[2][j] = (S.oz + S.h * (float)(bz + j)) - x2;
                 }
             }
+            if constexpr (MODE == 2) {
+                // the folded grid: momentum plus force of node (a, b, c) as
+                // yz R(a) + Nx[a] Syz(b,c) (the regrouping of the fused
+                // kernel's P2G half), (y, z) pairs outermost
+                const float vk3[3] = {v0, v1, v2};
+                float ma[W], R[W][3];
+#pragma unroll
+                for (int a = 0; a < W; ++a) {
+                    ma[a] = m * Nx[a];
+#pragma unroll
+                    for (int k = 0; k < 3; ++k) R[a][k] = fmaf(ma[a], vk3[k] + Cm[k][0] * dxn[0][a], vs[k][0] * dNx[a]);
+                }
+#pragma unroll
+                for (int b = 0; b < W; ++b)
+#pragma unroll
+                    for (int c = 0; c < W; ++c) {
+                        const float yzc = Ny[b] * Nz[c], dyzc = dNy[b] * Nz[c], ydzc = Ny[b] * dNz[c];
+                        const float myz = m * yzc;
+                        float Syz[3];
+#pragma unroll
+                        for (int k = 0; k < 3; ++k)
+                            Syz[k] = fmaf(myz, Cm[k][1] * dxn[1][b] + Cm[k][2] * dxn[2][c],
+                                          vs[k][1] * dyzc + vs[k][2] * ydzc);
+#pragma unroll
+                        for (int a = 0; a < W; ++a) {
+                            float v[NF];
+                            v[0] = ma[a] * yzc;
+#pragma unroll
+                            for (int k = 0; k < 3; ++k) v[1 + k] = fmaf(yzc, R[a][k], Nx[a] * Syz[k]);
+                            seg_scan<NF>(v, sp);
+                            const int q = (c * T + b) * T + a;
+                            if (writer) {
+                                float o[NF];
+#pragma unroll
+                                for (int f = 0; f < NF; ++f) o[f] = tq[f * TN + q];
+#pragma unroll
+                                for (int f = 0; f < NF; ++f) tq[f * TN + q] = o[f] + v[f];
+                            }
+                        }
+                    }
+                continue;
+            }
             float yz[W][W], dyz[W][W], ydz[W][W];
 #pragma unroll
             for (int b = 0; b < W; ++b)
@@ -1303,10 +1346,14 @@ __device__ __forceinline__ void apic_gather(const float* tq, const float (&Nx)[W
 // (particlesToGrid, src/apic_mpm_solver3d.cu:245-304, with p + dt f summed
 // per node): mass m w and m w (v + C (x_i - x_p)) + vs grad w, vs = dt tau.
 // Tensor-product factors, (y, z) pairs outermost so that few of them are
-// live at once:
-//   mass     m w                   = (m Nx[a]) yz
-//   momentum m w (v + C (x_i - x_p)) = mip (cx(a) + cyz(b,c))
-//   force    vs grad w             = dNx[a] (vs_0 yz) + Nx[a] (vs_1 dyz + vs_2 ydz)
+// live at once.  With mip = m Nx[a] yz, cx(a) = v + C_0 dx(a) and
+// cyz(b,c) = C_1 dy(b) + C_2 dz(c), the momentum plus force of a node,
+//   mip (cx(a) + cyz(b,c)) + vs_0 dNx[a] yz + vs_1 Nx[a] dyz + vs_2 Nx[a] ydz,
+// regroups by the node's x factor into
+//   yz R(a) + Nx[a] Syz(b,c),
+//   R(a)     = m Nx[a] cx(a) + vs_0 dNx[a]        (per particle and a)
+//   Syz(b,c) = m yz cyz(b,c) + vs_1 dyz + vs_2 ydz  (per (b, c)),
+// two operations per component and node; the mass is m Nx[a] yz.
 // Each node's four values are summed over the lanes of the particle's cell
 // (segmented scan) and the cell's last lane adds them to the tile at tq
 // (the stencil's first node).  The read-add-write of the tile is pipelined
@@ -1323,12 +1370,12 @@ __device__ __forceinline__ void folded_cell_sums(float* tq, const SegPlan& sp, f
                                                  const float (&dxn)[3][W], const float (&Nx)[W],
                                                  const float (&Ny)[W], const float (&Nz)[W], const float (&dNx)[W],
                                                  const float (&dNy)[W], const float (&dNz)[W]) {
-    float ma[W], cx[W][3];
+    float ma[W], R[W][3];
 #pragma unroll
     for (int a = 0; a < W; ++a) {
         ma[a] = m * Nx[a];
 #pragma unroll
-        for (int k = 0; k < 3; ++k) cx[a][k] = vel[k] + Cm[k][0] * dxn[0][a];
+        for (int k = 0; k < 3; ++k) R[a][k] = fmaf(ma[a], vel[k] + Cm[k][0] * dxn[0][a], vs[k][0] * dNx[a]);
     }
     float pv[4], po[4];  // the pending node: its sums and the tile values read for it
     int pq = 0;
@@ -1337,20 +1384,17 @@ __device__ __forceinline__ void folded_cell_sums(float* tq, const SegPlan& sp, f
 #pragma unroll
         for (int c = 0; c < W; ++c) {
             const float yz = Ny[b] * Nz[c], dyz = dNy[b] * Nz[c], ydz = Ny[b] * dNz[c];
-            float cyz[3], g0[3], g12[3];
+            const float myz = m * yz;
+            float Syz[3];
 #pragma unroll
-            for (int k = 0; k < 3; ++k) {
-                cyz[k] = Cm[k][1] * dxn[1][b] + Cm[k][2] * dxn[2][c];
-                g0[k] = vs[k][0] * yz;
-                g12[k] = vs[k][1] * dyz + vs[k][2] * ydz;
-            }
+            for (int k = 0; k < 3; ++k)
+                Syz[k] = fmaf(myz, Cm[k][1] * dxn[1][b] + Cm[k][2] * dxn[2][c], vs[k][1] * dyz + vs[k][2] * ydz);
 #pragma unroll
             for (int a = 0; a < W; ++a) {
-                const float mip = ma[a] * yz;
                 float v[4];
-                v[0] = mip;
+                v[0] = ma[a] * yz;
 #pragma unroll
-                for (int k = 0; k < 3; ++k) v[1 + k] = fmaf(mip, cx[a][k] + cyz[k], dNx[a] * g0[k] + Nx[a] * g12[k]);
+                for (int k = 0; k < 3; ++k) v[1 + k] = fmaf(yz, R[a][k], Nx[a] * Syz[k]);
                 seg_scan<4>(v, sp);
                 const int q = (c * T + b) * T + a;
                 const bool first = b == 0 && c == 0 && a == 0;

@@ -2,6 +2,11 @@
 src/mpm_solver3d.cu:510-572) against the oracle's restatement of the
 reference's host raster, on the same positions.
 
+After a fused step (APIC, quadratic) the raster reads the grid's mass
+plane, which the fused kernel has already summed for the next step; the
+other cases run the raster's own sort and mass-only P2G.  The same bar
+holds for both.
+
 Bar: normwise relative error <= 1e-5 (float atomics reorder the FP32 sums);
 the block copy, the dense copy and the file hold the same bits; every node
 the oracle gives a nonzero density lies in a returned block.
@@ -73,6 +78,21 @@ def test_density_clipped_stencils_and_escapes():
     report("density near walls", e)
     assert e["density"] <= TOL
     _check_blocks(s, dense, ref)
+
+
+def test_density_after_fused_step_equals_standalone_raster():
+    sc = scenes.snow_cube(seed=24, n=8000, res=32, transfer=N.APIC, velocity=(0.4, -0.6, 0.2))
+    a = Solver(sc.desc, sc.particles)  # fused steps: the raster is the grid's mass plane
+    a.step(4)
+    da = a.density()
+    b = Solver(sc.desc, sc.particles)
+    b.set_transfer_fusion(False)  # two-kernel steps: the raster's own P2G
+    b.step(4)
+    db = b.density()
+    e = {"density": rel_err(da, db)}
+    report("raster after fused steps vs standalone raster", e)
+    assert e["density"] <= TOL
+    assert a.density_blocks().count == b.density_blocks().count
 
 
 def test_density_cached_and_refreshed():
