@@ -131,6 +131,8 @@ def density_bench(solver, sc, frames=5, cpu=True):
     mass-only P2G into its own plane), wall time of the block readback, and
     the reference's host raster (the oracle's restatement) on a sample.
     Algorithmic bytes: x and m per particle (16 B) + 4 B per nonzero node."""
+    from mpm_amd import native as N
+
     n = sc.particles.count
     # the standalone raster: after a fused step the grid already holds the
     # mass sums of the current positions and the raster reads them, so the
@@ -174,8 +176,12 @@ def density_bench(solver, sc, frames=5, cpu=True):
         out["cpu_host_raster"] = {"particles_per_s": sel.size / t, "cores": 1, "kind": "port",
                                   "sample": f"serial host raster of {sel.size} particles (slab sub-box x,z<0.3), "
                                             f"{t:.2f} s"}
+    fused = sc.desc.transfer == N.APIC and sc.desc.interp == N.QUADRATIC_BSPLINE
     log(f"density raster: {ms:.3f} ms/frame ({out['particles_per_s']:.3e} particles/s), {blk.count} blocks, "
-        f"block readback {read_ms:.1f} ms; after a fused step no raster pass, block readback {read_fused_ms:.1f} ms")
+        f"block readback {read_ms:.1f} ms" +
+        (f"; after a fused step no raster pass, block readback {read_fused_ms:.1f} ms" if fused else ""))
+    if not fused:
+        out.pop("after_fused_step")
     return out
 
 
@@ -356,7 +362,7 @@ def batch_bench(count, steps, warmup, seed=3):
         "batched": out["eager"], "batched_graph": out["graph"],
         "speedup_vs_plain": best["value"] / plain,
         "tune": {"item_max": tune.item_max, "g2p_threads": tune.g2p_threads, "workgroups": tune.workgroups,
-                 "bucket": tune.bucket},
+                 "fuse_transfer": tune.fuse_transfer, "bucket": tune.bucket},
         "work_items": info["work_items"], "key_bits": info["key_bits"],
     }
 
