@@ -57,15 +57,22 @@ __global__ void __launch_bounds__(kSortWG) k_radix_hist(const uint32_t* __restri
         for (int i = tid; i < kRadix; i += kSortWG) h[i] = 0;
         __syncthreads();
         const int64_t base = tile * kSortTile;
+        // all of the wave's key loads are issued before the first is used
+        uint32_t kk[kChunksPerWave];
 #pragma unroll
         for (int c = 0; c < kChunksPerWave; ++c) {
-            int64_t idx = base + (int64_t)(wave * kChunksPerWave + c) * 64 + lane;
-            bool valid = idx < n;
-            uint32_t digit = valid ? ((keys[idx] >> shift) & mask) : 0u;
-            uint64_t active = __ballot(valid);
-            if (active == 0) break;
-            uint64_t m = digit_match<BITS>(digit, active);
-            int leader = __ffsll((unsigned long long)m) - 1;
+            const int64_t idx = base + (int64_t)(wave * kChunksPerWave + c) * 64 + lane;
+            kk[c] = idx < n ? keys[idx] : 0u;
+        }
+#pragma unroll
+        for (int c = 0; c < kChunksPerWave; ++c) {
+            const int64_t idx = base + (int64_t)(wave * kChunksPerWave + c) * 64 + lane;
+            const bool valid = idx < n;
+            const uint64_t active = __ballot(valid);
+            if (active == 0) continue;  // wave-uniform: the tail of the last tile
+            const uint32_t digit = (kk[c] >> shift) & mask;
+            const uint64_t m = digit_match<BITS>(digit, active);
+            const int leader = __ffsll((unsigned long long)m) - 1;
             if (valid && lane == leader) atomicAdd(&h[digit], (uint32_t)__popcll(m));
         }
         __syncthreads();
