@@ -912,6 +912,42 @@ __device__ __forceinline__ void record_move(const ItemsDev& I, bool moved, uint3
     }
 }
 
+// ------------------------------------------------------ XCD-aware items
+// Workgroups are dispatched to the 8 XCDs in turn (workgroup b on XCD b % 8),
+// and each XCD has its own L2.  Consecutive work items are neighbouring key
+// blocks whose tiles share halo nodes, so a persistent kernel hands out
+// chunks of kXcdChunk consecutive items per XCD: XCD x takes chunks x,
+// x + 8, ..., and its workgroups step through the items of those chunks.
+// Which workgroup runs on which XCD only changes the L2 hits, not the
+// coverage: every item is taken exactly once (fewer than 8 workgroups walk
+// the plain order).
+#ifndef MPM_XCD_CHUNK
+#define MPM_XCD_CHUNK 16
+#endif
+constexpr uint32_t kXcds = 8;
+constexpr uint32_t kXcdChunk = MPM_XCD_CHUNK;
+struct XcdWalk {
+    uint32_t x, step, k;
+    bool on;
+};
+__device__ __forceinline__ XcdWalk xcd_walk() {
+    XcdWalk w;
+    w.on = kXcdChunk > 0 && gridDim.x >= kXcds;
+    w.x = blockIdx.x % kXcds;
+    w.step = w.on ? (gridDim.x - 1 - w.x) / kXcds + 1 : gridDim.x;
+    w.k = w.on ? blockIdx.x / kXcds : blockIdx.x;
+    return w;
+}
+// Item at the walk's k-th position (increasing in k for one XCD).
+__device__ __forceinline__ uint32_t xcd_item(const XcdWalk& w, uint32_t k) {
+    if constexpr (kXcdChunk == 0) {
+        return k;
+    } else {
+        if (!w.on) return k;
+        return (w.x + kXcds * (k / kXcdChunk)) * kXcdChunk + k % kXcdChunk;
+    }
+}
+
 // -------------------------------------------------------------------- G2P
 // Folded-grid tile of one item, in two halves.  fetch_folded_tile issues
 // the lane's tile-node loads (mass and p + dt f) as global_load_lds_dword:
@@ -1498,15 +1534,17 @@ k_g2p2g(SceneDev S0, ParticlesDev P0, ItemsDev I, const SceneEntry* __restrict__
     const uint32_t nitems = *I.count;
     float* const ot = otile + wave * (4 * TN);
     ItemGeom gnext{};
-    if (blockIdx.x < nitems) {
+    const XcdWalk walk = xcd_walk();
+    const uint32_t it0 = xcd_item(walk, walk.k);
+    if (it0 < nitems) {
         SceneDev Sn;
         ParticlesDev Pn;
         uint32_t pbn, kon;
-        item_scene<BATCH>(S0, P0, I, tab, blockIdx.x, Sn, Pn, pbn, kon);
-        gnext = item_geom(Sn, I, blockIdx.x, nitems, kon);
+        item_scene<BATCH>(S0, P0, I, tab, it0, Sn, Pn, pbn, kon);
+        gnext = item_geom(Sn, I, it0, nitems, kon);
         fetch_folded_tile<T, TR, WG>(Sn, gnext, stage);
     }
-    for (uint32_t it = blockIdx.x; it < nitems; it += gridDim.x) {
+    for (uint32_t k = walk.k, it = it0; it < nitems; k += walk.step, it = xcd_item(walk, k)) {
         SceneDev S;
         ParticlesDev P;
         uint32_t pbase, koff;
@@ -1525,7 +1563,7 @@ k_g2p2g(SceneDev S0, ParticlesDev P0, ItemsDev I, const SceneEntry* __restrict__
         if (!g.escape) commit_folded_tile<T, TR, WG>(S, g, stage, tile);
         for (int q = threadIdx.x; q < NW * 4 * TN; q += WG) otile[q] = 0.0f;
         __syncthreads();
-        const uint32_t itn = it + gridDim.x;
+        const uint32_t itn = xcd_item(walk, k + walk.step);
         if (itn < nitems) {
             SceneDev Sn;
             ParticlesDev Pn;
