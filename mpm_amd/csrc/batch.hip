@@ -351,7 +351,20 @@ int sync_members(Batch& b) {
     const bool fuse = fuses(b);
     for (size_t i = 0; i < b.members.size(); ++i) {
         Solver& s = *b.members[i];
-        if (!fuse) batch_unfuse_member(&b, s);  // fusion turned off (tune) or no longer possible
+        if (!fuse) {
+            batch_unfuse_member(&b, s);  // fusion turned off (tune) or no longer possible
+            if (s.grid2) {  // its second grid is not needed while a member (it returns on first use)
+                // the scene's own captured step may use either buffer: it is captured again
+                if (s.gexec) { (void)hipGraphExecDestroy(s.gexec); s.gexec = nullptr; }
+                if (s.graph) { (void)hipGraphDestroy(s.graph); s.graph = nullptr; }
+                s.graph_grid = nullptr;
+                dfree(s, s.grid2);
+                dfree(s, s.touched2);
+                s.grid2 = nullptr;
+                s.touched2 = nullptr;
+                s.grid2_dirty = false;
+            }
+        }
         if (fuse) {
             int rc = ensure_grid2_buffers(s);
             if (rc) return rc;
