@@ -270,9 +270,9 @@ MPM_API int mpm_sort(void* handle);
  * reference never reorders: src/mpm_solver3d_builder.cu:38-65). */
 MPM_API int mpm_read_particles(void* handle, mpm_particle_state* out);
 /* The same readback without allocating device memory: no counterpart in the
- * reference.  For callers that cannot call hipMalloc (tight serving loops,
- * where a hipFree would also wait for the whole device, and harnesses that
- * forbid allocation).  The device staging is a file-scope array of the
+ * reference.  For callers that cannot call hipMalloc (code under graph
+ * capture, tight loops and harnesses that forbid allocation).  The device
+ * staging is a file-scope array of the
  * library (39.3 MB per GPU, held until the process ends) that every call on
  * that GPU shares: at most 262,144 particles (larger handles get
  * MPM_ERR_UNSUPPORTED; use mpm_read_particles).  Calls from several host
