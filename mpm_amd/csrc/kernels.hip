@@ -1032,17 +1032,19 @@ __global__ void __launch_bounds__(WG) __attribute__((amdgpu_waves_per_eu(G2PWave
     __shared__ float pstage[PSTAGE ? PW * WG : 1];
     const uint32_t nitems = *I.count;
     ItemGeom gnext{};
+    const XcdWalk walk = xcd_walk();  // items in XCD-local chunks (neighbouring tiles share an L2)
+    const uint32_t it0 = xcd_item(walk, walk.k);
     if constexpr (PREFETCH) {
-        if (blockIdx.x < nitems) {
+        if (it0 < nitems) {
             SceneDev Sn;
             ParticlesDev Pn;
             uint32_t pbn, kon;
-            item_scene<BATCH>(S0, P0, I, tab, blockIdx.x, Sn, Pn, pbn, kon);
-            gnext = item_geom(Sn, I, blockIdx.x, nitems, kon);
+            item_scene<BATCH>(S0, P0, I, tab, it0, Sn, Pn, pbn, kon);
+            gnext = item_geom(Sn, I, it0, nitems, kon);
             fetch_folded_tile<T, TR, WG>(Sn, gnext, stage);
         }
     }
-    for (uint32_t it = blockIdx.x; it < nitems; it += gridDim.x) {
+    for (uint32_t k = walk.k, it = it0; it < nitems; k += walk.step, it = xcd_item(walk, k)) {
         SceneDev S;
         ParticlesDev P;
         uint32_t pbase, koff;
@@ -1093,7 +1095,7 @@ __global__ void __launch_bounds__(WG) __attribute__((amdgpu_waves_per_eu(G2PWave
         }
         __syncthreads();
         if constexpr (PREFETCH) {
-            const uint32_t itn = it + gridDim.x;
+            const uint32_t itn = xcd_item(walk, k + walk.step);
             if (itn < nitems) {
                 SceneDev Sn;  // the next item's scene (a batch's items span scenes)
                 ParticlesDev Pn;
