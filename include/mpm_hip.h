@@ -394,6 +394,7 @@ typedef struct mpm_batch_stats {
     int64_t device_bytes;    /* batch-owned device memory                */
     double last_step_ms;     /* host time of the last step call (enqueue) */
     mpm_tune tune;           /* parameters in use                        */
+    int64_t incremental_sorts; /* steps sorted by the incremental update */
 } mpm_batch_stats;
 
 MPM_API int mpm_batch_create(int32_t interp, int32_t transfer, void** out_batch);
@@ -417,6 +418,21 @@ MPM_API int mpm_batch_step_graph(void* batch, int n);
  * right after a reorder and 3.9 ms 100 steps later, and one reorder costs
  * ~3.3 ms, so 16 steps (3.0 ms per step on average) beats 100 (3.5 ms). */
 MPM_API int mpm_batch_set_reorder_interval(void* batch, int steps);
+/* The batch's cell sort updates the last step's order (the particles that
+ * changed cell taken out and put back) instead of sorting in full, when
+ * the list of moves fits (up to max(16384, particles / 8)); bit-identical
+ * to the full sort.  On by default; 0 sorts in full every step.  The
+ * single-scene counterpart is automatic (scenes of 2^20 particles and up).
+ * No reference counterpart: the reference sorts nothing, its P2G adds each
+ * particle's stencil with global atomics (src/mpm_solver3d.cu:254-285). */
+MPM_API int mpm_batch_set_incremental_sort(void* batch, int enable);
+/* The last batch step's sorted order: keys[i] (scene key offset included)
+ * and the global slot (member stores concatenated in membership order) of
+ * sorted position i, after that step's store reorder of a chunk of members
+ * renumbered their slots.  count must equal the batch's particle count;
+ * either pointer may be NULL.  No reference counterpart (the reference
+ * sorts nothing); for tests and debugging. */
+MPM_API int mpm_batch_read_sort(void* batch, uint32_t* keys, uint32_t* slots, int64_t count);
 MPM_API int mpm_batch_synchronize(void* batch);
 /* Fixes the launch parameters; NULL returns to the tuning table. */
 MPM_API int mpm_batch_set_tune(void* batch, const mpm_tune* tune);

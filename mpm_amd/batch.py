@@ -93,6 +93,19 @@ class Batch:
         steps (0 = never; default 16)."""
         N.check(self._lib.mpm_batch_set_reorder_interval(self._h, int(steps)))
 
+    def set_incremental_sort(self, enable: bool):
+        """Update the last step's cell order instead of sorting in full (on
+        by default; bit-identical to the full sort)."""
+        N.check(self._lib.mpm_batch_set_incremental_sort(self._h, int(bool(enable))))
+
+    def read_sort(self):
+        """(keys, global slots) of the last step's sorted order."""
+        n = int(self.info()["particles"])
+        keys = np.empty(n, np.uint32)
+        slots = np.empty(n, np.uint32)
+        N.check(self._lib.mpm_batch_read_sort(self._h, N.uptr(keys), N.uptr(slots), n))
+        return keys, slots
+
     def info(self) -> dict:
         s = N.MpmBatchStats()
         N.check(self._lib.mpm_batch_info(self._h, C.byref(s)))

@@ -111,6 +111,15 @@ struct Batch {
     int64_t slot_scene_cap = 0;
     uint32_t* chunk_scratch = nullptr;
     int64_t chunk_scratch_cap = 0;
+    // incremental sort (incsort.hip; on unless switched off): snapshot of the
+    // sorted order, plan words, move list; snap_ok: the device snapshot and
+    // list describe the current keys (else the next step sorts in full)
+    bool incr = true;
+    bool snap_ok = false;
+    uint32_t *snap_k = nullptr, *snap_p = nullptr, *plan = nullptr;
+    MoveList moves{};
+    MoveList* moves_dev = nullptr;
+    uint32_t move_cap = 0;  // capacity of one sub-list (also in the plan words)
     // per-member health words (3 per member), allocated on first use
     uint32_t* health = nullptr;
     int health_cap = 0;
@@ -167,14 +176,30 @@ int ensure_capacity(Batch& b, int64_t n) {
     if (n <= b.cap && b.kin) return MPM_OK;
     (void)hipStreamSynchronize(b.st);
     int64_t c = std::max<int64_t>(std::max<int64_t>(n, 1), b.cap + b.cap / 2);
-    uint32_t** bufs[] = {&b.kin, &b.iota, &b.ka, &b.va, &b.kb, &b.vb, &b.flags, &b.item_scene};
+    uint32_t** bufs[] = {&b.kin, &b.iota, &b.ka, &b.va, &b.kb, &b.vb, &b.flags, &b.item_scene, &b.snap_k, &b.snap_p};
+    uint32_t** lists[] = {&b.moves.key, &b.moves.slot, &b.moves.pos, &b.moves.lb, &b.moves.del_b, &b.moves.ins_b};
+    uint32_t** tiles[] = {&b.moves.tcount, &b.moves.tnext};
     for (auto pp : bufs) bfree(b, *(void**)pp);
+    for (auto pp : lists) bfree(b, *(void**)pp);
+    for (auto pp : tiles) bfree(b, *(void**)pp);
     bfree(b, *(void**)&b.items);
     bfree(b, *(void**)&b.hist);
     bfree(b, *(void**)&b.scan_temp);
     int rc;
     for (auto pp : bufs)
         if ((rc = balloc(b, (void**)pp, (size_t)c * 4))) return rc;
+    // the incremental sort's list and tile counts for up to c particles
+    b.move_cap = inc_sublist_cap(c);
+    for (auto pp : lists)
+        if ((rc = balloc(b, (void**)pp, (size_t)kMoveLists * b.move_cap * 4))) return rc;
+    for (auto pp : tiles)
+        if ((rc = balloc(b, (void**)pp, (size_t)2 * (inc_tiles(c) + 1) * 4))) return rc;
+    if (!b.moves_dev && (rc = balloc(b, (void**)&b.moves_dev, sizeof(MoveList)))) return rc;
+    if (!b.plan && (rc = balloc(b, (void**)&b.plan, kPlanWords * 4))) return rc;
+    HIP_TRY(hipMemcpyAsync(b.moves_dev, &b.moves, sizeof(MoveList), hipMemcpyHostToDevice, b.st));
+    HIP_TRY(hipMemsetAsync(b.plan, 0, kPlanWords * 4, b.st));  // no snapshot yet
+    HIP_TRY(hipMemcpyAsync(b.plan + kPlanCap, &b.move_cap, 4, hipMemcpyHostToDevice, b.st));
+    b.snap_ok = false;
     if ((rc = balloc(b, (void**)&b.items, (size_t)c * 8))) return rc;  // starts + blocks
     const int64_t hw = radix_hist_words(c);
     if ((rc = balloc(b, (void**)&b.hist, (size_t)hw * 4))) return rc;
@@ -327,6 +352,14 @@ int relayout(Batch& b) {
     b.I.count = b.counters;
     b.I.scene = b.item_scene;
     b.I.n = n;
+    if (b.incr) {
+        b.I.snap_k = b.snap_k;
+        b.I.snap_p = b.snap_p;
+        b.I.plan = b.plan;
+        b.I.moves = b.moves_dev;
+        b.I.move_cap = b.move_cap;
+    }
+    b.snap_ok = false;  // slots and keys laid out anew
     if (!b.tune_fixed) {
         std::lock_guard<std::mutex> g(g_tune_mu);
         auto it = g_tune.find({b.interp, b.apic, size_bucket(n)});
@@ -381,6 +414,7 @@ int sync_members(Batch& b) {
         if (b.synced_version[i] != s.version) {
             launch_keys(s.S, s.P, b.interp, b.kin + b.tab_host[i].pstart, b.st, b.tab_host[i].key_offset);
             b.synced_version[i] = s.version;
+            b.snap_ok = false;  // keys the last step's moves do not describe
         }
     }
     if (upload) {
@@ -392,12 +426,12 @@ int sync_members(Batch& b) {
     return MPM_OK;
 }
 
-// reorder: permute every member's store by this step's own sort (eager
-// steps; reorder_scratch allocated), after which the step runs on the
-// identity permutation.
 // fused: the fused transfer; scattered: every member's grid already holds
-// this step's P2G (else the step clears and scatters the others first).
-void enqueue_step(Batch& b, bool fused, bool scattered) {
+// this step's P2G (else the step clears and scatters the others first);
+// full_sort: the step is known to sort in full (the snapshot was just
+// invalidated), else the radix launches use the light plan of a sort that
+// mostly returns at once.
+void enqueue_step(Batch& b, bool fused, bool scattered, bool full_sort) {
     const int ns = (int)b.members.size();
     if (!fused) {
         launch_clear_batch(b.tab, b.block_start, ns, b.total_blocks, b.fold ? GF_FOLDED : GF_PRIMARY, b.st);
@@ -405,15 +439,24 @@ void enqueue_step(Batch& b, bool fused, bool scattered) {
         if (!scattered) launch_clear_batch(b.tab, b.block_start, ns, b.total_blocks, GF_FOLDED, b.st, 1);
         launch_clear_batch(b.tab, b.block_start, ns, b.total_blocks, GF_FOLDED, b.st, 2);
     }
+    // the sort: the update of the last step's order or the full radix sort,
+    // as the plan words decide on the device (a captured step holds both)
+    uint32_t* const plan = b.incr ? b.plan : nullptr;
+    if (plan) launch_inc_prepare(plan, b.moves, b.snap_k, b.snap_p, b.ntot, b.st);
     const int sel = radix_sort_pairs(b.kin, b.iota, b.ka, b.va, b.kb, b.vb, b.ntot, b.key_bits, b.hist, b.scan_temp,
-                                     b.st, ProfileHook{});
+                                     b.st, ProfileHook{}, plan, plan && !full_sort);
     b.I.sorted_keys = sel ? b.kb : b.ka;
     b.I.perm = sel ? b.vb : b.va;
-    if (b.reorder_interval > 0 && b.nchunks > 0)  // one chunk of members into this step's order
+    if (plan) launch_inc_merge(plan, b.moves, b.snap_k, b.snap_p, b.ntot, sel ? b.kb : b.ka, sel ? b.vb : b.va, b.st);
+    // one chunk of members into this step's order: their slots become their
+    // sorted positions, which keeps the (key, slot) order of every cell (a
+    // member's cells hold only its particles), so the snapshot the transfer
+    // takes afterwards stays sorted
+    if (b.reorder_interval > 0 && b.nchunks > 0)
         launch_reorder_chunk(b.tab, b.slot_scene, sel ? b.vb : b.va, b.chunk_bounds, b.chunk_rr, b.nchunks,
                              b.chunk_max, b.chunk_scratch, b.apic != 0, b.st);
     launch_items(b.I.sorted_keys, b.ntot, b.flags, b.scan_temp, b.items, b.counters, b.st, (uint32_t)b.tune.item_max,
-                 b.key_offsets, ns, b.item_scene);
+                 b.key_offsets, ns, b.item_scene, plan);
     const SceneDev S0{};
     const ParticlesDev P0{};
     const int wgs = effective_wgs(b);
@@ -502,6 +545,7 @@ int reorder_members(Batch& b) {
     launch_reorder_store(ParticlesDev{}, b.tab, (int)b.members.size(), perm, b.ntot, b.reorder_scratch, b.apic != 0,
                          b.st, b.flags);
     HIP_TRY(hipMemcpyAsync(b.kin, sorted, (size_t)b.ntot * 4, hipMemcpyDeviceToDevice, b.st));
+    b.snap_ok = false;  // slots renumbered
     for (Solver* m : b.members) {
         m->keys_valid = false;  // the member's own sort input is in the old slot order
         m->sort_valid = false;
@@ -512,7 +556,7 @@ int reorder_members(Batch& b) {
 int capture_step(Batch& b, bool fused) {
     drop_graph(b);
     HIP_TRY(hipStreamBeginCapture(b.st, hipStreamCaptureModeThreadLocal));
-    enqueue_step(b, fused, fused);
+    enqueue_step(b, fused, fused, false);
     hipError_t e = hipStreamEndCapture(b.st, &b.graph);
     if (e != hipSuccess) return fail(MPM_ERR_HIP, std::string("batch graph capture: ") + hipGetErrorString(e));
     HIP_TRY(hipGraphInstantiate(&b.gexec, b.graph, nullptr, nullptr, 0));
@@ -545,8 +589,13 @@ int run_steps(Batch& b, int n, bool graph) {
             e1 = take_event(b);
             if (e0 && e1) HIP_TRY(hipEventRecord(e0, b.st));
         }
+        const bool full_sort = !b.incr || !b.snap_ok;
+        if (b.incr && !b.snap_ok) {  // the full sort this time (outside a captured step)
+            launch_inc_set(b.plan, 0u, b.st);
+            b.snap_ok = true;
+        }
         if (graph && !first_scatter) HIP_TRY(hipGraphLaunch(b.gexec, b.st));
-        else enqueue_step(b, fused, scattered);
+        else enqueue_step(b, fused, scattered, full_sort);
         if (e0 && e1) {
             HIP_TRY(hipEventRecord(e1, b.st));
             b.pending.push_back({e0, e1});
@@ -737,6 +786,30 @@ MPM_API int mpm_batch_info(void* batch, mpm_batch_stats* out) {
     out->device_bytes = b->device_bytes;
     out->last_step_ms = b->last_step_ms;
     to_c(*b, b->tune, &out->tune, 0.0);
+    if (b->plan) {
+        uint32_t inc = 0;
+        HIP_TRY(hipMemcpy(&inc, b->plan + kPlanIncSteps, 4, hipMemcpyDeviceToHost));
+        out->incremental_sorts = inc;
+    }
+    return MPM_OK;
+}
+
+MPM_API int mpm_batch_read_sort(void* batch, uint32_t* keys, uint32_t* slots, int64_t count) {
+    GET_BATCH(b, batch);
+    if (count != b->ntot) return fail(MPM_ERR_INVALID, "count must equal the batch's particle count");
+    if (b->steps == 0 || !b->I.sorted_keys) return fail(MPM_ERR_STATE, "the batch has not stepped");
+    if (keys) HIP_TRY(hipMemcpyAsync(keys, b->I.sorted_keys, (size_t)count * 4, hipMemcpyDeviceToHost, b->st));
+    if (slots) HIP_TRY(hipMemcpyAsync(slots, b->I.perm, (size_t)count * 4, hipMemcpyDeviceToHost, b->st));
+    HIP_TRY(hipStreamSynchronize(b->st));
+    return MPM_OK;
+}
+
+MPM_API int mpm_batch_set_incremental_sort(void* batch, int enable) {
+    GET_BATCH(b, batch);
+    HIP_TRY(hipStreamSynchronize(b->st));
+    b->incr = enable != 0;
+    b->layout_valid = false;  // the items' view of the snapshot and list is set by the layout
+    drop_graph(*b);
     return MPM_OK;
 }
 

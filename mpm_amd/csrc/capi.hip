@@ -399,6 +399,10 @@ void run_sort_incremental(Solver& s) {
         s.snap_ok = false;
     }
     if (!s.snap_ok) launch_inc_set(s.plan, 0u, s.st);  // the full sort this time
+    {
+        Scope sc(h, "sort_incr");
+        launch_inc_prepare(s.plan, s.moves, s.snap_k, s.snap_p, s.N, s.st);
+    }
     const int sel = radix_sort_pairs(s.kin, s.iota, s.ka, s.va, s.kb, s.vb, s.N, s.key_bits, s.hist, s.scan_temp,
                                      s.st, h, s.plan, s.few_passes);
     uint32_t* out_k = sel ? s.kb : s.ka;
@@ -411,7 +415,7 @@ void run_sort_incremental(Solver& s) {
     s.I.n = s.N;
     {
         Scope sc(h, "sort_incr");
-        launch_inc_sort(s.plan, s.moves, s.snap_k, s.snap_p, s.N, out_k, out_p, s.st);
+        launch_inc_merge(s.plan, s.moves, s.snap_k, s.snap_p, s.N, out_k, out_p, s.st);
     }
     {
         Scope sc(h, "items");  // also resets the plan words for this step's G2P
@@ -506,6 +510,7 @@ void run_fused_step(Solver& s) {
         I.snap_p = s.snap_p;
         I.plan = s.plan;
         I.moves = s.moves_dev;
+        I.move_cap = s.move_cap;
     }
     // APIC's G2P reads v* alone, so by default its P2G sums p + dt f into
     // one set of planes (4 instead of 7 per node)
@@ -611,12 +616,15 @@ int build(Solver& s, HostState& H, int run_init) {
         if ((rc = dalloc(s, (void**)&s.snap_p, (size_t)n * 4))) return rc;
         if ((rc = dalloc(s, (void**)&s.plan, kPlanWords * 4))) return rc;
         HIP_TRY(hipMemsetAsync(s.plan, 0, kPlanWords * 4, s.st));  // snapshot not valid yet
-        uint32_t** lists[] = {&s.moves.key, &s.moves.slot, &s.moves.pos, &s.moves.lb, &s.moves.del_sorted,
-                              &s.moves.lb_sorted};
+        s.move_cap = inc_sublist_cap(s.N);
+        uint32_t** lists[] = {&s.moves.key, &s.moves.slot, &s.moves.pos, &s.moves.lb, &s.moves.del_b, &s.moves.ins_b};
         for (auto pp : lists)
-            if ((rc = dalloc(s, (void**)pp, (size_t)kMoveMax * 4))) return rc;
-        if ((rc = dalloc(s, (void**)&s.moves.part, (size_t)3 * kIncSlices * kMoveMax * 4))) return rc;
+            if ((rc = dalloc(s, (void**)pp, (size_t)kMoveLists * s.move_cap * 4))) return rc;
+        const size_t tw = (size_t)2 * (inc_tiles(s.N) + 1);
+        if ((rc = dalloc(s, (void**)&s.moves.tcount, tw * 4))) return rc;
+        if ((rc = dalloc(s, (void**)&s.moves.tnext, tw * 4))) return rc;
         if ((rc = dalloc(s, (void**)&s.moves_dev, sizeof(MoveList)))) return rc;
+        HIP_TRY(hipMemcpyAsync(s.plan + kPlanCap, &s.move_cap, 4, hipMemcpyHostToDevice, s.st));
         HIP_TRY(hipHostMalloc((void**)&s.inc_host, 64, hipHostMallocDefault));
         HIP_TRY(hipEventCreateWithFlags(&s.inc_ev, hipEventDisableTiming));
         HIP_TRY(hipMemcpyAsync(s.moves_dev, &s.moves, sizeof(MoveList), hipMemcpyHostToDevice, s.st));

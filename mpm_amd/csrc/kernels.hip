@@ -182,10 +182,13 @@ __global__ void __launch_bounds__(kScanWG) k_item_count(const uint32_t* __restri
                                                         uint32_t item_max, uint32_t* __restrict__ partial,
                                                         uint32_t* __restrict__ plan) {
     __shared__ uint32_t ws[kScanWG / 64];
-    if (plan && blockIdx.x == 0 && threadIdx.x == 0) {
-        plan[kPlanIncSteps] += plan_incremental(plan) ? 1u : 0u;
-        plan[kPlanMoved] = 0u;
-        plan[kPlanSnapOk] = 1u;
+    if (plan && blockIdx.x == 0) {
+        if (threadIdx.x < kMoveLists) plan[kPlanLists + threadIdx.x * kMoveCountStride] = 0u;
+        if (threadIdx.x == 0) {
+            plan[kPlanIncSteps] += plan_incremental(plan) ? 1u : 0u;
+            plan[kPlanSnapOk] = 1u;
+            plan[kPlanOverflow] = 0u;
+        }
     }
     const int64_t base = ((int64_t)blockIdx.x * kScanWG + threadIdx.x) * IPT;
     uint32_t key[IPT];
@@ -890,24 +893,30 @@ void launch_p2g(const SceneDev& S, const ParticlesDev& P, const ItemsDev& I, int
 }
 
 // A particle whose cell key changed in this step goes on the move list of
-// the incremental sort: one counter atomic per wave, then each moved lane
-// writes (new key, slot, old sorted position) at its rank.  Entries past
-// kMoveMax are counted but not stored (the next step then sorts in full).
+// the incremental sort: the wave takes sub-list (global wave index mod
+// kMoveLists), one counter atomic per wave, then each moved lane writes
+// (new key, slot, old sorted position) at its rank.  Past the sub-list's
+// capacity nothing is stored and the overflow word sends the next step to
+// the full sort.
 __device__ __forceinline__ void record_move(const ItemsDev& I, bool moved, uint32_t key, uint32_t slot,
                                             uint32_t pos) {
     const uint64_t m = __ballot(moved);
     if (m == 0ull) return;
     const int lane = threadIdx.x & 63;
     const int leader = __ffsll((unsigned long long)m) - 1;
+    const uint32_t r = (blockIdx.x * (blockDim.x >> 6) + (threadIdx.x >> 6)) & (uint32_t)(kMoveLists - 1);
     uint32_t base = 0;
-    if (lane == leader) base = atomicAdd(&I.plan[kPlanMoved], (uint32_t)__popcll(m));
+    if (lane == leader) base = atomicAdd(&I.plan[kPlanLists + r * kMoveCountStride], (uint32_t)__popcll(m));
     base = __shfl(base, leader, 64);
     if (moved) {
         const uint32_t idx = base + (uint32_t)__popcll(m & ((1ull << lane) - 1ull));
-        if (idx < (uint32_t)kMoveMax) {
-            I.moves->key[idx] = key;
-            I.moves->slot[idx] = slot;
-            I.moves->pos[idx] = pos;
+        if (idx < I.move_cap) {
+            const size_t e = (size_t)r * I.move_cap + idx;
+            I.moves->key[e] = key;
+            I.moves->slot[e] = slot;
+            I.moves->pos[e] = pos;
+        } else {
+            I.plan[kPlanOverflow] = 1u;
         }
     }
 }
@@ -1107,6 +1116,9 @@ __global__ void __launch_bounds__(WG) __attribute__((amdgpu_waves_per_eu(G2PWave
         }
         for (uint32_t i = g.s + threadIdx.x; i < g.e; i += WG) {
             const uint32_t p = I.perm[i] - pbase;
+            // the key it was sorted by, for the move list: loaded with the
+            // slot, long before the comparison needs it
+            const uint32_t kold = I.plan ? I.sorted_keys[i] : 0u;
             float* const pst = pstage + (threadIdx.x >> 6) * (PW * 64);  // the wave's region
             if constexpr (PSTAGE) {
 #pragma unroll 1
@@ -1284,8 +1296,8 @@ __global__ void __launch_bounds__(WG) __attribute__((amdgpu_waves_per_eu(G2PWave
                 if (next_keys) {
                     const uint32_t knew = koff + cell_key(S, y0, y1, y2, Stencil<INTERP>::SHIFT, Stencil<INTERP>::W);
                     next_keys[pbase + p] = knew;
-                    if (I.plan)  // uniform: the fused step of one large scene (the key it was sorted by)
-                        record_move(I, knew != I.sorted_keys[i], knew, pbase + p, i);
+                    if (I.plan)  // uniform: the incremental sort is on
+                        record_move(I, knew != kold, knew, pbase + p, i);
                 }
             }
             if constexpr (!PSTAGE) {
@@ -1633,11 +1645,11 @@ k_g2p2g(SceneDev S0, ParticlesDev P0, ItemsDev I, const SceneEntry* __restrict__
                 for (int c = 0; c < 3; ++c) { Cn[r][c] = 0.0f; vs[r][c] = 0.0f; }
             if (active) {
                 // ---- step n: gridToParticles (src/apic_mpm_solver3d.cu:326-388)
-                const uint32_t p = I.perm[i] - pbase;
+                const uint32_t praw = I.perm[i], p = praw - pbase;
                 kold = I.sorted_keys[i];
                 if (I.snap_k) {  // the sorted order, kept for the next step's incremental sort
                     I.snap_k[i] = kold;
-                    I.snap_p[i] = p;
+                    I.snap_p[i] = praw;
                 }
                 float* const pst = pstage + wave * (PW * 64);
 #pragma unroll 1
@@ -1683,7 +1695,7 @@ k_g2p2g(SceneDev S0, ParticlesDev P0, ItemsDev I, const SceneEntry* __restrict__
                 P.x[2 * N + p] = xn[2];
                 knew = koff + cell_key(S, xn[0], xn[1], xn[2], Stencil<INTERP>::SHIFT, Stencil<INTERP>::W);
                 next_keys[pbase + p] = knew;
-                if (I.plan) record_move(I, knew != kold, knew, p, i);  // one scene only: pbase is 0
+                if (I.plan) record_move(I, knew != kold, knew, praw, i);
                 // updateDeformationGradient and the stress of the new state
                 float Fe[3][3];
                 asm volatile("s_waitcnt vmcnt(0)" ::: "memory");  // staged words have landed

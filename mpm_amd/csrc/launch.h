@@ -33,13 +33,26 @@ int radix_sort_pairs(const uint32_t* kin, const uint32_t* vin, uint32_t* ka, uin
                      const ProfileHook& prof, const uint32_t* plan = nullptr, bool few_passes = false);
 
 // ---- incsort.hip
-// plan words: reset the move counter and set whether the snapshot is the
-// previous step's (0 forces the full sort next).
+// plan words: reset the move counter and the overflow and set whether the
+// snapshot is the previous step's (0 forces the full sort next).
 void launch_inc_set(uint32_t* plan, uint32_t snap_ok, hipStream_t st);
-// The incremental update (returns at once unless the plan chose it): the
-// snapshot (sorted by the previous keys) and the move list -> out_k/out_p.
-void launch_inc_sort(const uint32_t* plan, const MoveList& L, const uint32_t* snap_k, const uint32_t* snap_p,
-                     int64_t n, uint32_t* out_k, uint32_t* out_p, hipStream_t st);
+// The incremental update (its launches return at once unless the plan
+// chose it), in two halves around the radix sort: prepare (places among the
+// snapshot, bucket offsets; may still send the step to the full sort), then
+// merge (the snapshot, sorted by the previous keys, and the move list ->
+// out_k/out_p).  n snapshot entries.
+void launch_inc_prepare(uint32_t* plan, const MoveList& L, const uint32_t* snap_k, const uint32_t* snap_p, int64_t n,
+                        hipStream_t st);
+void launch_inc_merge(const uint32_t* plan, const MoveList& L, const uint32_t* snap_k, const uint32_t* snap_p,
+                      int64_t n, uint32_t* out_k, uint32_t* out_p, hipStream_t st);
+// Sizes of the update's buffers for n particles: snapshot tiles (the tile
+// arrays hold 2 * (tiles + 1) words each) and the capacity of one sub-list
+// (the list arrays hold kMoveLists times that).
+uint32_t inc_tiles(int64_t n);
+inline uint32_t inc_sublist_cap(int64_t n) {
+    const int64_t cap = n / 8 > kMoveMin ? n / 8 : kMoveMin;
+    return (uint32_t)((cap + kMoveLists - 1) / kMoveLists);
+}
 
 // ---- kernels.hip
 // Persistent launch width of the item kernels.

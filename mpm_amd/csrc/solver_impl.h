@@ -130,6 +130,7 @@ struct Solver {
     uint32_t* plan = nullptr;
     MoveList moves{};
     MoveList* moves_dev = nullptr;  // device copy of `moves` (the kernels' view)
+    uint32_t move_cap = 0;          // capacity of one sub-list (also in the plan words)
     // how often the update ran lately: the incremental-sort count read back
     // every few steps through pinned memory (never waited for); mostly
     // updates -> the full sort's launches use fewer passes (few_passes)

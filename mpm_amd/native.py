@@ -135,6 +135,7 @@ class MpmBatchStats(C.Structure):
         ("device_bytes", C.c_int64),
         ("last_step_ms", C.c_double),
         ("tune", MpmTune),
+        ("incremental_sorts", C.c_int64),
     ]
 
 
@@ -206,6 +207,8 @@ SIGNATURES = {
     "mpm_batch_step_graph": (C.c_int, [C.c_void_p, C.c_int]),
     "mpm_batch_synchronize": (C.c_int, [C.c_void_p]),
     "mpm_batch_set_reorder_interval": (C.c_int, [C.c_void_p, C.c_int]),
+    "mpm_batch_set_incremental_sort": (C.c_int, [C.c_void_p, C.c_int]),
+    "mpm_batch_read_sort": (C.c_int, [C.c_void_p, C.c_void_p, C.c_void_p, C.c_int64]),
     "mpm_batch_set_tune": (C.c_int, [C.c_void_p, C.POINTER(MpmTune)]),
     "mpm_batch_autotune": (C.c_int, [C.c_void_p, C.c_int, C.POINTER(MpmTune)]),
     "mpm_tune_lookup": (C.c_int, [C.c_int32, C.c_int32, C.c_int64, C.POINTER(MpmTune)]),
