@@ -194,6 +194,8 @@ int ensure_capacity(Batch& b, int64_t n) {
         if ((rc = balloc(b, (void**)pp, (size_t)kMoveLists * b.move_cap * 4))) return rc;
     for (auto pp : tiles)
         if ((rc = balloc(b, (void**)pp, (size_t)2 * (inc_tiles(c) + 1) * 4))) return rc;
+    // the merge keeps the counts zero afterwards
+    HIP_TRY(hipMemsetAsync(b.moves.tcount, 0, (size_t)2 * (inc_tiles(c) + 1) * 4, b.st));
     if (!b.moves_dev && (rc = balloc(b, (void**)&b.moves_dev, sizeof(MoveList)))) return rc;
     if (!b.plan && (rc = balloc(b, (void**)&b.plan, kPlanWords * 4))) return rc;
     HIP_TRY(hipMemcpyAsync(b.moves_dev, &b.moves, sizeof(MoveList), hipMemcpyHostToDevice, b.st));

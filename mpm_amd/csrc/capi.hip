@@ -622,6 +622,7 @@ int build(Solver& s, HostState& H, int run_init) {
             if ((rc = dalloc(s, (void**)pp, (size_t)kMoveLists * s.move_cap * 4))) return rc;
         const size_t tw = (size_t)2 * (inc_tiles(s.N) + 1);
         if ((rc = dalloc(s, (void**)&s.moves.tcount, tw * 4))) return rc;
+        HIP_TRY(hipMemsetAsync(s.moves.tcount, 0, tw * 4, s.st));  // the merge keeps them zero afterwards
         if ((rc = dalloc(s, (void**)&s.moves.tnext, tw * 4))) return rc;
         if ((rc = dalloc(s, (void**)&s.moves_dev, sizeof(MoveList)))) return rc;
         HIP_TRY(hipMemcpyAsync(s.plan + kPlanCap, &s.move_cap, 4, hipMemcpyHostToDevice, s.st));

@@ -139,7 +139,7 @@ __host__ __device__ inline bool plan_incremental(const uint32_t* plan) {
 struct MoveList {
     uint32_t *key, *slot, *pos;  // appended by the transfer: sub-list r at [r * cap, r * cap + count_r)
     uint32_t* lb;                // per entry: its place among the snapshot
-    uint32_t* tcount;            // [2][tiles + 1]: removed positions, insertion places per snapshot tile
+    uint32_t* tcount;            // [2][tiles + 1]: removed positions, insertion places per snapshot tile (zero between steps)
     uint32_t* tnext;             // [2][tiles + 1]: bucket cursors
     uint32_t *del_b, *ins_b;     // removed positions, entry indices: bucketed by tile (capacity each)
 };

@@ -76,7 +76,8 @@ __global__ void k_inc_set(uint32_t* plan, uint32_t snap_ok) {
 
 // Each list entry's place among the snapshot, by binary search (a chain
 // of dependent loads, the entries all in flight at once), and the tile
-// counts of its old position and of its place (cleared by k_inc_clear).
+// counts of its old position and of its place (zero on entry: the merge
+// clears them after use).
 __global__ void __launch_bounds__(kIncWG) k_inc_locate(const uint32_t* __restrict__ plan, MoveList L,
                                                        const uint32_t* __restrict__ snap_k,
                                                        const uint32_t* __restrict__ snap_p, int64_t n,
@@ -94,13 +95,6 @@ __global__ void __launch_bounds__(kIncWG) k_inc_locate(const uint32_t* __restric
         atomicAdd(&L.tcount[L.pos[j] / (uint32_t)kIncTile], 1u);
         atomicAdd(&L.tcount[tiles + 1u + lo / (uint32_t)kIncTile], 1u);
     }
-}
-
-__global__ void __launch_bounds__(kIncWG) k_inc_clear(const uint32_t* __restrict__ plan, MoveList L,
-                                                      uint32_t tiles) {
-    if (!plan_incremental(plan)) return;
-    for (uint32_t t = blockIdx.x * kIncWG + threadIdx.x; t < 2u * (tiles + 1u); t += gridDim.x * kIncWG)
-        L.tcount[t] = 0u;
 }
 
 // Bucket offsets: exclusive scans of the two count arrays (tiles + 1
@@ -145,7 +139,9 @@ __global__ void __launch_bounds__(kIncWG) k_inc_bucket(const uint32_t* __restric
 }
 
 // One snapshot tile [i0, i1) per workgroup (the last one, past n when n is
-// a multiple of the tile, takes the insertions at n).  After the bucketing
+// a multiple of the tile, takes the insertions at n); it also zeroes the
+// tile's counts, also on a step the update does not run (after an overflow
+// the counts are left over).  After the bucketing
 // the cursors sit at the end of each bucket, so a bucket starts at
 // cursor - count.  Removed positions are distinct and list entries have
 // distinct (key, slot), so ranking by counting gives each its own place.
@@ -153,11 +149,17 @@ __global__ void __launch_bounds__(kIncWG) k_inc_merge(const uint32_t* __restrict
                                                       const uint32_t* __restrict__ snap_k,
                                                       const uint32_t* __restrict__ snap_p, int64_t n, uint32_t tiles,
                                                       uint32_t* __restrict__ out_k, uint32_t* __restrict__ out_p) {
-    if (!plan_incremental(plan)) return;
+    const uint32_t t = blockIdx.x;
+    if (!plan_incremental(plan)) {  // an overflow after the counting: clear them for the next step
+        if (threadIdx.x == 0) {
+            L.tcount[t] = 0u;
+            L.tcount[tiles + 1u + t] = 0u;
+        }
+        return;
+    }
     __shared__ uint32_t dr[kIncStage], sdel[kIncStage];              // removed positions: as bucketed, ascending
     __shared__ uint32_t ir[kIncStage], ik[kIncStage], is[kIncStage];  // insertions: place, key, slot
     __shared__ uint32_t slb[kIncStage];                                // insertion places, ascending
-    const uint32_t t = blockIdx.x;
     const int64_t i0 = (int64_t)t * kIncTile;
     const int64_t i1 = i0 + kIncTile < n ? i0 + kIncTile : n;
     const uint32_t dc = L.tcount[t], rc = L.tcount[tiles + 1u + t];  // <= kIncStage (else overflow)
@@ -169,7 +171,11 @@ __global__ void __launch_bounds__(kIncWG) k_inc_merge(const uint32_t* __restrict
         ik[q] = L.key[j];
         is[q] = L.slot[j];
     }
-    __syncthreads();
+    __syncthreads();  // every thread has read the counts
+    if (threadIdx.x == 0) {  // zero for the next step's counting
+        L.tcount[t] = 0u;
+        L.tcount[tiles + 1u + t] = 0u;
+    }
     for (uint32_t q = threadIdx.x; q < dc; q += kIncWG) {
         const uint32_t v = dr[q];
         uint32_t r = 0;
@@ -211,9 +217,7 @@ void launch_inc_set(uint32_t* plan, uint32_t snap_ok, hipStream_t st) {
 void launch_inc_prepare(uint32_t* plan, const MoveList& L, const uint32_t* snap_k, const uint32_t* snap_p, int64_t n,
                         hipStream_t st) {
     const uint32_t tiles = inc_tiles(n);
-    const unsigned clear_blocks = std::min<unsigned>(1024u, (2u * (tiles + 1u) + kIncWG - 1) / kIncWG);
-    hipLaunchKernelGGL(k_inc_clear, dim3(clear_blocks), dim3(kIncWG), 0, st, plan, L, tiles);
-    // 16 blocks per sub-list
+    // 16 workgroups per sub-list
     hipLaunchKernelGGL(k_inc_locate, dim3(16 * kMoveLists), dim3(kIncWG), 0, st, plan, L, snap_k, snap_p, n, tiles);
     hipLaunchKernelGGL(k_inc_offsets, dim3(2), dim3(kScanWG), 0, st, plan, L, tiles);
 }
