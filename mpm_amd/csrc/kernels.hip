@@ -897,21 +897,33 @@ void launch_p2g(const SceneDev& S, const ParticlesDev& P, const ItemsDev& I, int
 // kMoveLists), one counter atomic per wave, then each moved lane writes
 // (new key, slot, old sorted position) at its rank.  Past the sub-list's
 // capacity nothing is stored and the overflow word sends the next step to
-// the full sort.
-__device__ __forceinline__ void record_move(const ItemsDev& I, bool moved, uint32_t key, uint32_t slot,
-                                            uint32_t pos) {
-    const uint64_t m = __ballot(moved);
-    if (m == 0ull) return;
+// the full sort.  In two halves, so that a kernel can issue the atomic
+// early and use its result late (the wave need not wait for it).
+struct MoveTicket {
+    uint64_t mask;  // the wave's moved lanes
+    uint32_t base;  // the leader lane's atomic result
+    uint32_t list;  // sub-list index
+};
+__device__ __forceinline__ MoveTicket move_begin(const ItemsDev& I, bool moved) {
+    MoveTicket t;
+    t.mask = __ballot(moved);
+    t.base = 0;
+    t.list = (blockIdx.x * (blockDim.x >> 6) + (threadIdx.x >> 6)) & (uint32_t)(kMoveLists - 1);
+    if (t.mask == 0ull) return t;
     const int lane = threadIdx.x & 63;
-    const int leader = __ffsll((unsigned long long)m) - 1;
-    const uint32_t r = (blockIdx.x * (blockDim.x >> 6) + (threadIdx.x >> 6)) & (uint32_t)(kMoveLists - 1);
-    uint32_t base = 0;
-    if (lane == leader) base = atomicAdd(&I.plan[kPlanLists + r * kMoveCountStride], (uint32_t)__popcll(m));
-    base = __shfl(base, leader, 64);
-    if (moved) {
-        const uint32_t idx = base + (uint32_t)__popcll(m & ((1ull << lane) - 1ull));
+    if (lane == __ffsll((unsigned long long)t.mask) - 1)
+        t.base = atomicAdd(&I.plan[kPlanLists + t.list * kMoveCountStride], (uint32_t)__popcll(t.mask));
+    return t;
+}
+__device__ __forceinline__ void move_end(const ItemsDev& I, const MoveTicket& t, uint32_t key, uint32_t slot,
+                                         uint32_t pos) {
+    if (t.mask == 0ull) return;
+    const int lane = threadIdx.x & 63;
+    const uint32_t base = __shfl(t.base, __ffsll((unsigned long long)t.mask) - 1, 64);
+    if ((t.mask >> lane) & 1ull) {
+        const uint32_t idx = base + (uint32_t)__popcll(t.mask & ((1ull << lane) - 1ull));
         if (idx < I.move_cap) {
-            const size_t e = (size_t)r * I.move_cap + idx;
+            const size_t e = (size_t)t.list * I.move_cap + idx;
             I.moves->key[e] = key;
             I.moves->slot[e] = slot;
             I.moves->pos[e] = pos;
@@ -919,6 +931,10 @@ __device__ __forceinline__ void record_move(const ItemsDev& I, bool moved, uint3
             I.plan[kPlanOverflow] = 1u;
         }
     }
+}
+__device__ __forceinline__ void record_move(const ItemsDev& I, bool moved, uint32_t key, uint32_t slot,
+                                            uint32_t pos) {
+    move_end(I, move_begin(I, moved), key, slot, pos);
 }
 
 // ------------------------------------------------------ XCD-aware items
@@ -1119,6 +1135,8 @@ __global__ void __launch_bounds__(WG) __attribute__((amdgpu_waves_per_eu(G2PWave
             // the key it was sorted by, for the move list: loaded with the
             // slot, long before the comparison needs it
             const uint32_t kold = I.plan ? I.sorted_keys[i] : 0u;
+            uint32_t knew = 0;
+            MoveTicket ticket{};
             float* const pst = pstage + (threadIdx.x >> 6) * (PW * 64);  // the wave's region
             if constexpr (PSTAGE) {
 #pragma unroll 1
@@ -1279,6 +1297,9 @@ __global__ void __launch_bounds__(WG) __attribute__((amdgpu_waves_per_eu(G2PWave
                 lam0 = q[11 * 64];
                 V0 = q[12 * 64];
             }
+            // the material before the move list's atomic: a wait for it
+            // would wait for the atomic too (memory returns count in order)
+            const MaterialDev mat = P.mats[P.mat[p]];
             // Velocity, B, position and next sort key first: they depend on
             // the gather only, and storing them before the F update frees
             // their registers for the SVD.
@@ -1294,10 +1315,11 @@ __global__ void __launch_bounds__(WG) __attribute__((amdgpu_waves_per_eu(G2PWave
                 P.x[N + p] = y1;
                 P.x[2 * N + p] = y2;
                 if (next_keys) {
-                    const uint32_t knew = koff + cell_key(S, y0, y1, y2, Stencil<INTERP>::SHIFT, Stencil<INTERP>::W);
+                    knew = koff + cell_key(S, y0, y1, y2, Stencil<INTERP>::SHIFT, Stencil<INTERP>::W);
                     next_keys[pbase + p] = knew;
-                    if (I.plan)  // uniform: the incremental sort is on
-                        record_move(I, knew != kold, knew, pbase + p, i);
+                    // uniform: the incremental sort is on; the entry is
+                    // written after the F update
+                    if (I.plan) ticket = move_begin(I, knew != kold);
                 }
             }
             if constexpr (!PSTAGE) {
@@ -1307,7 +1329,6 @@ __global__ void __launch_bounds__(WG) __attribute__((amdgpu_waves_per_eu(G2PWave
                 lam0 = P.lam0[p];
                 V0 = P.V0[p];
             }
-            const MaterialDev mat = P.mats[P.mat[p]];
             float Fn[3][3], E[6], J, tau[6];
             trial_deformation(Fe, grad, S.dt, Fn);
             const bool elastic = mat.model == MPM_MODEL_FIXED_COROTATED && elastic_trial(Fn, mat, E, J);
@@ -1331,6 +1352,7 @@ __global__ void __launch_bounds__(WG) __attribute__((amdgpu_waves_per_eu(G2PWave
             }
 #pragma unroll
             for (int k = 0; k < 6; ++k) P.tau[(int64_t)k * N + p] = tau[k];
+            if (I.plan && advect && next_keys) move_end(I, ticket, knew, pbase + p, i);
         }
         __syncthreads();
     }
@@ -1695,10 +1717,15 @@ k_g2p2g(SceneDev S0, ParticlesDev P0, ItemsDev I, const SceneEntry* __restrict__
                 P.x[2 * N + p] = xn[2];
                 knew = koff + cell_key(S, xn[0], xn[1], xn[2], Stencil<INTERP>::SHIFT, Stencil<INTERP>::W);
                 next_keys[pbase + p] = knew;
-                if (I.plan) record_move(I, knew != kold, knew, praw, i);
                 // updateDeformationGradient and the stress of the new state
+                // (the material loaded before the wait below, and the move
+                // list's atomic issued after it: its result is used only
+                // after the F update)
+                const MaterialDev mat = P.mats[P.mat[p]];
                 float Fe[3][3];
                 asm volatile("s_waitcnt vmcnt(0)" ::: "memory");  // staged words have landed
+                MoveTicket ticket{};
+                if (I.plan) ticket = move_begin(I, knew != kold);
                 float* const q = pst + lane;
 #pragma unroll
                 for (int f = 0; f < 9; ++f) Fe[f / 3][f % 3] = q[f * 64];
@@ -1709,7 +1736,6 @@ k_g2p2g(SceneDev S0, ParticlesDev P0, ItemsDev I, const SceneEntry* __restrict__
                 for (int k = 0; k < 3; ++k) q[k * 64] = xn[k];
 #pragma unroll
                 for (int f = 0; f < 9; ++f) q[(3 + f) * 64] = Bm[f / 3][f % 3];
-                const MaterialDev mat = P.mats[P.mat[p]];
                 float Fn[3][3], E[6], J, tau[6];
                 trial_deformation(Fe, grad, S.dt, Fn);
                 const bool elastic = mat.model == MPM_MODEL_FIXED_COROTATED && elastic_trial(Fn, mat, E, J);
@@ -1727,6 +1753,7 @@ k_g2p2g(SceneDev S0, ParticlesDev P0, ItemsDev I, const SceneEntry* __restrict__
                     store_m3(P.Fp, N, p, Fp);
                     P.Jp[p] = Jn;
                 }
+                if (I.plan) move_end(I, ticket, knew, praw, i);
 #pragma unroll
                 for (int k = 0; k < 3; ++k) {
                     vn[k] = v[k];
