@@ -77,6 +77,8 @@ struct Batch {
     SceneEntry* tab = nullptr;
     uint32_t* key_offsets = nullptr;
     int64_t* block_start = nullptr;
+    uint32_t* range_start = nullptr;  // tab_cap + 1 words: each member's first global slot, then ntot
+    std::vector<uint32_t> range_host;
     int tab_cap = 0;
     uint32_t *kin = nullptr, *iota = nullptr, *ka = nullptr, *va = nullptr, *kb = nullptr, *vb = nullptr;
     uint32_t *flags = nullptr, *items = nullptr, *item_scene = nullptr, *hist = nullptr, *scan_temp = nullptr;
@@ -336,10 +338,12 @@ int relayout(Batch& b) {
         bfree(b, *(void**)&b.tab);
         bfree(b, *(void**)&b.key_offsets);
         bfree(b, *(void**)&b.block_start);
+        bfree(b, *(void**)&b.range_start);
         const int c = std::max(ns, std::max(16, b.tab_cap * 2));
         if ((rc = balloc(b, (void**)&b.tab, (size_t)c * sizeof(SceneEntry)))) return rc;
         if ((rc = balloc(b, (void**)&b.key_offsets, (size_t)c * 4))) return rc;
         if ((rc = balloc(b, (void**)&b.block_start, (size_t)(c + 1) * 8))) return rc;
+        if ((rc = balloc(b, (void**)&b.range_start, (size_t)(c + 1) * 4))) return rc;
         b.tab_cap = c;
     }
     if (ns > 0) {
@@ -347,6 +351,12 @@ int relayout(Batch& b) {
         HIP_TRY(hipMemcpyAsync(b.key_offsets, b.koff_host.data(), (size_t)ns * 4, hipMemcpyHostToDevice, b.st));
     }
     HIP_TRY(hipMemcpyAsync(b.block_start, b.bstart_host.data(), (size_t)(ns + 1) * 8, hipMemcpyHostToDevice, b.st));
+    b.range_host.resize((size_t)ns + 1);
+    for (int i = 0; i < ns; ++i) b.range_host[(size_t)i] = (uint32_t)b.tab_host[(size_t)i].pstart;
+    b.range_host[(size_t)ns] = (uint32_t)n;
+    HIP_TRY(hipMemcpyAsync(b.range_start, b.range_host.data(), (size_t)(ns + 1) * 4, hipMemcpyHostToDevice, b.st));
+    b.moves.range_start = b.range_start;
+    b.moves.nranges = (uint32_t)ns;
     launch_iota(b.iota, n, b.st);
     b.I = ItemsDev{};
     b.I.start = b.items;

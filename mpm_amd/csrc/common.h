@@ -142,6 +142,11 @@ struct MoveList {
     uint32_t* tcount;            // [2][tiles + 1]: removed positions, insertion places per snapshot tile (zero between steps)
     uint32_t* tnext;             // [2][tiles + 1]: bucket cursors
     uint32_t *del_b, *ins_b;     // removed positions, entry indices: bucketed by tile (capacity each)
+    // a batch (else nranges 0): range_start[0..nranges] the first global slot
+    // of each member and the total; a member's particles also fill exactly
+    // that range of sorted positions (its keys are one range of the key space)
+    const uint32_t* range_start;
+    uint32_t nranges;
 };
 
 // Work items of the block-tiled kernels: a contiguous run of sorted

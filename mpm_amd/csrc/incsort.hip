@@ -75,9 +75,9 @@ __global__ void k_inc_set(uint32_t* plan, uint32_t snap_ok) {
          q_ += step_, j = base_ + q_)
 
 // Each list entry's place among the snapshot, by binary search (a chain
-// of dependent loads, the entries all in flight at once), and the tile
-// counts of its old position and of its place (zero on entry: the merge
-// clears them after use).
+// of dependent loads, the entries all in flight at once; in a batch within
+// the entry's member), and the tile counts of its old position and of its
+// place (zero on entry: the merge clears them after use).
 __global__ void __launch_bounds__(kIncWG) k_inc_locate(const uint32_t* __restrict__ plan, MoveList L,
                                                        const uint32_t* __restrict__ snap_k,
                                                        const uint32_t* __restrict__ snap_p, int64_t n,
@@ -86,6 +86,16 @@ __global__ void __launch_bounds__(kIncWG) k_inc_locate(const uint32_t* __restric
     FOR_LIST_ENTRIES(plan, j) {
         const uint32_t k = L.key[j], s = L.slot[j];
         uint32_t lo = 0, hi = (uint32_t)n;  // the place lies in [lo, hi]
+        if (L.nranges) {  // a batch: within its member's range (a short search in a cached table first)
+            uint32_t a = 0, b = L.nranges;
+            while (b - a > 1) {
+                const uint32_t mid = (a + b) >> 1;
+                if (L.range_start[mid] <= s) a = mid;
+                else b = mid;
+            }
+            lo = L.range_start[a];
+            hi = L.range_start[a + 1];
+        }
         while (lo < hi) {
             const uint32_t mid = lo + ((hi - lo) >> 1);
             if (entry_less(snap_k[mid], snap_p[mid], k, s)) lo = mid + 1;
