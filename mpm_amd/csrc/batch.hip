@@ -64,6 +64,10 @@ struct Batch {
     int device = 0;
     int cus = 1;
     hipStream_t st = nullptr;
+    // eager steps clear the grids on aux while the sort runs on st (they
+    // share no buffer); P2G waits for the join event
+    hipStream_t aux = nullptr;
+    hipEvent_t fork = nullptr, join = nullptr;
     std::vector<Solver*> members;
     std::vector<uint64_t> synced_version;  // member version whose keys are in kin
     // layout
@@ -443,13 +447,23 @@ int sync_members(Batch& b) {
 // full_sort: the step is known to sort in full (the snapshot was just
 // invalidated), else the radix launches use the light plan of a sort that
 // mostly returns at once.
-void enqueue_step(Batch& b, bool fused, bool scattered, bool full_sort) {
+// overlap: the grid clears run on the second stream (eager steps; a
+// captured step keeps them on the batch stream).
+void enqueue_step(Batch& b, bool fused, bool scattered, bool full_sort, bool overlap) {
     const int ns = (int)b.members.size();
+    const bool forked = overlap && b.aux && hipEventRecord(b.fork, b.st) == hipSuccess &&
+                        hipStreamWaitEvent(b.aux, b.fork, 0) == hipSuccess;
+    hipStream_t cst = forked ? b.aux : b.st;
     if (!fused) {
-        launch_clear_batch(b.tab, b.block_start, ns, b.total_blocks, b.fold ? GF_FOLDED : GF_PRIMARY, b.st);
+        launch_clear_batch(b.tab, b.block_start, ns, b.total_blocks, b.fold ? GF_FOLDED : GF_PRIMARY, cst);
     } else {
-        if (!scattered) launch_clear_batch(b.tab, b.block_start, ns, b.total_blocks, GF_FOLDED, b.st, 1);
-        launch_clear_batch(b.tab, b.block_start, ns, b.total_blocks, GF_FOLDED, b.st, 2);
+        if (!scattered) launch_clear_batch(b.tab, b.block_start, ns, b.total_blocks, GF_FOLDED, cst, 1);
+        launch_clear_batch(b.tab, b.block_start, ns, b.total_blocks, GF_FOLDED, cst, 2);
+    }
+    bool joined = !forked;
+    if (forked && hipEventRecord(b.join, b.aux) != hipSuccess) {
+        (void)hipStreamSynchronize(b.aux);
+        joined = true;
     }
     // the sort: the update of the last step's order or the full radix sort,
     // as the plan words decide on the device (a captured step holds both)
@@ -472,6 +486,7 @@ void enqueue_step(Batch& b, bool fused, bool scattered, bool full_sort) {
     const SceneDev S0{};
     const ParticlesDev P0{};
     const int wgs = effective_wgs(b);
+    if (!joined && hipStreamWaitEvent(b.st, b.join, 0) != hipSuccess) (void)hipStreamSynchronize(b.aux);
     if (fused) {
         // P2G only for members whose grid is not pre-scattered (the kernel
         // skips the others' items), then G2P fused with the next P2G
@@ -568,7 +583,7 @@ int reorder_members(Batch& b) {
 int capture_step(Batch& b, bool fused) {
     drop_graph(b);
     HIP_TRY(hipStreamBeginCapture(b.st, hipStreamCaptureModeThreadLocal));
-    enqueue_step(b, fused, fused, false);
+    enqueue_step(b, fused, fused, false, false);
     hipError_t e = hipStreamEndCapture(b.st, &b.graph);
     if (e != hipSuccess) return fail(MPM_ERR_HIP, std::string("batch graph capture: ") + hipGetErrorString(e));
     HIP_TRY(hipGraphInstantiate(&b.gexec, b.graph, nullptr, nullptr, 0));
@@ -607,7 +622,7 @@ int run_steps(Batch& b, int n, bool graph) {
             b.snap_ok = true;
         }
         if (graph && !first_scatter) HIP_TRY(hipGraphLaunch(b.gexec, b.st));
-        else enqueue_step(b, fused, scattered, full_sort);
+        else enqueue_step(b, fused, scattered, full_sort, true);
         if (e0 && e1) {
             HIP_TRY(hipEventRecord(e1, b.st));
             b.pending.push_back({e0, e1});
@@ -647,6 +662,12 @@ void batch_destroy(Batch* b) {
     for (auto& pr : b->pending) b->event_pool.push_back(pr.first), b->event_pool.push_back(pr.second);
     for (auto e : b->event_pool) (void)hipEventDestroy(e);
     for (void* p : b->allocs) (void)devcache::release(p);  // the stream is synchronised above
+    if (b->aux) {
+        (void)hipStreamSynchronize(b->aux);
+        (void)hipStreamDestroy(b->aux);
+    }
+    if (b->fork) (void)hipEventDestroy(b->fork);
+    if (b->join) (void)hipEventDestroy(b->join);
     if (b->st) (void)hipStreamDestroy(b->st);
     delete b;
 }
@@ -725,6 +746,9 @@ MPM_API int mpm_batch_create(int32_t interp, int32_t transfer, void** out_batch)
     hipDeviceProp_t prop;
     if (e == hipSuccess) e = hipGetDeviceProperties(&prop, b->device);
     if (e == hipSuccess) e = hipStreamCreateWithFlags(&b->st, hipStreamNonBlocking);
+    if (e == hipSuccess) e = hipStreamCreateWithFlags(&b->aux, hipStreamNonBlocking);
+    if (e == hipSuccess) e = hipEventCreateWithFlags(&b->fork, hipEventDisableTiming);
+    if (e == hipSuccess) e = hipEventCreateWithFlags(&b->join, hipEventDisableTiming);
     if (e == hipSuccess) b->cus = prop.multiProcessorCount;
     int rc = MPM_OK;
     if (e != hipSuccess) rc = fail(MPM_ERR_HIP, std::string("batch create: ") + hipGetErrorString(e));
