@@ -11,9 +11,8 @@
 //     list (new key, slot, old sorted position; kMoveLists sub-lists);
 //   * here every list entry finds its place among the snapshot (lb: the
 //     number of snapshot entries before it in (key, slot) order) by a binary
-//     search, and the removed
-//     positions and insertion places are bucketed by snapshot tile (counts,
-//     one scan, then an append per entry);
+//     search, and the removed positions and insertion places are bucketed
+//     by snapshot tile (counts, one scan, then an append per entry);
 //   * one pass over the snapshot tiles finishes the job: each workgroup
 //     ranks its tile's few removals and insertions in LDS and writes every
 //     unmoved entry at i - (removed before i) + (inserted at or before i)
@@ -23,14 +22,12 @@
 // device-side plan words pick this path or the full radix sort each step
 // (the other path's launches return at once): the full sort runs when the
 // list overflowed, a tile drew more entries than its merge stages, or the
-// snapshot is not the previous step's (first step, state written, store
-// reordered).  The step's item count resets the words afterwards
+// snapshot is not the previous step's (first step, state written, a
+// single scene's store reordered, a batch's layout changed).  The step's item count resets the words afterwards
 // (launch_items with plan).  Launch order within a step: launch_inc_prepare,
 // the radix sort (which reads the decision), launch_inc_merge.
 #include <hip/hip_runtime.h>
 #include <stdint.h>
-
-#include <algorithm>
 
 #include "blockscan.h"
 #include "launch.h"
