@@ -148,15 +148,21 @@ def test_served_stream_config_sizes():
     assert rep.completed == len(reqs)
     recs = {r.rid: r for r in rep.records}
     for r in reqs:
+        # hundreds to thousands of steps amplify the float-atomic order
+        # chaotically: the envelope is the largest difference among three
+        # runs of the request alone, and the served result must lie as close
+        # to one of them
         runs = []
-        for _ in range(2):
+        for _ in range(3):
             alone = Solver(r.desc, r.particles)
             alone.step(r.steps)
             runs.append(alone.read_particles())
             alone.close()
         pm = recs[r.rid].result
-        e = _many_step(pm, runs[0], r.desc.stride)
-        spread = _many_step(runs[1], runs[0], r.desc.stride)
+        errs = [_many_step(pm, a, r.desc.stride) for a in runs]
+        e = {f: min(x[f] for x in errs) for f in errs[0]}
+        pairs = [_many_step(runs[i], runs[j], r.desc.stride) for i in range(3) for j in range(i + 1, 3)]
+        spread = {f: max(x[f] for x in pairs) for f in pairs[0]}
         e.update({"spread_pos/dx": spread["pos/dx"], "spread_Fe": spread["Fe"]})
         report(f"[served] rid={r.rid} N={r.particles.count} res={r.desc.resolution[0]} steps={r.steps} vs alone", e)
         assert e["mass"] == 0.0 and np.all(np.isfinite(pm.position)) and np.all(np.isfinite(pm.Fe))
