@@ -932,10 +932,6 @@ __device__ __forceinline__ void move_end(const ItemsDev& I, const MoveTicket& t,
         }
     }
 }
-__device__ __forceinline__ void record_move(const ItemsDev& I, bool moved, uint32_t key, uint32_t slot,
-                                            uint32_t pos) {
-    move_end(I, move_begin(I, moved), key, slot, pos);
-}
 
 // ------------------------------------------------------ XCD-aware items
 // Workgroups are dispatched to the 8 XCDs in turn (workgroup b on XCD b % 8),
